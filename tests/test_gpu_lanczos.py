@@ -1,0 +1,83 @@
+"""Parity of the HIP thick-restart Lanczos with the FP64 oracle
+(solver/lanczos.py:239-284 restated in oracle/lanczos.py)."""
+import numpy as np
+import pytest
+import scipy.sparse
+
+from oracle import lanczos as olz, maxcut as omc
+from usbs_amd import problems
+from usbs_amd.lanczos import eigsh_smallest
+
+pytestmark = pytest.mark.gpu
+
+
+def _maxcut_operator(W, seed):
+    C = omc.laplacian_cost(W)
+    n = C.shape[0]
+    A_data, A_indices, _, _ = omc.get_all_problem_data(n)
+    y = np.random.default_rng(seed).standard_normal(n)
+    q0 = problems.unit_start_vector(n, seed + 1)
+    return (-C).tocsr(), A_data, A_indices, y, q0
+
+
+@pytest.mark.parametrize("exec_mode", ["perop", "persistent"])
+def test_lanczos_fp64_matches_oracle(gpu_ctx, exec_mode):
+    W = problems.erdos_renyi(300, 0.06, seed=3)
+    negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 7)
+    n = negC.shape[0]
+    print("case: ER(300,0.06) nd=10 m=32 exec=%s" % exec_mode, flush=True)
+    ev_o, V_o, st = olz.eigsh_smallest(n, negC, A_data, A_indices, y, q0, 10, 32, 10, 1e-7,
+                                       return_state=True)
+    ev_g, V_g, info = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 10, 32, 10, 1e-7,
+                                     exec_mode=exec_mode, return_info=True, context=gpu_ctx)
+    assert info["restarts"] == st.restart
+    np.testing.assert_allclose(ev_g, ev_o, rtol=1e-9, atol=1e-9 * np.abs(ev_o).max())
+    # eigenvectors agree up to sign
+    for j in range(10):
+        s = np.sign(V_g[:, j] @ V_o[:, j])
+        np.testing.assert_allclose(s * V_g[:, j], V_o[:, j], atol=1e-6)
+    # against a dense eigensolver: the converged head
+    dense = np.linalg.eigvalsh((negC + scipy.sparse.diags(y)).toarray())
+    np.testing.assert_allclose(ev_g[:3], dense[:3], rtol=1e-9, atol=1e-8)
+    # orthonormal basis
+    np.testing.assert_allclose(V_g.T @ V_g, np.eye(10), atol=1e-10)
+
+
+def test_persistent_bitwise_equals_perop(gpu_ctx):
+    W = problems.torus(20, 30, seed=1)
+    negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 11)
+    n = negC.shape[0]
+    a = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 10, 32, 10, 1e-7, exec_mode="perop",
+                       context=gpu_ctx)
+    b = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 10, 32, 10, 1e-7,
+                       exec_mode="persistent", context=gpu_ctx)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("exec_mode", ["perop", "persistent"])
+def test_lanczos_mixed_precision_error_stated(gpu_ctx, exec_mode):
+    """FP32 basis/operator, FP64 accumulation + Ritz: eigenvalues within
+    2e-6 relative of the FP64 oracle (stated bound for the mixed form)."""
+    W = problems.torus(40, 50, seed=2)
+    negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 5)
+    n = negC.shape[0]
+    dense = np.linalg.eigvalsh((negC + scipy.sparse.diags(y)).toarray())
+    ev_g, V_g = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 10, 32, 10, 1e-5,
+                               precision="mixed", exec_mode=exec_mode, context=gpu_ctx)
+    scale = np.abs(dense).max()
+    assert np.abs(ev_g[0] - dense[0]) <= 2e-6 * scale
+
+
+def test_krylov_breakdown_small_n(gpu_ctx):
+    """n <= 32: the Krylov space is exhausted inside the first restart, which
+    forces the random-restart branch (lanczos.py:107-117)."""
+    W = problems.erdos_renyi(24, 0.3, seed=4)
+    negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 9)
+    n = negC.shape[0]
+    ev_o, _ = olz.eigsh_smallest(n, negC, A_data, A_indices, y, q0, 4, n, 3, 1e-7)
+    ev_g, V_g, info = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 4, n, 3, 1e-7,
+                                     return_info=True, context=gpu_ctx)
+    dense = np.linalg.eigvalsh((negC + scipy.sparse.diags(y)).toarray())
+    np.testing.assert_allclose(ev_g, dense[:4], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(ev_o, dense[:4], rtol=1e-9, atol=1e-9)
+    assert info["breakdowns"] >= 1

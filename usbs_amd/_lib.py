@@ -1,0 +1,120 @@
+"""ctypes binding of libusbs_hip.so (the C ABI declared in include/usbs_hip.h).
+
+The library is built in-tree by ``python -m usbs_amd.build``.  There is no
+CPU fallback: if the shared library is missing or fails to load, importing
+anything that needs it raises ``RuntimeError``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libusbs_hip.so")
+
+USBS_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of device memory",
+          -4: "non-finite input", -5: "bad call order", -6: "device wait expired"}
+
+PRECISION = {"fp64": 0, "mixed": 1}
+EXEC_MODE = {"perop": 0, "persistent": 1}
+
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class USBSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("usbs-hip error %d (%s): %s" % (code, ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+def _declare(lib):
+    lib.usbs_version.restype = ctypes.c_char_p
+    lib.usbs_last_error.restype = ctypes.c_char_p
+    lib.usbs_last_error.argtypes = [_vp]
+    lib.usbs_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int]
+    lib.usbs_destroy.argtypes = [_vp]
+    lib.usbs_destroy.restype = None
+    lib.usbs_set_problem.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _ip, _ip,
+                                     _dp, ctypes.c_int64, _ip, _dp, _dp, _dp, _dp]
+    lib.usbs_lanczos_smallest.argtypes = [_vp, ctypes.c_double, _dp, _dp, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
+    for name, args in EXTRA_SIGNATURES.items():
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = args
+
+
+# Filled by modules that bind further entry points (kept here so one loader
+# declares every symbol before first use).
+EXTRA_SIGNATURES = {}
+
+
+def lib():
+    """Load (once) and return the shared library; raise if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libusbs_hip.so not built (run `python -m usbs_amd.build`); "
+                               "there is no CPU fallback for the HIP backend")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def as_f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def as_i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+class Context:
+    """An owned device context (one HIP stream + resident problem data)."""
+
+    def __init__(self, device=0):
+        self._lib = lib()
+        h = _vp()
+        rc = self._lib.usbs_create(ctypes.byref(h), int(device))
+        if rc != USBS_OK:
+            raise USBSError(rc, "usbs_create(device=%d) failed" % device)
+        self.h = h
+        self.n = None
+        self.m = None
+
+    def check(self, rc):
+        if rc != USBS_OK:
+            raise USBSError(rc, self._lib.usbs_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.usbs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_problem(self, n, m, C_rows, C_cols, C_vals, A_indices, A_data, b, b_ineq_mask, SCALE_A):
+        C_rows, C_cols, C_vals = as_i32(C_rows), as_i32(C_cols), as_f64(C_vals)
+        A_indices = as_i32(np.asarray(A_indices).reshape(-1, 3))
+        A_data, b, mask, sa = as_f64(A_data), as_f64(b), as_f64(b_ineq_mask), as_f64(SCALE_A)
+        self.check(self._lib.usbs_set_problem(self.h, n, m, C_vals.size, iptr(C_rows), iptr(C_cols),
+                                              dptr(C_vals), A_data.size, iptr(A_indices),
+                                              dptr(A_data), dptr(b), dptr(mask), dptr(sa)))
+        self.n, self.m = int(n), int(m)

@@ -1,0 +1,252 @@
+// Small dense FP64 linear algebra for one workgroup, operands in LDS.
+//
+// Used by the Ritz solve (m <= 64), the IPM subproblems (k <= 12,
+// svec dimension d = k(k+1)/2 <= 78), the bundle eigendecomposition and the
+// Nystrom core.  All matrices are row-major with leading dimension = their
+// column count.  Every routine is called by all threads of the block and
+// ends with the block synchronised.
+#pragma once
+#include "dev_common.hpp"
+
+namespace usbs {
+
+// ---------------------------------------------------------------------------
+// Symmetric eigendecomposition by parallel cyclic Jacobi (round-robin
+// ordering of disjoint rotation pairs).  A (m x m) is destroyed.  On exit
+// w[0..m) holds the eigenvalues in ascending order and column j of V the
+// eigenvector of w[j].  tmp: m*m doubles; coef: 2*m doubles; part: m ints.
+// Returns the number of sweeps used.
+__device__ int dense_eigh(double* A, int m, double* w, double* V, double* tmp, double* coef,
+                          int* part, double* red) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int idx = t; idx < m * m; idx += nt) V[idx] = (idx / m == idx % m) ? 1.0 : 0.0;
+  __syncthreads();
+  const int M = m + (m & 1);
+  int sweep = 0;
+  for (; sweep < 60 && m > 1; ++sweep) {
+    // convergence test on entry to the sweep
+    double off = 0.0, dg = 0.0;
+    for (int idx = t; idx < m * m; idx += nt) {
+      const int i = idx / m, j = idx % m;
+      const double v = A[idx] * A[idx];
+      if (i == j) dg += v; else off += v;
+    }
+    off = block_sum1(off, red);
+    dg = block_sum1(dg, red);
+    if (off <= 1e-34 * dg || off == 0.0 || !(off == off)) break;
+    for (int r = 0; r < M - 1; ++r) {
+      for (int kp = t; kp < M / 2; kp += nt) {
+        int pa = (kp == 0) ? 0 : 1 + ((kp - 1 + r) % (M - 1));
+        int pb = 1 + ((M - 2 - kp + r) % (M - 1));
+        int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
+        if (q >= m) {
+          if (p < m) { part[p] = p; coef[2 * p] = 1.0; coef[2 * p + 1] = 0.0; }
+          continue;
+        }
+        const double apq = A[p * m + q], app = A[p * m + p], aqq = A[q * m + q];
+        double c = 1.0, s = 0.0;
+        if (fabs(apq) > 1e-300) {
+          const double theta = (aqq - app) / (2.0 * apq);
+          const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          c = 1.0 / sqrt(tt * tt + 1.0);
+          s = tt * c;
+        }
+        part[p] = q; part[q] = p;
+        coef[2 * p] = c; coef[2 * p + 1] = -s;
+        coef[2 * q] = c; coef[2 * q + 1] = s;
+      }
+      __syncthreads();
+      for (int idx = t; idx < m * m; idx += nt) {
+        const int i = idx / m, j = idx % m, pi = part[i], pj = part[j];
+        const double ai = coef[2 * i], bi = coef[2 * i + 1], aj = coef[2 * j], bj = coef[2 * j + 1];
+        double v = ai * aj * A[i * m + j] + ai * bj * A[i * m + pj] + bi * aj * A[pi * m + j] +
+                   bi * bj * A[pi * m + pj];
+        if (pi == j && i != j) v = 0.0;
+        tmp[idx] = v;
+      }
+      // V <- V J, in place per (row, pair)
+      for (int idx = t; idx < m * (M / 2); idx += nt) {
+        const int i = idx / (M / 2), kp = idx % (M / 2);
+        int pa = (kp == 0) ? 0 : 1 + ((kp - 1 + r) % (M - 1));
+        int pb = 1 + ((M - 2 - kp + r) % (M - 1));
+        int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
+        if (q >= m) continue;
+        const double vp = V[i * m + p], vq = V[i * m + q];
+        V[i * m + p] = coef[2 * p] * vp + coef[2 * p + 1] * vq;
+        V[i * m + q] = coef[2 * q] * vq + coef[2 * q + 1] * vp;
+      }
+      __syncthreads();
+      for (int idx = t; idx < m * m; idx += nt) A[idx] = tmp[idx];
+      __syncthreads();
+    }
+  }
+  // sort ascending (stable in index), permute eigenvector columns via tmp
+  for (int i = t; i < m; i += nt) {
+    const double di = A[i * m + i];
+    int rank = 0;
+    for (int j = 0; j < m; ++j) {
+      const double dj = A[j * m + j];
+      rank += (dj < di) || (dj == di && j < i);
+    }
+    part[i] = rank;
+  }
+  __syncthreads();
+  for (int i = t; i < m; i += nt) w[part[i]] = A[i * m + i];
+  for (int idx = t; idx < m * m; idx += nt) {
+    const int r = idx / m, c = idx % m;
+    tmp[r * m + part[c]] = V[idx];
+  }
+  __syncthreads();
+  for (int idx = t; idx < m * m; idx += nt) V[idx] = tmp[idx];
+  __syncthreads();
+  return sweep;
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting (first maximal |pivot|, as LAPACK's idamax), then
+// solves A X = B for nrhs right-hand sides (B: d x nrhs row-major, overwritten
+// by X).  A is destroyed.  A zero pivot yields inf/NaN, like getrf/getrs.
+__device__ void dense_lu_solve(double* A, int d, double* B, int nrhs, double* red, int* ired) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k < d; ++k) {
+    // pivot search (one wave does it; all threads read the result)
+    if (t < 64) {
+      double best = -1.0;
+      int bi = k;
+      for (int i = k + t; i < d; i += 64) {
+        const double v = fabs(A[i * d + k]);
+        if (v > best) { best = v; bi = i; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (t == 0) ired[0] = bi;
+    }
+    __syncthreads();
+    const int piv = ired[0];
+    if (piv != k) {
+      for (int j = t; j < d; j += nt) {
+        const double a = A[k * d + j]; A[k * d + j] = A[piv * d + j]; A[piv * d + j] = a;
+      }
+      for (int j = t; j < nrhs; j += nt) {
+        const double a = B[k * nrhs + j]; B[k * nrhs + j] = B[piv * nrhs + j]; B[piv * nrhs + j] = a;
+      }
+    }
+    __syncthreads();
+    const double akk = A[k * d + k];
+    for (int i = k + 1 + t; i < d; i += nt) A[i * d + k] = A[i * d + k] / akk;
+    __syncthreads();
+    const int rows = d - k - 1, cols = d - k - 1 + nrhs;
+    for (int idx = t; idx < rows * cols; idx += nt) {
+      const int i = k + 1 + idx / cols, jj = idx % cols;
+      const double l = A[i * d + k];
+      if (jj < d - k - 1) {
+        const int j = k + 1 + jj;
+        A[i * d + j] -= l * A[k * d + j];
+      } else {
+        const int j = jj - (d - k - 1);
+        B[i * nrhs + j] -= l * B[k * nrhs + j];
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = d - 1; k >= 0; --k) {
+    for (int j = t; j < nrhs; j += nt) B[k * nrhs + j] /= A[k * d + k];
+    __syncthreads();
+    const int cnt = k * nrhs;
+    for (int idx = t; idx < cnt; idx += nt) {
+      const int i = idx / nrhs, j = idx % nrhs;
+      B[i * nrhs + j] -= A[i * d + k] * B[k * nrhs + j];
+    }
+    __syncthreads();
+  }
+  (void)red;
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky A = L L^T of a symmetric m x m matrix in place (lower triangle
+// holds L, strict upper set to zero).  Returns false (for every thread) if a
+// pivot is not positive or not finite.
+__device__ bool dense_cholesky(double* A, int m, int* flag) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  if (t == 0) flag[0] = 1;
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    if (t == 0) {
+      const double d = A[k * m + k];
+      if (!(d > 0.0) || !(d < INFINITY)) flag[0] = 0;
+      A[k * m + k] = sqrt(d);
+    }
+    __syncthreads();
+    if (!flag[0]) return false;
+    const double lkk = A[k * m + k];
+    for (int i = k + 1 + t; i < m; i += nt) A[i * m + k] /= lkk;
+    __syncthreads();
+    const int r = m - k - 1;
+    for (int idx = t; idx < r * r; idx += nt) {
+      const int i = k + 1 + idx / r, j = k + 1 + idx % r;
+      if (j <= i) A[i * m + j] -= A[i * m + k] * A[j * m + k];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < m * m; idx += nt)
+    if (idx % m > idx / m) A[idx] = 0.0;
+  __syncthreads();
+  return true;
+}
+
+// Per-thread positive-definiteness test of M + s*dM (k x k, row-major, in
+// LDS) by an unpivoted Cholesky in the thread's own scratch (k*k doubles).
+__device__ __forceinline__ bool thread_is_pd(const double* M, const double* dM, double s, int k,
+                                             double* L) {
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j <= i; ++j) L[i * k + j] = M[i * k + j] + s * dM[i * k + j];
+  for (int j = 0; j < k; ++j) {
+    double d = L[j * k + j];
+    for (int p = 0; p < j; ++p) d -= L[j * k + p] * L[j * k + p];
+    if (!(d > 0.0)) return false;
+    d = sqrt(d);
+    L[j * k + j] = d;
+    for (int i = j + 1; i < k; ++i) {
+      double v = L[i * k + j];
+      for (int p = 0; p < j; ++p) v -= L[i * k + p] * L[j * k + p];
+      L[i * k + j] = v / d;
+    }
+  }
+  return true;
+}
+
+// svec index a <-> (b, c), b >= c, in numpy.tril_indices (row-major) order.
+__device__ __forceinline__ void svec_pair(int a, int& b, int& c) {
+  b = (int)((sqrt(8.0 * a + 1.0) - 1.0) * 0.5);
+  while ((b + 1) * (b + 2) / 2 <= a) ++b;
+  while (b * (b + 1) / 2 > a) --b;
+  c = a - b * (b + 1) / 2;
+}
+
+// v = svec(M) for symmetric-ish k x k M (off-diagonal: (M_bc + M_cb)/sqrt 2).
+__device__ __forceinline__ void dense_svec(const double* M, int k, double* v) {
+  const int d = k * (k + 1) / 2;
+  const double u = 0.70710678118654752440;
+  for (int a = threadIdx.x; a < d; a += blockDim.x) {
+    int b, c;
+    svec_pair(a, b, c);
+    v[a] = (b == c) ? M[b * k + c] : (M[b * k + c] * u + M[c * k + b] * u);
+  }
+}
+
+// M = svec^{-1}(v).
+__device__ __forceinline__ void dense_svec_inv(const double* v, int k, double* M) {
+  const int d = k * (k + 1) / 2;
+  const double u = 0.70710678118654752440;
+  for (int a = threadIdx.x; a < d; a += blockDim.x) {
+    int b, c;
+    svec_pair(a, b, c);
+    if (b == c) M[b * k + c] = v[a];
+    else { M[b * k + c] = v[a] * u; M[c * k + b] = v[a] * u; }
+  }
+}
+
+}  // namespace usbs

@@ -1,0 +1,158 @@
+// Shared device-side helpers for the USBS HIP library (gfx950 / CDNA4).
+//
+// Conventions used by every kernel in this library:
+//  * Tall vectors and bases are column-major with an explicit leading
+//    dimension; "T" is the storage type (double or float); all reductions
+//    accumulate in double.
+//  * Rows are split into fixed tiles of `tile` rows (a function of n only).
+//    Every reduction over rows first produces one partial per tile, then sums
+//    the tile partials in tile order.  Results therefore do not depend on the
+//    launch shape, on how many workgroups a persistent launch has, or on which
+//    other problems share the GPU: a solve is bitwise the same alone and in a
+//    batch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define USBS_NT 256          // threads per block of every row-parallel kernel
+#define USBS_NPART 8         // second-level split of the LDS transpose reduction
+#define USBS_DENSE_NT 64     // one wavefront for the small dense FP64 kernels
+
+namespace usbs {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Deterministic block reduction of `c` per-thread accumulators, processed
+// in chunks of USBS_RCH columns.  red: LDS of USBS_RCH*USBS_NT doubles,
+// red2: LDS of USBS_RCH*USBS_NPART doubles.  On return out[j] (LDS or
+// global) holds the block sum of acc[j] for j < c; the summation order is
+// fixed (thread order within a part, then part order).
+#define USBS_RCH 16
+template <int MAXC>
+__device__ __forceinline__ void block_reduce_cols(const double (&acc)[MAXC], int c, double* red,
+                                                  double* red2, double* out) {
+  const int t = threadIdx.x;
+  constexpr int SEG = USBS_NT / USBS_NPART;
+#pragma unroll
+  for (int c0 = 0; c0 < MAXC; c0 += USBS_RCH) {
+    if (c0 < c) {
+      const int cc = (c - c0) < USBS_RCH ? (c - c0) : USBS_RCH;
+#pragma unroll
+      for (int j = 0; j < USBS_RCH; ++j)
+        if (c0 + j < MAXC && j < cc) red[j * USBS_NT + t] = acc[(c0 + j < MAXC) ? c0 + j : 0];
+      __syncthreads();
+      for (int idx = t; idx < cc * USBS_NPART; idx += USBS_NT) {
+        const int j = idx / USBS_NPART, part = idx % USBS_NPART;
+        const double* p = red + j * USBS_NT + part * SEG;
+        double s = 0.0;
+#pragma unroll 8
+        for (int q = 0; q < SEG; ++q) s += p[q];
+        red2[idx] = s;
+      }
+      __syncthreads();
+      for (int j = t; j < cc; j += USBS_NT) {
+        double s = 0.0;
+#pragma unroll
+        for (int part = 0; part < USBS_NPART; ++part) s += red2[j * USBS_NPART + part];
+        out[c0 + j] = s;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Sum of one accumulator across the block, deterministic.
+__device__ __forceinline__ double block_sum1(double v, double* lds /* >= NT/64 doubles */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += lds[w];
+  __syncthreads();
+  return s;
+}
+
+// Sum tile partials part[tile * stride + j] over tiles in order; j < c.
+// Every thread of the block computes the sums for its j (j = t, t+NT, ...)
+// and stores them in out (LDS).
+__device__ __forceinline__ void sum_tile_partials(const double* __restrict__ part, int ntiles,
+                                                  int stride, int c, double* out) {
+  for (int j = threadIdx.x; j < c; j += blockDim.x) {
+    double s = 0.0;
+    for (int tl = 0; tl < ntiles; ++tl) s += part[(size_t)tl * stride + j];
+    out[j] = s;
+  }
+  __syncthreads();
+}
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Standard normal number idx of stream `key`: Box-Muller on two SplitMix64
+// uniforms with counter key*2^32 + 2*idx (+1).
+__device__ __forceinline__ double counter_normal(uint64_t key, uint64_t idx) {
+  const uint64_t base = key << 32;
+  const uint64_t a = splitmix64(base + 2 * idx), b = splitmix64(base + 2 * idx + 1);
+  const double u1 = (double)(a >> 11) * (1.0 / 9007199254740992.0);
+  const double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+  return sqrt(-2.0 * log1p(-u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+// Device-wide barrier among `nblocks` workgroups that share `counter`
+// (monotonic, zeroed before the launch).  The caller's workgroups must all be
+// resident (cooperative launch).  `target` is kept by the caller and grows by
+// nblocks each call.  A bounded wait: after `spin_limit` polls the barrier
+// records an error in *err and every later barrier returns at once, so a
+// wait that can never complete ends instead of hanging the device.
+__device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned nblocks, unsigned& target,
+                                             int* err, unsigned long long spin_limit) {
+  __syncthreads();
+  target += nblocks;
+  __shared__ int s_ok;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      ok = 0;
+    } else {
+      __atomic_thread_fence(__ATOMIC_RELEASE);
+      __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned long long polls = 0;
+      while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++polls > spin_limit) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        if ((polls & 1023) == 0 &&
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ok = 0;
+          break;
+        }
+      }
+      __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+}  // namespace usbs

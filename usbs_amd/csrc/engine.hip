@@ -1,0 +1,367 @@
+// Context, problem upload and the standalone Lanczos entry point.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "../../include/usbs_hip.h"
+
+struct usbs_ctx {
+  usbs::Ctx c;
+};
+
+namespace usbs {
+
+int set_err(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+void* ctx_alloc(Ctx* c, std::vector<DevBuf>& pool, size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 8;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    c->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed";
+    return nullptr;
+  }
+  pool.push_back({p, bytes});
+  return p;
+}
+
+void ctx_free_pool(std::vector<DevBuf>& pool) {
+  for (auto& b : pool)
+    if (b.p) (void)hipFree(b.p);
+  pool.clear();
+}
+
+template <typename X>
+static X* upload(Ctx* c, std::vector<DevBuf>& pool, const X* host, size_t count) {
+  X* d = (X*)ctx_alloc(c, pool, sizeof(X) * count);
+  if (!d) return nullptr;
+  if (count && hipMemcpy(d, host, sizeof(X) * count, hipMemcpyHostToDevice) != hipSuccess) {
+    c->err = "hipMemcpy upload failed";
+    return nullptr;
+  }
+  return d;
+}
+
+// Build the merged CSR pattern of C and of the triples' (i, j), C's values in
+// it, and the slot->triple and constraint->triple maps (counting sorts, so
+// each slot and each constraint lists its triples in input order).
+static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32_t* Cr,
+                         const int32_t* Cc, const double* Cv, int64_t nA, const int32_t* Ai,
+                         const double* Ad, const double* b, const double* mask, const double* sa) {
+  if (n <= 0 || m <= 0 || nnzC < 0 || nA < 0 || n > (int64_t)INT32_MAX / 2 || m > INT32_MAX)
+    return set_err(c, USBS_ERR_INVALID, "invalid problem sizes");
+  for (int64_t e = 0; e < nnzC; ++e) {
+    if (Cr[e] < 0 || Cr[e] >= n || Cc[e] < 0 || Cc[e] >= n)
+      return set_err(c, USBS_ERR_INVALID, "C index out of range at entry " + std::to_string(e));
+    if (!std::isfinite(Cv[e])) return set_err(c, USBS_ERR_NONFINITE, "non-finite C value");
+  }
+  for (int64_t t = 0; t < nA; ++t) {
+    const int32_t cc = Ai[3 * t], ii = Ai[3 * t + 1], jj = Ai[3 * t + 2];
+    if (cc < 0 || cc >= m || ii < 0 || ii >= n || jj < 0 || jj >= n)
+      return set_err(c, USBS_ERR_INVALID, "A_indices out of range at triple " + std::to_string(t));
+    if (!std::isfinite(Ad[t])) return set_err(c, USBS_ERR_NONFINITE, "non-finite A_data");
+  }
+  for (int64_t i = 0; i < m; ++i)
+    if (!std::isfinite(b[i]) || !std::isfinite(mask[i]) || !std::isfinite(sa[i]))
+      return set_err(c, USBS_ERR_NONFINITE, "non-finite b / b_ineq_mask / SCALE_A");
+
+  // pattern keys
+  std::vector<uint64_t> keys;
+  keys.reserve((size_t)(nnzC + nA));
+  for (int64_t e = 0; e < nnzC; ++e) keys.push_back((uint64_t)Cr[e] * (uint64_t)n + (uint64_t)Cc[e]);
+  for (int64_t t = 0; t < nA; ++t)
+    keys.push_back((uint64_t)Ai[3 * t + 1] * (uint64_t)n + (uint64_t)Ai[3 * t + 2]);
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  const int64_t nnz = (int64_t)keys.size();
+  if (nnz > INT32_MAX) return set_err(c, USBS_ERR_INVALID, "operator pattern exceeds int32");
+  std::vector<int> row_ptr(n + 1, 0), col(nnz);
+  for (int64_t s = 0; s < nnz; ++s) {
+    row_ptr[keys[s] / n + 1]++;
+    col[s] = (int)(keys[s] % n);
+  }
+  for (int64_t i = 0; i < n; ++i) row_ptr[i + 1] += row_ptr[i];
+  auto slot_of = [&](uint64_t key) -> int64_t {
+    return std::lower_bound(keys.begin(), keys.end(), key) - keys.begin();
+  };
+  std::vector<double> cval(nnz, 0.0);
+  for (int64_t e = 0; e < nnzC; ++e) cval[slot_of((uint64_t)Cr[e] * n + Cc[e])] += Cv[e];
+  bool cut_int = true;
+  std::vector<int> cw4(nnz, 0);
+  for (int64_t s = 0; s < nnz; ++s) {
+    const double v4 = 4.0 * cval[s];
+    if (std::fabs(v4) > 1e9 || v4 != std::nearbyint(v4)) cut_int = false;
+    else cw4[s] = (int)std::nearbyint(v4);
+  }
+  std::vector<int> tslot(nA), slot_ptr(nnz + 1, 0), slot_trip(nA), con_ptr(m + 1, 0), con_trip(nA);
+  std::vector<int> tcon(nA), ti(nA), tj(nA);
+  for (int64_t t = 0; t < nA; ++t) {
+    tcon[t] = Ai[3 * t];
+    ti[t] = Ai[3 * t + 1];
+    tj[t] = Ai[3 * t + 2];
+    tslot[t] = (int)slot_of((uint64_t)ti[t] * n + tj[t]);
+    slot_ptr[tslot[t] + 1]++;
+    con_ptr[tcon[t] + 1]++;
+  }
+  for (int64_t s = 0; s < nnz; ++s) slot_ptr[s + 1] += slot_ptr[s];
+  for (int64_t i = 0; i < m; ++i) con_ptr[i + 1] += con_ptr[i];
+  {
+    std::vector<int> fill_s(slot_ptr.begin(), slot_ptr.end() - 1), fill_c(con_ptr.begin(), con_ptr.end() - 1);
+    for (int64_t t = 0; t < nA; ++t) {
+      slot_trip[fill_s[tslot[t]]++] = (int)t;
+      con_trip[fill_c[tcon[t]]++] = (int)t;
+    }
+  }
+  bool diag = (nA == n && m == n);
+  for (int64_t t = 0; t < nA && diag; ++t)
+    diag = (tcon[t] == t && ti[t] == t && tj[t] == t && Ad[t] == 1.0);
+  bool ineq = false;
+  for (int64_t i = 0; i < m; ++i) ineq |= (mask[i] != 0.0);
+
+  ctx_free_pool(c->prob_bufs);
+  ctx_free_pool(c->work_bufs);
+  c->has_problem = false;
+  Problem& P = c->P;
+  P = Problem();
+  P.n = (int)n;
+  P.m = (int)m;
+  P.nnz = nnz;
+  P.nA = nA;
+  P.tile = tile_rows_for((int)n);
+  P.ntiles = (int)((n + P.tile - 1) / P.tile);
+  P.mtile = tile_rows_for((int)m);
+  P.mntiles = (int)((m + P.mtile - 1) / P.mtile);
+  P.diag_A = diag;
+  P.has_ineq = ineq;
+  P.cut_integer = cut_int;
+  auto& pool = c->prob_bufs;
+  std::vector<double> tdata(Ad, Ad + nA);
+  if (!(P.row_ptr = upload(c, pool, row_ptr.data(), row_ptr.size()))) return USBS_ERR_OOM;
+  if (!(P.col = upload(c, pool, col.data(), col.size()))) return USBS_ERR_OOM;
+  if (!(P.cval = upload(c, pool, cval.data(), cval.size()))) return USBS_ERR_OOM;
+  if (!(P.cw4 = upload(c, pool, cw4.data(), cw4.size()))) return USBS_ERR_OOM;
+  if (!(P.slot_ptr = upload(c, pool, slot_ptr.data(), slot_ptr.size()))) return USBS_ERR_OOM;
+  if (!(P.slot_trip = upload(c, pool, slot_trip.data(), slot_trip.size()))) return USBS_ERR_OOM;
+  if (!(P.tcon = upload(c, pool, tcon.data(), tcon.size()))) return USBS_ERR_OOM;
+  if (!(P.ti = upload(c, pool, ti.data(), ti.size()))) return USBS_ERR_OOM;
+  if (!(P.tj = upload(c, pool, tj.data(), tj.size()))) return USBS_ERR_OOM;
+  if (!(P.tdata = upload(c, pool, tdata.data(), tdata.size()))) return USBS_ERR_OOM;
+  if (!(P.con_ptr = upload(c, pool, con_ptr.data(), con_ptr.size()))) return USBS_ERR_OOM;
+  if (!(P.con_trip = upload(c, pool, con_trip.data(), con_trip.size()))) return USBS_ERR_OOM;
+  if (!(P.b = upload(c, pool, b, (size_t)m))) return USBS_ERR_OOM;
+  if (!(P.mask = upload(c, pool, mask, (size_t)m))) return USBS_ERR_OOM;
+  if (!(P.scale_a = upload(c, pool, sa, (size_t)m))) return USBS_ERR_OOM;
+  c->has_problem = true;
+  return USBS_OK;
+}
+
+// Lanczos workspace for one instance of dimension n, basis m, storage T.
+template <typename T>
+static int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int nd,
+                        int max_restarts, double tol, const T* opval, LzInst<T>& L) {
+  L = LzInst<T>();
+  L.n = P.n;
+  L.tile = P.tile;
+  L.ntiles = P.ntiles;
+  L.m = m;
+  L.nd = nd;
+  L.max_restarts = max_restarts;
+  L.tol = tol;
+  L.pstride = ((m + 1) + 7) / 8 * 8;
+  L.nblocks = 1;
+  L.row_ptr = P.row_ptr;
+  L.col = P.col;
+  L.val = opval;
+  L.ldq = ((long long)P.n + 63) / 64 * 64;
+  if (!(L.Q = (T*)ctx_alloc(c, pool, sizeof(T) * L.ldq * (m + 1)))) return USBS_ERR_OOM;
+  if (!(L.w = (T*)ctx_alloc(c, pool, sizeof(T) * L.ldq))) return USBS_ERR_OOM;
+  const size_t ps = sizeof(double) * (size_t)L.ntiles * L.pstride;
+  if (!(L.part1 = (double*)ctx_alloc(c, pool, ps))) return USBS_ERR_OOM;
+  if (!(L.part2 = (double*)ctx_alloc(c, pool, ps))) return USBS_ERR_OOM;
+  if (!(L.part3 = (double*)ctx_alloc(c, pool, ps))) return USBS_ERR_OOM;
+  if (!(L.h1 = (double*)ctx_alloc(c, pool, sizeof(double) * (m + 1)))) return USBS_ERR_OOM;
+  if (!(L.h2 = (double*)ctx_alloc(c, pool, sizeof(double) * (m + 1)))) return USBS_ERR_OOM;
+  if (!(L.alpha = (double*)ctx_alloc(c, pool, sizeof(double) * m))) return USBS_ERR_OOM;
+  if (!(L.beta = (double*)ctx_alloc(c, pool, sizeof(double) * m))) return USBS_ERR_OOM;
+  if (!(L.Yr = (double*)ctx_alloc(c, pool, sizeof(double) * m * m))) return USBS_ERR_OOM;
+  if (!(L.ctrl = (int*)ctx_alloc(c, pool, sizeof(int) * LZ_NCTRL))) return USBS_ERR_OOM;
+  if (!(L.bar = (unsigned*)ctx_alloc(c, pool, sizeof(unsigned) * 64))) return USBS_ERR_OOM;
+  return USBS_OK;
+}
+
+// Reset per-call Lanczos state: Q = [q0, 0...], alpha = beta = 0, ctrl = 0.
+template <typename T>
+__global__ void k_lz_reset(LzInst<T> L, const double* q0) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long tot = L.ldq * (L.m + 1);
+  for (long long x = i; x < tot; x += (long long)gridDim.x * blockDim.x) {
+    const long long r = x % L.ldq, cidx = x / L.ldq;
+    L.Q[x] = (cidx == 0 && r < L.n) ? (T)q0[r] : (T)0;
+  }
+  if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
+  if (i < LZ_NCTRL) L.ctrl[i] = 0;
+  if (i < 64) L.bar[i] = 0u;
+}
+
+template <typename T>
+__global__ void k_copy_cols_out(const T* Q, long long ldq, int n, int nd, double* out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * nd) return;
+  const long long r = i / nd, j = i % nd;
+  out[i] = (double)Q[r + j * ldq];
+}
+
+template <typename T>
+int reset_lanczos(const LzInst<T>& L, const double* q0, hipStream_t s) {
+  k_lz_reset<T><<<1024, 256, 0, s>>>(L, q0);
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const double* q0, int nd,
+                                  int m, int max_restarts, double tol, int exec_mode,
+                                  double* eigvals, double* eigvecs, int* info) {
+  const Problem& P = c->P;
+  std::vector<DevBuf> pool;
+  struct Guard {
+    std::vector<DevBuf>& p;
+    ~Guard() { ctx_free_pool(p); }
+  } guard{pool};
+  double* d_y = upload(c, pool, y, (size_t)P.m);
+  double* d_q0 = upload(c, pool, q0, (size_t)P.n);
+  double* d_val64 = (double*)ctx_alloc(c, pool, sizeof(double) * P.nnz);
+  float* d_val32 = nullptr;
+  if (!d_y || !d_q0 || !d_val64) return USBS_ERR_OOM;
+  if (sizeof(T) == 4 && !(d_val32 = (float*)ctx_alloc(c, pool, sizeof(float) * P.nnz))) return USBS_ERR_OOM;
+  int rc = launch_assemble(P, C_coef, d_y, 1.0, nullptr, 0.0, nullptr, 0.0, d_val64, d_val32, c->stream);
+  if (rc) return set_err(c, USBS_ERR_HIP, "assemble launch failed");
+  LzInst<T> L;
+  rc = make_lanczos<T>(c, pool, P, m, nd, max_restarts, tol,
+                       sizeof(T) == 4 ? (const T*)d_val32 : (const T*)d_val64, L);
+  if (rc) return rc;
+  if (reset_lanczos(L, d_q0, c->stream)) return set_err(c, USBS_ERR_HIP, "reset launch failed");
+  int host_ctrl[LZ_NCTRL] = {0};
+  if (exec_mode == USBS_EXEC_PEROP) {
+    rc = lanczos_perop<T>(L, c->stream, host_ctrl);
+    if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos per-op: ") + hipGetErrorString((hipError_t)rc));
+  } else {
+    int nb = std::min(L.ntiles, std::max(1, c->coresident));
+    L.nblocks = nb;
+    LzInst<T>* d_inst = (LzInst<T>*)ctx_alloc(c, pool, sizeof(LzInst<T>));
+    int* d_map = (int*)ctx_alloc(c, pool, sizeof(int2) * nb);
+    if (!d_inst || !d_map) return USBS_ERR_OOM;
+    std::vector<int2> map(nb);
+    for (int i = 0; i < nb; ++i) map[i] = make_int2(0, i);
+    USBS_HIP(c, hipMemcpyAsync(d_inst, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
+    USBS_HIP(c, hipMemcpyAsync(d_map, map.data(), sizeof(int2) * nb, hipMemcpyHostToDevice, c->stream));
+    rc = lanczos_persistent<T>(d_inst, 1, &nb, m, c->stream, d_map, 1ull << 26);
+    if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos persistent: ") + hipGetErrorString((hipError_t)rc));
+    USBS_HIP(c, hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, c->stream));
+    USBS_HIP(c, hipStreamSynchronize(c->stream));
+    if (host_ctrl[LZ_ERROR]) return set_err(c, USBS_ERR_TIMEOUT, "persistent Lanczos barrier wait expired");
+  }
+  double* d_vecs = (double*)ctx_alloc(c, pool, sizeof(double) * (size_t)P.n * nd);
+  if (!d_vecs) return USBS_ERR_OOM;
+  const long long tot = (long long)P.n * nd;
+  k_copy_cols_out<T><<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(L.Q, L.ldq, P.n, nd, d_vecs);
+  USBS_HIP(c, hipGetLastError());
+  USBS_HIP(c, hipMemcpyAsync(eigvals, L.alpha, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream));
+  USBS_HIP(c, hipMemcpyAsync(eigvecs, d_vecs, sizeof(double) * tot, hipMemcpyDeviceToHost, c->stream));
+  USBS_HIP(c, hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, c->stream));
+  USBS_HIP(c, hipStreamSynchronize(c->stream));
+  if (info) {
+    info[0] = host_ctrl[LZ_RESTART];
+    info[1] = host_ctrl[LZ_NCONV];
+    info[2] = host_ctrl[LZ_STEPS];
+    info[3] = host_ctrl[LZ_NBREAK];
+  }
+  return USBS_OK;
+}
+
+}  // namespace usbs
+
+using namespace usbs;
+
+extern "C" {
+
+const char* usbs_version(void) { return "usbs-hip 0.1 (gfx950)"; }
+
+int usbs_create(usbs_ctx** out, int device) {
+  if (!out) return USBS_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return USBS_ERR_HIP;
+  if (device < 0 || device >= ndev) return USBS_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return USBS_ERR_HIP;
+  usbs_ctx* h = new usbs_ctx();
+  h->c.device = device;
+  if (hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return USBS_ERR_HIP;
+  }
+  h->c.coresident = lanczos_max_coresident_blocks(32, device);
+  *out = h;
+  return USBS_OK;
+}
+
+void usbs_destroy(usbs_ctx* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->c.device);
+  (void)hipStreamSynchronize(h->c.stream);
+  ctx_free_pool(h->c.work_bufs);
+  ctx_free_pool(h->c.prob_bufs);
+  (void)hipStreamDestroy(h->c.stream);
+  delete h;
+}
+
+const char* usbs_last_error(const usbs_ctx* h) { return h ? h->c.err.c_str() : "null context"; }
+
+int usbs_set_problem(usbs_ctx* h, int64_t n, int64_t m, int64_t nnzC, const int32_t* C_rows,
+                     const int32_t* C_cols, const double* C_vals, int64_t nA,
+                     const int32_t* A_indices, const double* A_data, const double* b,
+                     const double* b_ineq_mask, const double* SCALE_A) {
+  if (!h) return USBS_ERR_INVALID;
+  if ((nnzC > 0 && (!C_rows || !C_cols || !C_vals)) || (nA > 0 && (!A_indices || !A_data)) || !b ||
+      !b_ineq_mask || !SCALE_A)
+    return set_err(&h->c, USBS_ERR_INVALID, "null array argument");
+  if (hipSetDevice(h->c.device) != hipSuccess) return set_err(&h->c, USBS_ERR_HIP, "hipSetDevice");
+  return build_problem(&h->c, n, m, nnzC, C_rows, C_cols, C_vals, nA, A_indices, A_data, b,
+                       b_ineq_mask, SCALE_A);
+}
+
+int usbs_lanczos_smallest(usbs_ctx* h, double C_coef, const double* y, const double* q0,
+                          int num_desired, int inner_iterations, int max_restarts, double tolerance,
+                          int precision, int exec_mode, double* eigvals, double* eigvecs, int* info) {
+  if (!h) return USBS_ERR_INVALID;
+  Ctx* c = &h->c;
+  if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
+  if (!y || !q0 || !eigvals || !eigvecs) return set_err(c, USBS_ERR_INVALID, "null array argument");
+  const int m = inner_iterations;
+  if (m < 1 || m > 48 || m > c->P.n)
+    return set_err(c, USBS_ERR_INVALID, "inner_iterations must be in [1, min(n, 48)]");
+  if (num_desired < 1 || num_desired > m)
+    return set_err(c, USBS_ERR_INVALID, "num_desired must be in [1, inner_iterations]");
+  if (max_restarts < 1) return set_err(c, USBS_ERR_INVALID, "max_restarts must be >= 1");
+  if (precision != USBS_PREC_FP64 && precision != USBS_PREC_MIXED)
+    return set_err(c, USBS_ERR_INVALID, "unknown precision");
+  if (exec_mode != USBS_EXEC_PEROP && exec_mode != USBS_EXEC_PERSISTENT)
+    return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
+  for (int i = 0; i < c->P.m; ++i)
+    if (!std::isfinite(y[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite adjoint_left_vec");
+  for (int i = 0; i < c->P.n; ++i)
+    if (!std::isfinite(q0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite q0");
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(c, USBS_ERR_HIP, "hipSetDevice");
+  if (precision == USBS_PREC_FP64)
+    return run_lanczos_standalone<double>(c, C_coef, y, q0, num_desired, m, max_restarts, tolerance,
+                                          exec_mode, eigvals, eigvecs, info);
+  return run_lanczos_standalone<float>(c, C_coef, y, q0, num_desired, m, max_restarts, tolerance,
+                                       exec_mode, eigvals, eigvecs, info);
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Host-side engine internals: device context, resident problem data and
+// workspaces.  Everything here sits behind the C ABI in include/usbs_hip.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "lanczos.hpp"
+
+namespace usbs {
+
+// A device buffer owned by a context.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// The operator pattern merges C's pattern with the (i, j) pattern of the
+// constraint triples, so that C_coef * C + A*(z) is one CSR matrix whose
+// values are assembled per call.
+struct Problem {
+  int n = 0, m = 0;
+  long long nnz = 0, nA = 0;
+  int tile = 0, ntiles = 0;     // row tiles of n
+  int mtile = 0, mntiles = 0;   // tiles over the m constraints
+  bool diag_A = false;          // A = {(i, i, i) with data 1}: max-cut's diag(X) = 1
+  bool has_ineq = false;        // any b_ineq_mask != 0
+  bool cut_integer = false;     // every off-diagonal 4*C_ij is an integer
+  int* row_ptr = nullptr;       // n+1
+  int* col = nullptr;           // nnz
+  double* cval = nullptr;       // nnz: C in the merged pattern (0 where C has no entry)
+  int* cw4 = nullptr;           // nnz: round(4*C) when cut_integer
+  int* slot_ptr = nullptr;      // nnz+1: triples per pattern slot
+  int* slot_trip = nullptr;     // nA
+  int* tcon = nullptr;          // nA: triple constraint index
+  int* ti = nullptr;            // nA
+  int* tj = nullptr;            // nA
+  double* tdata = nullptr;      // nA
+  int* con_ptr = nullptr;       // m+1: triples per constraint
+  int* con_trip = nullptr;      // nA
+  double* b = nullptr;          // m
+  double* mask = nullptr;       // m
+  double* scale_a = nullptr;    // m
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool has_problem = false;
+  Problem P;
+  std::vector<DevBuf> prob_bufs;  // freed on new problem / destroy
+  std::vector<DevBuf> work_bufs;  // solve workspaces
+  int coresident = 0;             // max co-resident Lanczos workgroups on this device
+};
+
+// error helpers
+int set_err(Ctx* c, int code, const std::string& msg);
+#define USBS_HIP(ctx, expr)                                                               \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return ::usbs::set_err(ctx, -2, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+void* ctx_alloc(Ctx* c, std::vector<DevBuf>& pool, size_t bytes);
+void ctx_free_pool(std::vector<DevBuf>& pool);
+
+// kernels in ops.hip
+int launch_assemble(const Problem& P, double coefC, const double* z1, double a1, const double* z2,
+                    double a2, const double* z3, double a3, double* out64, float* out32,
+                    hipStream_t s);
+
+}  // namespace usbs

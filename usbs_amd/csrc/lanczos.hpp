@@ -1,0 +1,56 @@
+// Thick-restart Lanczos on the GPU: shared declarations.
+//
+// One Lanczos problem ("instance") is described by LzInst: the operator in
+// CSR (values assembled for C_arg + A*(v) beforehand), the basis Q
+// (n x (m+1), column-major), the work vector, tile partials and the small
+// FP64 state (alpha, beta, Ritz rotation, control words).
+#pragma once
+#include "dev_common.hpp"
+
+namespace usbs {
+
+enum LzCtrl {
+  LZ_K = 0,          // first step of the next restart (= number of saved Ritz vectors)
+  LZ_RESTART = 1,    // restarts completed
+  LZ_NCONV = 2,      // num_converged of the last restart
+  LZ_DONE = 3,       // 1 once num_converged >= num_desired or max_restarts reached
+  LZ_BREAKDOWN = 4,  // 1 if the current step hit beta < 1e-15
+  LZ_ERROR = 5,      // nonzero: a bounded wait expired (persistent launch)
+  LZ_NBREAK = 6,     // number of breakdowns seen
+  LZ_STEPS = 7,      // Lanczos steps taken (operator applications)
+  LZ_NCTRL = 8
+};
+
+template <typename T>
+struct LzInst {
+  int n, tile, ntiles, m, nd, max_restarts, pstride, nblocks;
+  double tol;
+  const int* row_ptr;
+  const int* col;
+  const T* val;
+  T* Q;          // n x (m+1), leading dimension ldq
+  long long ldq;
+  T* w;          // n
+  double* part1; // ntiles x pstride, one array per reduction pass
+  double* part2;
+  double* part3;
+  double* h1;    // m+1: first CGS coefficients of the current step
+  double* h2;    // m+1: second CGS coefficients
+  double* alpha; // m
+  double* beta;  // m
+  double* Yr;    // m x m row-major: saved Ritz vectors (columns) of T
+  int* ctrl;     // LZ_NCTRL
+  unsigned* bar; // barrier counter (persistent launch)
+};
+
+int tile_rows_for(int n);
+
+// Host launchers (lanczos.hip).  All return hipError_t as int.
+template <typename T>
+int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl);
+template <typename T>
+int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, const int* host_nblocks, int m,
+                       hipStream_t s, int* dev_wgmap, unsigned long long spin_limit);
+int lanczos_max_coresident_blocks(int m, int device);
+
+}  // namespace usbs
