@@ -75,6 +75,82 @@ int usbs_lanczos_smallest(usbs_ctx* ctx, double C_coef, const double* adjoint_le
                           int max_restarts, double tolerance, int precision, int exec_mode,
                           double* eigvals, double* eigvecs, int* info);
 
+
+/* Parameters of usbs() (solver/usbs.py:436-459), plus the execution choices. */
+typedef struct usbs_params {
+  int32_t k_curr, k_past;
+  int32_t max_iters;
+  int32_t lanczos_inner_iterations, lanczos_max_restarts;
+  int32_t subprob_max_iters;
+  int32_t sketch_dim;  /* r > 0: Nystrom sketch P (n x r); 0: diag-only P (n)       */
+  int32_t rounding;    /* 1: max-cut rounding at every condition check (needs r > 0) */
+  int32_t precision;   /* USBS_PREC_*                                               */
+  int32_t exec_mode;   /* USBS_EXEC_* for the Lanczos calls                          */
+  double trace_ub, trace_factor, rho, beta;
+  double max_time, obj_gap_eps, infeas_gap_eps, max_infeas_eps;
+  double subprob_eps, cond_exp_base;
+  double SCALE_C, SCALE_X;
+} usbs_params;
+
+/* One condition-check record: the fields of the reference's log line
+ * (solver/usbs.py:514-530) plus solver statistics. */
+typedef struct usbs_record {
+  double t, time, pen_dual_obj, cand_pen_dual_obj, lb_spec_est, primal_obj, dual_obj, obj_ub;
+  double obj_gap, infeas_gap, max_infeas, callback_val;
+  double ipm_iters, lb_iters, lanczos_restarts, descent;
+} usbs_record;
+
+/*
+ * The whole USBS loop on the device (solver/usbs.py:436-789): cold or warm
+ * start from (y0, z0, tr_X0, primal_obj0, P0), Omega the sketch test matrix
+ * (n x r row-major, NULL for diag-only), q0 the Lanczos start vector
+ * (solver/usbs.py:691-692).  Outputs y_out (m), z_out (m), P_out (n x r or n),
+ * scal_out = [tr_X, primal_obj, iterations, obj_gap, infeas_gap, max_infeas].
+ * Every condition check appends a record to log (up to log_cap).
+ */
+int usbs_solve(usbs_ctx* ctx, const usbs_params* params, const double* y0, const double* z0,
+               double tr_X0, double primal_obj0, const double* Omega, const double* P0,
+               const double* q0, double* y_out, double* z_out, double* P_out, double* scal_out,
+               usbs_record* log, int log_cap, int* log_len);
+
+/*
+ * Nystrom fixed-rank reconstruction from the sketch (utils/common.py:99-116):
+ * E (n x r row-major), Lambda (r), singular values in descending order.  A
+ * core that is not positive definite (e.g. P = 0) yields NaN, as XLA does.
+ */
+int usbs_reconstruct_from_sketch(usbs_ctx* ctx, int64_t n, int32_t r, const double* Omega,
+                                 const double* P, double* E, double* Lambda);
+
+/*
+ * Max-cut rounding callback (utils/maxcut_helpers.py:292-302) against the
+ * uploaded C: values[j] = sgn_j^T C sgn_j with sgn = 2 (E > 0) - 1 from the
+ * Nystrom factor; computed as an exact integer sum when every 4*C_ij is an
+ * integer.  signs (n x r row-major, may be NULL) returns the sign vectors.
+ */
+int usbs_maxcut_round(usbs_ctx* ctx, int32_t r, const double* P, const double* Omega,
+                      double* values, int8_t* signs);
+
+/*
+ * Sketch update (solver/usbs.py:580, 636; diag-only form usbs.py:574, 630
+ * when r == 0): P = eta Pbar + F (F^T Omega), Pbar' = eta Pbar + F_c (F_c^T
+ * Omega) with F_c the first kc columns.  F: n x k, Omega, Pbar: n x r
+ * row-major.  precision selects FP64 or FP32 storage.
+ */
+int usbs_sketch_update(usbs_ctx* ctx, int64_t n, int32_t k, int32_t kc, int32_t r,
+                       const double* F, const double* Omega, const double* Pbar, double eta,
+                       int32_t precision, double* P_out, double* Pbar_out);
+
+/*
+ * Primal objective, constraint map, dual candidate and infeasibility sums of
+ * one USBS iteration (solver/usbs.py:571-585, 637-652) for the uploaded
+ * problem: z = eta zbar + A(F F^T), zbar' = eta zbar + A(F_c F_c^T),
+ * ycand = y - (b - z - upsilon) / rho;  out6 = [tr(F^T C F), tr(F_c^T C F_c),
+ * b.ycand, zbar.ycand, |infeas/SCALE_A|^2, max |infeas|/SCALE_A].
+ */
+int usbs_objective_infeas(usbs_ctx* ctx, int32_t k, int32_t kc, const double* F, double eta,
+                          const double* zbar, const double* y, const double* upsilon, double rho,
+                          double* z_out, double* zbar_next_out, double* ycand_out, double* out6);
+
 #ifdef __cplusplus
 }
 #endif

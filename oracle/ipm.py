@@ -17,6 +17,24 @@ def _min_eig_positive_steps(M, dM, steps):
     return np.linalg.eigvalsh(M[None, :, :] + steps[:, None, None] * dM[None, :, :])[:, 0] > 0
 
 
+def quad_subprob_data(C, A_data, A_indices, U, b, upsilon, trace_ub, rho, bar_primal_obj, z_bar,
+                      tr_X_bar, y, V, n, m, k):
+    """Constants of the prox QP (solver/usbs.py:53-61): the subproblem is
+    min 1/2 [s; eta]^T [[Q11, q12], [q12^T, q22]] [s; eta] + h1^T s + h2 eta
+    over s = svec(S), S >= 0, eta >= 0, trace(S) + eta <= 1."""
+    svec = lambda mx: U @ mx.reshape(-1)
+    tr_X_bar = tr_X_bar if tr_X_bar > 0.0 else trace_ub
+    Q_11 = (trace_ub ** 2 / rho) * create_Q_base(m, k, U, A_data, A_indices, V)
+    q_12 = (trace_ub ** 2 / (rho * tr_X_bar)) * svec(
+        V.T @ apply_A_adjoint_batched(n, A_data, A_indices, z_bar, V))
+    q_22 = (trace_ub ** 2 / (rho * tr_X_bar ** 2)) * np.dot(z_bar, z_bar)
+    h_1 = trace_ub * svec(V.T @ (apply_A_adjoint_batched(n, A_data, A_indices, y, V)
+                                 - C @ V
+                                 - apply_A_adjoint_batched(n, A_data, A_indices, b - upsilon, V) / rho))
+    h_2 = (trace_ub / tr_X_bar) * (np.dot(z_bar, y) - bar_primal_obj - (np.dot(z_bar, b - upsilon) / rho))
+    return Q_11, q_12, q_22, h_1, h_2, tr_X_bar
+
+
 IPMState = namedtuple("IPMState", ["i", "S", "eta", "T", "zeta", "omega", "mu", "step_size"])
 
 
@@ -27,15 +45,9 @@ def solve_quad_subprob_ipm(C, A_data, A_indices, U, b, upsilon, trace_ub, rho, b
     svec = lambda mx: U @ mx.reshape(-1)
     svec_inv = lambda vec: (U.T @ vec).reshape(k, k)
 
-    tr_X_bar = tr_X_bar if tr_X_bar > 0.0 else trace_ub
-    Q_11 = (trace_ub ** 2 / rho) * create_Q_base(m, k, U, A_data, A_indices, V)
-    q_12 = (trace_ub ** 2 / (rho * tr_X_bar)) * svec(
-        V.T @ apply_A_adjoint_batched(n, A_data, A_indices, z_bar, V))
-    q_22 = (trace_ub ** 2 / (rho * tr_X_bar ** 2)) * np.dot(z_bar, z_bar)
-    h_1 = trace_ub * svec(V.T @ (apply_A_adjoint_batched(n, A_data, A_indices, y, V)
-                                 - C @ V
-                                 - apply_A_adjoint_batched(n, A_data, A_indices, b - upsilon, V) / rho))
-    h_2 = (trace_ub / tr_X_bar) * (np.dot(z_bar, y) - bar_primal_obj - (np.dot(z_bar, b - upsilon) / rho))
+    Q_11, q_12, q_22, h_1, h_2, tr_X_bar = quad_subprob_data(
+        C, A_data, A_indices, U, b, upsilon, trace_ub, rho, bar_primal_obj, z_bar, tr_X_bar, y, V,
+        n, m, k)
     svec_I = svec(np.eye(k))
 
     eta0 = (tr_X_bar / trace_ub) * eta_init

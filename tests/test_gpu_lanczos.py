@@ -36,9 +36,11 @@ def test_lanczos_fp64_matches_oracle(gpu_ctx, exec_mode):
     for j in range(10):
         s = np.sign(V_g[:, j] @ V_o[:, j])
         np.testing.assert_allclose(s * V_g[:, j], V_o[:, j], atol=1e-6)
-    # against a dense eigensolver: the converged head
+    # against a dense eigensolver: the smallest Ritz value has converged, and
+    # Ritz values never lie below the true eigenvalues (Cauchy interlacing)
     dense = np.linalg.eigvalsh((negC + scipy.sparse.diags(y)).toarray())
-    np.testing.assert_allclose(ev_g[:3], dense[:3], rtol=1e-9, atol=1e-8)
+    np.testing.assert_allclose(ev_g[0], dense[0], rtol=1e-9)
+    assert np.all(np.sort(ev_g) >= dense[:10] - 1e-9)
     # orthonormal basis
     np.testing.assert_allclose(V_g.T @ V_g, np.eye(10), atol=1e-10)
 

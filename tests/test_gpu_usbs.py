@@ -1,0 +1,74 @@
+"""End-to-end usbs() on the GPU against the FP64 oracle loop
+(solver/usbs.py:436-789 restated in oracle/usbs.py)."""
+import numpy as np
+import pytest
+
+from oracle import maxcut as omc, usbs as ousbs
+from usbs_amd import maxcut_helpers, problems
+from usbs_amd.usbs import usbs
+
+pytestmark = pytest.mark.gpu
+
+HP = dict(trace_factor=2.0, beta=0.25, max_time=1e9, obj_gap_eps=-1.0, infeas_gap_eps=-1.0,
+          max_infeas_eps=-1.0, lanczos_max_restarts=10, subprob_eps=1e-7, subprob_max_iters=100,
+          cond_exp_base=1.0)
+
+
+def _run_both(W, iters, k_curr, k_past, r, rho, precision="fp64", exec_mode="perop", ctx=None):
+    n = W.shape[0]
+    Om = problems.gaussian_matrix(n, r, 1)
+    q0 = problems.unit_start_vector(n, 2)
+    so = omc.initialize_state(W, r, Om)
+    log_o = []
+    ousbs.usbs(so, n, n, 2.0 * n, HP["trace_factor"], rho, HP["beta"], k_curr, k_past, iters,
+               HP["max_time"], -1.0, -1.0, -1.0, min(n, 32), 10, 1e-7, 100, 1.0,
+               omc.compute_max_cut, None, so.C, q0, log=log_o)
+    sg = maxcut_helpers.initialize_state(W, r, Omega=Om)
+    log_g = []
+    out = usbs(sg, n, n, 2.0 * n, HP["trace_factor"], rho, HP["beta"], k_curr, k_past, iters,
+               HP["max_time"], -1.0, -1.0, -1.0, min(n, 32), 10, 1e-7, 100, 1.0,
+               maxcut_helpers.compute_max_cut, None, sg.C, precision=precision, exec_mode=exec_mode,
+               q0=q0, log=log_g, context=ctx)
+    return log_o, log_g, out
+
+
+@pytest.mark.parametrize("exec_mode", ["perop", "persistent"])
+def test_usbs_fp64_trajectory_matches_oracle(gpu_ctx, exec_mode):
+    W = problems.erdos_renyi(120, 0.08, seed=11)
+    print("case: ER(120,0.08) k=4+1 r=5 rho=1 exec=%s" % exec_mode, flush=True)
+    log_o, log_g, _ = _run_both(W, 12, 4, 1, 5, 1.0, exec_mode=exec_mode, ctx=gpu_ctx)
+    assert len(log_o) == len(log_g) == 13
+    for a, b in zip(log_o[1:], log_g[1:]):
+        for key in ("primal_obj", "obj_ub", "pen_dual_obj", "lb_spec_est", "obj_gap", "infeas_gap"):
+            np.testing.assert_allclose(b[key], a[key], rtol=1e-6, atol=1e-9, err_msg=key)
+        assert b["callback_val"] == a["callback_val"]
+
+
+def test_usbs_reference_config_er800_like(gpu_ctx):
+    """k_curr=10, k_past=1, sketch 10, rho=0.01 (the reference's max-cut
+    settings) on a smaller ER graph: 6 iterations agree with the oracle."""
+    W = problems.erdos_renyi(200, 0.06, seed=0)
+    log_o, log_g, _ = _run_both(W, 6, 10, 1, 10, 0.01, ctx=gpu_ctx)
+    for a, b in zip(log_o[1:], log_g[1:]):
+        for key in ("primal_obj", "obj_ub", "obj_gap", "infeas_gap", "max_infeas"):
+            np.testing.assert_allclose(b[key], a[key], rtol=1e-6, atol=1e-9, err_msg=key)
+
+
+def test_usbs_bipartite_closed_form(gpu_ctx):
+    """Unit-weight even torus is bipartite: SDP value = |E| = 200."""
+    W = problems.torus(10, 10, seed=0, signed=False)
+    n = W.shape[0]
+    sg = maxcut_helpers.initialize_state(W, 10, omega_seed=3)
+    log = []
+    out = usbs(sg, n, n, 2.0 * n, 2.0, 0.01, 0.25, 10, 1, 50, 1e9, 1e-7, 1e-7, 1e-7, 32, 10, 1e-7,
+               100, 1.0, maxcut_helpers.compute_max_cut, None, sg.C, log=log, context=gpu_ctx)
+    assert abs(out.primal_obj - 200.0) < 1e-3
+    assert log[-1]["callback_val"] == 200.0
+
+
+def test_usbs_mixed_precision_close_to_fp64(gpu_ctx):
+    W = problems.erdos_renyi(150, 0.08, seed=12, signed=True)
+    log_o, log_g, _ = _run_both(W, 10, 4, 1, 5, 1.0, precision="mixed", exec_mode="persistent",
+                                ctx=gpu_ctx)
+    for a, b in zip(log_o[1:], log_g[1:]):
+        np.testing.assert_allclose(b["primal_obj"], a["primal_obj"], rtol=1e-4)

@@ -47,6 +47,7 @@ def _declare(lib):
     for name, args in EXTRA_SIGNATURES.items():
         if hasattr(lib, name):
             getattr(lib, name).argtypes = args
+    _declared[0] = len(EXTRA_SIGNATURES)
 
 
 # Filled by modules that bind further entry points (kept here so one loader
@@ -63,7 +64,12 @@ def lib():
                                "there is no CPU fallback for the HIP backend")
         _lib = ctypes.CDLL(LIB_PATH)
         _declare(_lib)
+    elif len(EXTRA_SIGNATURES) != _declared[0]:
+        _declare(_lib)
     return _lib
+
+
+_declared = [0]
 
 
 def dptr(a):

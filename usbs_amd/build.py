@@ -15,13 +15,13 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libusbs_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["engine.hip", "lanczos.hip", "ops.hip", "dense_kernels.hip", "solve.hip"]
+SOURCES = ["engine.hip", "lanczos.hip", "ops.hip", "dense_kernels.hip", "solve.hip", "api_ops.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-    # packed FP32 ops are kept off for every kernel (numerics)
+    # packed FP32 instructions stay disabled for every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics", "-Wno-unused-result",
+    "-Wno-unused-result",
 ]
 
 

@@ -1,0 +1,516 @@
+// Single-workgroup FP64 kernels of the USBS loop: the two prox-bundle IPM
+// subproblems (solver/usbs.py:25-356) with the bundle eigendecomposition
+// epilogue, the Nystrom core of the max-cut rounding (utils/common.py:99-116),
+// and the scalar bookkeeping / convergence test of one iteration
+// (solver/usbs.py:581-652).  One wavefront runs each; all operands are in
+// LDS, so the workgroup barriers are cheap.
+#include "dense.hpp"
+#include "solve.hpp"
+
+namespace usbs {
+
+#define IPM_NT 64
+#define IPM_MAXK 11
+
+__device__ __forceinline__ double dotd(const double* x, const double* y, int d, double* red) {
+  double s = 0.0;
+  for (int a = threadIdx.x; a < d; a += blockDim.x) s += x[a] * y[a];
+  return block_sum1(s, red);
+}
+
+// 0.5 * U (T (x) Sinv + Sinv (x) T) U^T, entry (a, b)  (solver/usbs.py:92-93)
+__device__ __forceinline__ double symkron(const double* T, const double* Si, int k, int a, int b) {
+  int b1, c1, b2, c2;
+  svec_pair(a, b1, c1);
+  svec_pair(b, b2, c2);
+  const double u = 0.70710678118654752440;
+  const int na = (b1 == c1) ? 1 : 2, nb = (b2 == c2) ? 1 : 2;
+  const double ua = (na == 1) ? 1.0 : u, ub = (nb == 1) ? 1.0 : u;
+  double s = 0.0;
+  for (int x = 0; x < na; ++x) {
+    const int p = x == 0 ? b1 : c1, q = x == 0 ? c1 : b1;
+    for (int y = 0; y < nb; ++y) {
+      const int pp = y == 0 ? b2 : c2, qq = y == 0 ? c2 : b2;
+      s += ua * ub * (T[p * k + pp] * Si[q * k + qq] + Si[p * k + pp] * T[q * k + qq]);
+    }
+  }
+  return 0.5 * s;
+}
+
+// inverse of a k x k matrix by LU with partial pivoting (as jnp.linalg.inv).
+__device__ void dense_inv(const double* S, int k, double* Sinv, double* tmp, double* red, int* ired) {
+  for (int idx = threadIdx.x; idx < k * k; idx += blockDim.x) {
+    tmp[idx] = S[idx];
+    Sinv[idx] = (idx / k == idx % k) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  dense_lu_solve(tmp, k, Sinv, k, red, ired);
+}
+
+// Largest 0.9^j (j < max_iters) with M + 0.9^j dM positive definite, or 0:
+// the vmapped eigvalsh test of solver/usbs.py:135-144 with a Cholesky test
+// per candidate (lanes of the wavefront test 32 candidates at a time).
+__device__ double pd_step(const double* M, const double* dM, int k, int max_iters, double* scratch) {
+  const int lane = threadIdx.x;
+  for (int j0 = 0; j0 < max_iters; j0 += 32) {
+    double v = 0.0;
+    if (lane < 32 && j0 + lane < max_iters) {
+      const double s = pow(0.9, (double)(j0 + lane));
+      if (thread_is_pd(M, dM, s, k, scratch + lane * k * k)) v = s;
+    }
+    v = wave_max(v);
+    if (v > 0.0) return v;
+  }
+  return 0.0;
+}
+
+__device__ void mat_from_svec(const double* v, int k, double* M) {
+  dense_svec_inv(v, k, M);
+  __syncthreads();
+}
+
+struct IpmLds {
+  double *coeff, *q12, *h1, *sI, *sS, *sT, *F1, *ordv, *dsS, *dsT, *tmpv;
+  double *S, *T, *Si, *dS, *dT, *tk, *red;
+  int* ired;
+};
+
+__device__ IpmLds ipm_layout(double* lds, int k) {
+  const int d = k * (k + 1) / 2, kk = k * k;
+  IpmLds L;
+  double* p = lds;
+  L.coeff = p; p += (d * d > 32 * kk ? d * d : 32 * kk);
+  L.q12 = p; p += d; L.h1 = p; p += d; L.sI = p; p += d; L.sS = p; p += d; L.sT = p; p += d;
+  L.F1 = p; p += d; L.ordv = p; p += d; L.dsS = p; p += d; L.dsT = p; p += d; L.tmpv = p; p += d;
+  L.S = p; p += kk; L.T = p; p += kk; L.Si = p; p += kk; L.dS = p; p += kk; L.dT = p; p += kk;
+  L.tk = p; p += kk;
+  L.red = p; p += 16;
+  L.ired = (int*)p;
+  return L;
+}
+
+static size_t ipm_lds_bytes(int k) {
+  const int d = k * (k + 1) / 2, kk = k * k;
+  const int big = d * d > 32 * kk ? d * d : 32 * kk;
+  return sizeof(double) * (big + 10 * d + 6 * kk + 16 + 16);
+}
+
+// mu of the current point (solver/usbs.py:75-77, 163-165)
+__device__ double ipm_mu(const IpmLds& L, int k, int d, double eta, double zeta, double omega) {
+  dense_svec(L.S, k, L.sS);
+  dense_svec(L.T, k, L.sT);
+  __syncthreads();
+  const double st = dotd(L.sS, L.sT, d, L.red);
+  const double trS = dotd(L.sI, L.sS, d, L.red);
+  return (st + eta * zeta + omega * (1 - trS - eta)) / (2 * k + 4.0);
+}
+
+// Initial T, zeta, omega from T0 (in L.T) and zeta0 (usbs.py:67-74 / 228-235).
+__device__ void ipm_center(const IpmLds& L, int k, double& zeta, double& omega) {
+  double mx = zeta, mn = zeta;
+  for (int i = 0; i < k; ++i) {
+    const double v = L.T[i * k + i];
+    mx = v > mx ? v : mx;
+    mn = v < mn ? v : mn;
+  }
+  double width = mx - mn;
+  width = width < 100.0 ? 100.0 : width;
+  omega = -1.01 * mn;
+  omega = omega < width ? width : omega;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) L.T[i * k + i] += omega;
+  zeta += omega;
+  __syncthreads();
+}
+
+// Shared step: sizes, line search, update; returns the new mu and step.
+__device__ void ipm_step(const IpmLds& L, int k, int d, int max_iters, double& eta, double& zeta,
+                         double& omega, double& mu, double& step, double d_eta, double d_zeta,
+                         double d_omega) {
+  mat_from_svec(L.dsS, k, L.dS);
+  mat_from_svec(L.dsT, k, L.dT);
+  const double r1 = d_eta / eta, r2 = d_zeta / zeta, r3 = d_omega / omega;
+  double rmin = r1;
+  if (r2 < rmin) rmin = r2;
+  if (r3 < rmin) rmin = r3;
+  if (r1 != r1 || r2 != r2 || r3 != r3) rmin = NAN;  // jnp.min propagates NaN
+  double st = -1.0 / rmin;
+  st = st > 1.0 ? 1.0 : st;
+  st = 0.99 * ((st <= 0.0) ? 1.0 : st);
+  const double cS = pd_step(L.S, L.dS, k, max_iters, L.coeff);
+  const double cT = pd_step(L.T, L.dT, k, max_iters, L.coeff);
+  const double cand = cS < cT ? cS : cT;
+  st = st < cand ? st : cand;
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < k * k; idx += blockDim.x) {
+    L.S[idx] += st * L.dS[idx];
+    L.T[idx] += st * L.dT[idx];
+  }
+  __syncthreads();
+  eta += st * d_eta;
+  zeta += st * d_zeta;
+  omega += st * d_omega;
+  double mun = ipm_mu(L, k, d, eta, zeta, omega);
+  mun *= (st > 0.2) ? (0.5 - 0.4 * st * st) : 1.0;
+  mu = mun > mu ? mu : mun;
+  step = st;
+}
+
+__device__ void qmatvec(const double* Gq, double cq, const double* v, int d, double* out) {
+  for (int a = threadIdx.x; a < d; a += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < d; ++b) s += (cq * Gq[a * d + b]) * v[b];
+    out[a] = s;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
+  extern __shared__ double lds[];
+  const int k = A.k, d = k * (k + 1) / 2, t = threadIdx.x;
+  IpmLds L = ipm_layout(lds, k);
+  double* sc = A.sc;
+  const double tu = A.tu, rho = A.rho;
+  const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
+  const double pb = sc[SC_PB];
+  const double cq = tu * tu / rho;
+  const double q22 = (tu * tu / (rho * trXb * trXb)) * sc[SC_ZZ];
+  const double h2 = (tu / trXb) * (sc[SC_ZY] - pb - (sc[SC_ZBU] / rho));
+  dense_svec(A.Gz, k, L.q12);
+  dense_svec(A.Gh, k, L.h1);
+  for (int a = t; a < d; a += IPM_NT) {
+    int b, c;
+    svec_pair(a, b, c);
+    L.sI[a] = (b == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int a = t; a < d; a += IPM_NT) {
+    L.q12[a] *= tu * tu / (rho * trXb);
+    L.h1[a] *= tu;
+  }
+  // warm start (solve_step_subprob init, usbs.py:422-427, then usbs.py:64-65)
+  double eta_in;
+  if (A.first) {
+    eta_in = (tu / trXb) * ((pb == 0.0) ? 0.00001 : 0.99 * (1.0 / (k + 1.0)));
+    for (int idx = t; idx < k * k; idx += IPM_NT)
+      L.S[idx] = (idx / k == idx % k) ? tu * (0.99 * (1.0 / (k + 1.0))) : 0.0;
+  } else {
+    eta_in = A.eta_io[0];
+    for (int idx = t; idx < k * k; idx += IPM_NT) L.S[idx] = A.S_io[idx];
+  }
+  __syncthreads();
+  double eta = (trXb / tu) * eta_in;
+  for (int idx = t; idx < k * k; idx += IPM_NT) L.S[idx] = L.S[idx] / tu;
+  __syncthreads();
+  dense_svec(L.S, k, L.sS);
+  __syncthreads();
+  qmatvec(A.Gq, cq, L.sS, d, L.tmpv);
+  for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = L.tmpv[a] + eta * L.q12[a] + L.h1[a];
+  __syncthreads();
+  mat_from_svec(L.tmpv, k, L.T);
+  double zeta = dotd(L.q12, L.sS, d, L.red) + eta * q22 + h2;
+  double omega;
+  ipm_center(L, k, zeta, omega);
+  double mu = ipm_mu(L, k, d, eta, zeta, omega);
+  double step = 1.0;
+  int it = 0;
+  const double eps = A.eps;
+  while (it < A.max_iters && mu > eps && step > eps * eps) {
+    // sS, sT are current (ipm_mu recomputes them)
+    const double trS = dotd(L.sI, L.sS, d, L.red);
+    const double k1 = (1 - trS - eta) / omega;
+    const double k2 = (zeta / eta) + q22;
+    qmatvec(A.Gq, cq, L.sS, d, L.F1);
+    for (int a = t; a < d; a += IPM_NT)
+      L.F1[a] = (L.F1[a] + eta * L.q12[a] + L.h1[a]) + (-L.sT[a] + omega * L.sI[a]);
+    __syncthreads();
+    double F2 = dotd(L.q12, L.sS, d, L.red) + eta * q22 + h2;
+    F2 = F2 + (-zeta + omega);
+    dense_inv(L.S, k, L.Si, L.tk, L.red, L.ired);
+    const double den = k1 * k2 + 1;
+    for (int idx = t; idx < d * d; idx += IPM_NT) {
+      const int a = idx / d, b = idx % d;
+      double v = cq * A.Gq[idx] + symkron(L.T, L.Si, k, a, b);
+      v -= (L.q12[a] * (k1 * L.q12[b] + L.sI[b]) + L.sI[a] * (L.q12[b] - k2 * L.sI[b])) / den;
+      L.coeff[idx] = v;
+    }
+    dense_svec(L.Si, k, L.tmpv);
+    __syncthreads();
+    const double c_a = (mu / omega + trS + eta - 1 + k1 * (F2 - mu / eta + zeta)) / den;
+    const double c_b = (F2 - mu / eta + zeta - k2 * (mu / omega + trS + eta - 1)) / den;
+    for (int a = t; a < d; a += IPM_NT) {
+      double v = -L.F1[a] + mu * L.tmpv[a] - L.sT[a];
+      v += c_a * L.q12[a];
+      v += c_b * L.sI[a];
+      L.dsS[a] = v;
+    }
+    __syncthreads();
+    dense_lu_solve(L.coeff, d, L.dsS, 1, L.red, L.ired);
+    // back-substitution for the other directions (usbs.py:111-122)
+    for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = k1 * L.q12[a] + L.sI[a];
+    __syncthreads();
+    const double p1 = dotd(L.tmpv, L.dsS, d, L.red);
+    const double d_eta = (-p1 - (mu / omega) - trS - eta + 1 + k1 * (-F2 + mu / eta - zeta)) / den;
+    for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = L.q12[a] - k2 * L.sI[a];
+    __syncthreads();
+    const double p2 = dotd(L.tmpv, L.dsS, d, L.red);
+    const double d_omega = (-F2 + mu / eta - zeta + k2 * (mu / omega + trS + eta - 1) - p2) / den;
+    qmatvec(A.Gq, cq, L.dsS, d, L.dsT);
+    for (int a = t; a < d; a += IPM_NT)
+      L.dsT[a] = L.dsT[a] + d_eta * L.q12[a] + d_omega * L.sI[a] + L.F1[a];
+    __syncthreads();
+    const double d_zeta = dotd(L.q12, L.dsS, d, L.red) + d_eta * q22 + d_omega + F2;
+    ipm_step(L, k, d, A.max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ++it;
+  }
+  // outputs (usbs.py:191-192) and the bundle factor (usbs.py:568-570)
+  const double eta_out = (tu / trXb) * eta;
+  for (int idx = t; idx < k * k; idx += IPM_NT) A.S_io[idx] = tu * L.S[idx];
+  if (t == 0) {
+    A.eta_io[0] = eta_out;
+    sc[SC_ETA] = eta_out;
+    sc[SC_IPM_IT] = it;
+  }
+  __syncthreads();
+  double* Ae = L.coeff;
+  double* Ve = Ae + k * k;
+  double* tmp = Ve + k * k;
+  double* w = tmp + k * k;
+  double* coef = w + k;
+  int* part = L.ired + 4;
+  for (int idx = t; idx < k * k; idx += IPM_NT) Ae[idx] = tu * L.S[idx];
+  __syncthreads();
+  double trs = 0.0;
+  for (int i = 0; i < k; ++i) trs += tu * L.S[i * k + i];
+  dense_eigh(Ae, k, w, Ve, tmp, coef, part, L.red);
+  for (int idx = t; idx < k * k; idx += IPM_NT) {
+    const int i = idx / k, x = idx % k;
+    const double lam = w[x] < 0.0 ? 0.0 : w[x];
+    A.GF[idx] = Ve[idx] * sqrt(lam);
+    if (x >= A.kc) A.Epast[i * (k - A.kc) + (x - A.kc)] = Ve[idx];
+  }
+  if (t == 0) {
+    double s = 0.0;
+    for (int x = 0; x < A.kc; ++x) s += (w[x] < 0.0 ? 0.0 : w[x]);
+    sc[SC_TRS] = trs;
+    sc[SC_SUMLAM] = s;
+  }
+}
+
+int launch_ipm_quad(const IpmArgs& a, hipStream_t s) {
+  if (a.k < 1 || a.k > IPM_MAXK) return (int)hipErrorInvalidValue;
+  k_ipm_quad<<<1, IPM_NT, ipm_lds_bytes(a.k), s>>>(a);
+  return (int)hipGetLastError();
+}
+
+// Spectral lower bound (solver/usbs.py:195-356); writes sc[SC_LB].
+__global__ void __launch_bounds__(IPM_NT)
+k_ipm_lb(int k, int max_iters, double tu, double eps, const double* Glb, double* sc) {
+  extern __shared__ double lds[];
+  const int d = k * (k + 1) / 2, t = threadIdx.x;
+  IpmLds L = ipm_layout(lds, k);
+  const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
+  const double pb = sc[SC_PB];
+  double* g1 = L.h1;
+  dense_svec(Glb, k, g1);
+  for (int a = t; a < d; a += IPM_NT) {
+    int b, c;
+    svec_pair(a, b, c);
+    L.sI[a] = (b == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int a = t; a < d; a += IPM_NT) g1[a] *= tu;
+  const double g2 = (tu / trXb) * (sc[SC_ZYC] - pb);
+  for (int idx = t; idx < k * k; idx += IPM_NT)
+    L.S[idx] = (idx / k == idx % k) ? 0.99 * (1.0 / (k + 1.0)) : 0.0;
+  __syncthreads();
+  double eta = (pb == 0.0) ? 0.00001 : 0.99 * (1.0 / (k + 1.0));
+  mat_from_svec(g1, k, L.T);
+  double zeta = g2, omega;
+  ipm_center(L, k, zeta, omega);
+  double mu = ipm_mu(L, k, d, eta, zeta, omega);
+  double step = 1.0;
+  int it = 0;
+  while (it < max_iters && mu > eps) {
+    const double trS = dotd(L.sI, L.sS, d, L.red);
+    const double k1 = (1 - trS - eta) / omega;
+    dense_inv(L.S, k, L.Si, L.tk, L.red, L.ired);
+    const double ze = zeta / eta;
+    const double cI = ze / (k1 * ze + 1);
+    for (int idx = t; idx < d * d; idx += IPM_NT) {
+      const int a = idx / d, b = idx % d;
+      L.coeff[idx] = symkron(L.T, L.Si, k, a, b) + cI * (L.sI[a] * L.sI[b]);
+    }
+    dense_svec(L.Si, k, L.tmpv);
+    __syncthreads();
+    const double cf = ze * (-k1 * (mu / eta - g2 - omega) + mu / omega + trS + eta - 1) / (-k1 * ze - 1);
+    for (int a = t; a < d; a += IPM_NT) {
+      double v = cf * L.sI[a];
+      v = v + ((g2 - mu / eta) * L.sI[a] - g1[a]);
+      v = v + mu * L.tmpv[a];
+      L.dsS[a] = v;
+    }
+    __syncthreads();
+    dense_lu_solve(L.coeff, d, L.dsS, 1, L.red, L.ired);
+    const double p1 = dotd(L.sI, L.dsS, d, L.red);
+    const double d_eta = (p1 - k1 * (mu / eta - g2 - omega) + mu / omega + trS + eta - 1) / (-k1 * ze - 1);
+    double d_omega = -ze * d_eta + mu / eta;
+    d_omega = d_omega - g2 - omega;
+    for (int a = t; a < d; a += IPM_NT)
+      L.dsT[a] = d_omega * L.sI[a] - L.sT[a] + g1[a] + omega * L.sI[a];
+    __syncthreads();
+    const double d_zeta = d_omega - zeta + g2 + omega;
+    ipm_step(L, k, d, max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ++it;
+  }
+  dense_svec(L.S, k, L.sS);
+  __syncthreads();
+  const double gs = dotd(g1, L.sS, d, L.red);
+  if (t == 0) {
+    sc[SC_LB] = sc[SC_BYC] - gs - eta * g2;
+    sc[SC_LB_IT] = it;
+  }
+}
+
+int launch_ipm_lb(int k, int max_iters, double tu, double eps, const double* Glb, double* sc,
+                  hipStream_t s) {
+  if (k < 1 || k > IPM_MAXK) return (int)hipErrorInvalidValue;
+  k_ipm_lb<<<1, IPM_NT, ipm_lds_bytes(k), s>>>(k, max_iters, tu, eps, Glb, sc);
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Nystrom core (utils/common.py:99-116) from the r x r Grams P^T P, Omega^T P,
+// Omega^T Omega:  rho = sqrt(n) 1e-6 |P|_2,  B = sym(Omega^T (P + rho Omega)),
+// L = chol(B),  M = (P + rho Omega) L^{-T} = E Sigma W^T with
+// M^T M = L^{-1} P_rho^T P_rho L^{-T}.  Outputs G = L^{-T} W Sigma^{-1}
+// (so E = P_rho G), lam = max(sigma^2 - rho, 0) in descending sigma order,
+// sc[SC_NYS_RHO] = rho.  A Cholesky failure gives NaN G and lam.
+__global__ void __launch_bounds__(IPM_NT)
+k_nystrom_core(int n, int r, const double* PtPO, const double* OtO, double* G, double* lam,
+               double* sc) {
+  extern __shared__ double lds[];
+  const int t = threadIdx.x, rr = r * r;
+  double* A = lds;
+  double* V = A + rr;
+  double* tmp = V + rr;
+  double* w = tmp + rr;
+  double* coef = w + r;
+  double* red = coef + 2 * r;
+  double* B = red + 16;
+  double* M = B + rr;
+  double* X = M + rr;
+  int* part = (int*)(X + rr);
+  int* flag = part + r + 4;
+  // PtPO = P^T [P | Omega] (r x 2r row-major): P^T P and P^T Omega
+#define PtP(i, j) PtPO[(i) * 2 * r + (j)]
+#define OtP(i, j) PtPO[(j) * 2 * r + r + (i)]
+  for (int idx = t; idx < rr; idx += IPM_NT) A[idx] = PtP(idx / r, idx % r);
+  __syncthreads();
+  dense_eigh(A, r, w, V, tmp, coef, part, red);
+  const double lmax = w[r - 1];
+  const double normP = sqrt(lmax > 0.0 ? lmax : 0.0);
+  const double rho = sqrt((double)n) * 1e-6 * normP;
+  for (int idx = t; idx < rr; idx += IPM_NT) {
+    const int i = idx / r, j = idx % r;
+    const double bij = OtP(i, j) + rho * OtO[i * r + j];
+    const double bji = OtP(j, i) + rho * OtO[j * r + i];
+    B[idx] = 0.5 * (bij + bji);
+    // P_rho^T P_rho = P^T P + rho (P^T Omega + Omega^T P) + rho^2 Omega^T Omega
+    M[idx] = PtP(i, j) + rho * (OtP(j, i) + OtP(i, j)) + rho * rho * OtO[idx];
+  }
+  __syncthreads();
+  const bool ok = dense_cholesky(B, r, flag);
+  if (!ok) {
+    for (int idx = t; idx < rr; idx += IPM_NT) G[idx] = NAN;
+    for (int j = t; j < r; j += IPM_NT) lam[j] = NAN;
+    if (t == 0) sc[SC_NYS_RHO] = rho;
+    return;
+  }
+  // X = L^{-1} M L^{-T}: forward solves on the columns of M, then of X^T.
+  if (t < r) {
+    for (int i = 0; i < r; ++i) {
+      double v = M[i * r + t];
+      for (int p = 0; p < i; ++p) v -= B[i * r + p] * tmp[p * r + t];
+      tmp[i * r + t] = v / B[i * r + i];
+    }
+  }
+  __syncthreads();
+  if (t < r) {
+    for (int i = 0; i < r; ++i) {
+      double v = tmp[t * r + i];
+      for (int p = 0; p < i; ++p) v -= B[i * r + p] * X[p * r + t];
+      X[i * r + t] = v / B[i * r + i];
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < rr; idx += IPM_NT) {
+    const int i = idx / r, j = idx % r;
+    A[idx] = 0.5 * (X[i * r + j] + X[j * r + i]);
+  }
+  __syncthreads();
+  dense_eigh(A, r, w, V, tmp, coef, part, red);
+  // G = L^{-T} W (descending order) Sigma^{-1}: back substitution with L^T.
+  if (t < r) {
+    const int src = r - 1 - t;
+    const double sig = sqrt(w[src] > 0.0 ? w[src] : 0.0);
+    for (int i = r - 1; i >= 0; --i) {
+      double v = V[i * r + src];
+      for (int p = i + 1; p < r; ++p) v -= B[p * r + i] * X[p * r + t];
+      X[i * r + t] = v / B[i * r + i];
+    }
+    for (int i = 0; i < r; ++i) G[i * r + t] = X[i * r + t] / sig;
+    const double l = w[src] - rho;
+    lam[t] = l > 0.0 ? l : 0.0;
+  }
+  if (t == 0) sc[SC_NYS_RHO] = rho;
+#undef PtP
+#undef OtP
+}
+
+int launch_nystrom_core(int n, int r, const double* PtPO, const double* OtO, double* G,
+                        double* lam, double* sc, hipStream_t s) {
+  if (r < 1 || r > 48) return (int)hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * (7 * r * r + 4 * r + 32);
+  k_nystrom_core<<<1, IPM_NT, lds, s>>>(n, r, PtPO, OtO, G, lam, sc);
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scalar bookkeeping of one USBS iteration (solver/usbs.py:581-652).
+__global__ void k_scalars(ScalarArgs a, double* sc) {
+  if (threadIdx.x != 0) return;
+  const double eta = sc[SC_ETA];
+  const double primal = eta * sc[SC_PB] + sc[SC_OBJF];
+  const double trX = eta * sc[SC_TRXB] + sc[SC_TRS];
+  double lam0 = -a.lanczos_alpha[0];
+  sc[SC_LAM0] = lam0;
+  const double lp = lam0 < 0.0 ? 0.0 : lam0;
+  const double cand_pen = sc[SC_BYC] + a.tu * lp;
+  double ub = sc[SC_BYC] + (a.tu / a.tf) * lp;
+  ub = ub > sc[SC_OBJUB] ? sc[SC_OBJUB] : ub;
+  const double pen = sc[SC_PEN];
+  const bool descent = a.beta * (pen - sc[SC_LB]) <= pen - cand_pen;
+  sc[SC_DESCENT] = descent ? 1.0 : 0.0;
+  sc[SC_PEN] = descent ? cand_pen : pen;
+  sc[SC_CANDPEN] = cand_pen;
+  sc[SC_OBJUB] = ub;
+  sc[SC_P] = primal;
+  sc[SC_TRX] = trX;
+  sc[SC_TRXB] = eta * sc[SC_TRXB] + sc[SC_SUMLAM];
+  sc[SC_PB] = eta * sc[SC_PB] + sc[SC_OBJFC];
+  const double scx = a.SC * a.SX;
+  const double ap = fabs(primal), au = fabs(ub);
+  double gap = fabs(ub - primal) / scx;
+  gap /= 1.0 + (ap < au ? au : ap) / scx;
+  sc[SC_GAP] = gap;
+  sc[SC_INFGAP] = (sqrt(sc[SC_INF2]) / a.SX) / (1.0 + sc[SC_BNORM]);
+  sc[SC_MAXINF] = sc[SC_INFMAX] / a.SX;
+  sc[SC_T] += 1.0;
+}
+
+int launch_scalars(const ScalarArgs& a, double* sc, hipStream_t s) {
+  k_scalars<<<1, 64, 0, s>>>(a, sc);
+  return (int)hipGetLastError();
+}
+
+}  // namespace usbs

@@ -8,9 +8,6 @@
 
 #include "../../include/usbs_hip.h"
 
-struct usbs_ctx {
-  usbs::Ctx c;
-};
 
 namespace usbs {
 
@@ -163,8 +160,8 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
 
 // Lanczos workspace for one instance of dimension n, basis m, storage T.
 template <typename T>
-static int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int nd,
-                        int max_restarts, double tol, const T* opval, LzInst<T>& L) {
+int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int nd,
+                 int max_restarts, double tol, const T* opval, LzInst<T>& L) {
   L = LzInst<T>();
   L.n = P.n;
   L.tile = P.tile;
@@ -194,6 +191,11 @@ static int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int
   if (!(L.bar = (unsigned*)ctx_alloc(c, pool, sizeof(unsigned) * 64))) return USBS_ERR_OOM;
   return USBS_OK;
 }
+
+template int make_lanczos<double>(Ctx*, std::vector<DevBuf>&, const Problem&, int, int, int, double,
+                                  const double*, LzInst<double>&);
+template int make_lanczos<float>(Ctx*, std::vector<DevBuf>&, const Problem&, int, int, int, double,
+                                 const float*, LzInst<float>&);
 
 // Reset per-call Lanczos state: Q = [q0, 0...], alpha = beta = 0, ctrl = 0.
 template <typename T>

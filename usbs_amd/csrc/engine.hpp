@@ -68,9 +68,18 @@ int set_err(Ctx* c, int code, const std::string& msg);
 void* ctx_alloc(Ctx* c, std::vector<DevBuf>& pool, size_t bytes);
 void ctx_free_pool(std::vector<DevBuf>& pool);
 
+template <typename T>
+int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int nd,
+                 int max_restarts, double tol, const T* opval, LzInst<T>& L);
+
 // kernels in ops.hip
 int launch_assemble(const Problem& P, double coefC, const double* z1, double a1, const double* z2,
                     double a2, const double* z3, double a3, double* out64, float* out32,
                     hipStream_t s);
 
 }  // namespace usbs
+
+// The opaque handle of the C ABI.
+struct usbs_ctx {
+  usbs::Ctx c;
+};

@@ -35,7 +35,8 @@ static inline int lds_doubles_for(int m, int maxc) {
 // ----------------------------------------------------------------------------
 // tile phases
 
-// mode 0: w = Op q_i (SpMV); mode 1: w = counter normals (breakdown restart).
+// mode 0: w = Op q_i (SpMV); mode 1: w = counter normals (breakdown restart);
+// mode 2: w = Q[:, i+1] (orthonormalise an existing column against 0..i).
 // Then part1[tile] = Q[:, 0..i]^T w.
 template <typename T, int MAXC>
 __device__ void tile_phaseA(const LzInst<T>& L, int tl, int i, int mode, double* lds) {
@@ -50,8 +51,10 @@ __device__ void tile_phaseA(const LzInst<T>& L, int tl, int i, int mode, double*
     if (mode == 0) {
       const int p0 = L.row_ptr[r], p1 = L.row_ptr[r + 1];
       for (int p = p0; p < p1; ++p) s += (double)L.val[p] * (double)x[L.col[p]];
-    } else {
+    } else if (mode == 1) {
       s = counter_normal((uint64_t)(i + 1), (uint64_t)r);
+    } else {
+      s = (double)L.Q[r + (long long)(i + 1) * L.ldq];
     }
     const T wv = (T)s;
     L.w[r] = wv;
@@ -232,9 +235,9 @@ __device__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alp
 // per-op kernels (one block per tile)
 
 template <typename T, int MAXC>
-__global__ void __launch_bounds__(USBS_NT) k_lz_phaseA(LzInst<T> L, int i) {
+__global__ void __launch_bounds__(USBS_NT) k_lz_phaseA(LzInst<T> L, int i, int mode) {
   extern __shared__ double lds[];
-  tile_phaseA<T, MAXC>(L, blockIdx.x, i, 0, lds);
+  tile_phaseA<T, MAXC>(L, blockIdx.x, i, mode, lds);
 }
 
 template <typename T, int MAXC>
@@ -304,7 +307,7 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
   int k = 0;
   for (int restart = 0; restart < L.max_restarts; ++restart) {
     for (int i = k; i < L.m; ++i) {
-      k_lz_phaseA<T, MAXC><<<g, b, lds, s>>>(L, i);
+      k_lz_phaseA<T, MAXC><<<g, b, lds, s>>>(L, i, 0);
       k_lz_phaseB<T, MAXC><<<g, b, lds, s>>>(L, i);
       k_lz_phaseC<T, MAXC><<<g, b, lds, s>>>(L, i);
       k_lz_phaseD<T><<<g, b, lds, s>>>(L, i);
@@ -322,6 +325,32 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
   }
   return (int)hipGetLastError();
 }
+
+// Column-by-column CGS2 orthonormalisation of Q[:, 1..m] (Q[:, 0] = 0):
+// step i orthonormalises column i+1 against columns 0..i with the Lanczos
+// step kernels in mode 2 (a breakdown draws a counter-based random column).
+template <typename T, int MAXC>
+static int orth_impl(const LzInst<T>& L, hipStream_t s) {
+  const size_t lds = sizeof(double) * lds_doubles_for(L.m, MAXC);
+  const dim3 g(L.ntiles), b(USBS_NT);
+  for (int i = 0; i < L.m; ++i) {
+    k_lz_phaseA<T, MAXC><<<g, b, lds, s>>>(L, i, 2);
+    k_lz_phaseB<T, MAXC><<<g, b, lds, s>>>(L, i);
+    k_lz_phaseC<T, MAXC><<<g, b, lds, s>>>(L, i);
+    k_lz_phaseD<T><<<g, b, lds, s>>>(L, i);
+    k_lz_breakdown<T, MAXC><<<1, b, lds, s>>>(L, i);
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+int orth_perop(const LzInst<T>& L, hipStream_t s) {
+  if (L.m + 1 <= 17) return orth_impl<T, 17>(L, s);
+  if (L.m + 1 <= 33) return orth_impl<T, 33>(L, s);
+  return (int)hipErrorInvalidValue;
+}
+template int orth_perop<double>(const LzInst<double>&, hipStream_t);
+template int orth_perop<float>(const LzInst<float>&, hipStream_t);
 
 template <typename T>
 int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {

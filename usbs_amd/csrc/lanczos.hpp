@@ -52,5 +52,7 @@ template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, const int* host_nblocks, int m,
                        hipStream_t s, int* dev_wgmap, unsigned long long spin_limit);
 int lanczos_max_coresident_blocks(int m, int device);
+template <typename T>
+int orth_perop(const LzInst<T>& L, hipStream_t s);
 
 }  // namespace usbs

@@ -1,0 +1,543 @@
+// The USBS outer loop (solver/usbs.py:436-789) driven from the host with all
+// state resident on the device.  One body = the kernel sequence below; the
+// host reads the scalar block back only at condition checks (every body, or
+// on the geometric schedule of utils/loop.py:85-98).
+//
+//   dots3 / assemble+SpMM+Gram (h_1, q_12) / Q_base Gram   IPM inputs
+//   IPM (one workgroup) + eigh(S) -> eta, S, F = V E sqrt(lam)
+//   [upsilon alternation for inequality rows]
+//   constraint update: z, zbar_next, ycand, b.ycand, zbar.ycand, infeasibility
+//   sketch F^T Omega + P/Pbar update;  objective tr(F^T C F)
+//   assemble -C + A*(ycand) -> Lanczos (per-op or persistent)
+//   SpMM+Gram (g_1) -> lower-bound IPM -> scalar bookkeeping -> y select
+//   V_next = orth([V E_S[:, kc:], cand_vecs])
+//   condition: Nystrom rounding (max-cut) + gaps
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "../../include/usbs_hip.h"
+#include "solve.hpp"
+
+namespace usbs {
+
+template <typename T>
+__global__ void k_load_colmajor(long long n, int r, const double* src, T* dst, long long ld) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * r) return;
+  const long long row = i / r, c = i % r;
+  dst[row + c * ld] = (T)src[i];
+}
+
+template <typename T>
+__global__ void k_store_rowmajor(long long n, int r, const T* src, long long ld, double* dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * r) return;
+  const long long row = i / r, c = i % r;
+  dst[i] = (double)src[row + c * ld];
+}
+
+template <typename T>
+__global__ void k_fill(T* p, long long count, T v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// Lanczos per-call reset: Q[:, 0] = q0, alpha = beta = 0, control and barrier 0.
+template <typename T>
+__global__ void k_lz_start(LzInst<T> L, const double* q0) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < L.n) L.Q[i] = (T)q0[i];
+  if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
+  if (i < LZ_NCTRL) L.ctrl[i] = 0;
+  if (i < 64) L.bar[i] = 0u;
+}
+
+// Penalised dual objective of the initial point (solver/usbs.py:704-706).
+__global__ void k_init_pen(double* sc, const double* by4, const double* alpha, double tu) {
+  if (threadIdx.x != 0) return;
+  const double lam = -alpha[0];
+  const double pen = by4[1] + tu * (lam < 0.0 ? 0.0 : lam);
+  sc[SC_PEN] = pen;
+  sc[SC_CANDPEN] = pen;
+  sc[SC_BY] = by4[1];
+}
+
+__global__ void k_set_by(double* sc, const double* by4) {
+  if (threadIdx.x == 0) sc[SC_BY] = by4[1];
+}
+
+__global__ void k_copy1(double* dst, const double* src) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+template <typename T>
+struct Solver {
+  Ctx* c = nullptr;
+  const Problem* P = nullptr;
+  usbs_params p{};
+  hipStream_t s = nullptr;
+  int n = 0, m = 0, k = 0, kc = 0, kp = 0, r = 0, d = 0, mL = 0;
+  long long ld = 0;
+  std::vector<DevBuf> pool;
+  double *y = nullptr, *z = nullptr, *zbar = nullptr, *zbn = nullptr, *ycand = nullptr,
+         *ups = nullptr, *tmp4 = nullptr;
+  T *Qa = nullptr, *Qb = nullptr, *F = nullptr, *PO = nullptr, *Pbar = nullptr;
+  double* W = nullptr;
+  double* op64 = nullptr;
+  float* op32 = nullptr;
+  double *partial = nullptr, *dpart = nullptr;
+  long long* ipart = nullptr;
+  double *Gq = nullptr, *Gz = nullptr, *Gh = nullptr, *Glb = nullptr, *FO = nullptr, *GF = nullptr,
+         *Epast = nullptr, *S_io = nullptr, *eta_io = nullptr, *PtPO = nullptr, *OtO = nullptr,
+         *Gn = nullptr, *lam = nullptr, *cut = nullptr, *sc = nullptr, *q0 = nullptr;
+  signed char* sg = nullptr;
+  LzInst<T> lz{}, qr{};
+  LzInst<T>* d_lz_k = nullptr;   // device copies for the persistent form (nd = k, nd = kc)
+  LzInst<T>* d_lz_kc = nullptr;
+  int* d_map = nullptr;
+  int nb = 1;
+  int host_ctrl[LZ_NCTRL] = {0};
+  int lz_restarts = 0;
+
+  ~Solver() { ctx_free_pool(pool); }
+
+  template <typename X>
+  X* alloc(size_t count) {
+    return (X*)ctx_alloc(c, pool, sizeof(X) * (count ? count : 1));
+  }
+  T* V() const { return Qa + ld; }      // current bundle basis = Qa[:, 1..k]
+  T* P_() const { return PO; }
+  T* Om() const { return PO + (long long)r * ld; }
+
+  int setup(Ctx* ctx, const usbs_params& prm) {
+    c = ctx;
+    P = &ctx->P;
+    p = prm;
+    s = ctx->stream;
+    n = P->n;
+    m = P->m;
+    kc = p.k_curr;
+    kp = p.k_past;
+    k = kc + kp;
+    r = p.sketch_dim;
+    d = k * (k + 1) / 2;
+    mL = p.lanczos_inner_iterations;
+    ld = ((long long)n + 63) / 64 * 64;
+    const int ntl = P->ntiles > P->mntiles ? P->ntiles : P->mntiles;
+    if (!(y = alloc<double>(m)) || !(z = alloc<double>(m)) || !(zbar = alloc<double>(m)) ||
+        !(zbn = alloc<double>(m)) || !(ycand = alloc<double>(m)) || !(ups = alloc<double>(m)) ||
+        !(tmp4 = alloc<double>(8)))
+      return USBS_ERR_OOM;
+    if (!(Qa = alloc<T>(ld * (k + 1))) || !(Qb = alloc<T>(ld * (k + 1))) || !(F = alloc<T>(ld * k)) ||
+        !(W = alloc<double>(ld * k)))
+      return USBS_ERR_OOM;
+    const int rr = r > 0 ? r : 1;
+    if (!(PO = alloc<T>(ld * 2 * rr)) || !(Pbar = alloc<T>(ld * rr))) return USBS_ERR_OOM;
+    if (!(op64 = alloc<double>(P->nnz))) return USBS_ERR_OOM;
+    if (sizeof(T) == 4 && !(op32 = alloc<float>(P->nnz))) return USBS_ERR_OOM;
+    const size_t pw = (size_t)ntl * (size_t)std::max(d * d, std::max(2 * rr * rr, std::max(k * k, 16 * 4)));
+    if (!(partial = alloc<double>(pw)) || !(dpart = alloc<double>((size_t)ntl * rr)) ||
+        !(ipart = alloc<long long>((size_t)ntl * rr)))
+      return USBS_ERR_OOM;
+    if (!(Gq = alloc<double>(d * d)) || !(Gz = alloc<double>(k * k)) || !(Gh = alloc<double>(k * k)) ||
+        !(Glb = alloc<double>(k * k)) || !(FO = alloc<double>(k * rr)) || !(GF = alloc<double>(k * k)) ||
+        !(Epast = alloc<double>(k * (kp > 0 ? kp : 1))) || !(S_io = alloc<double>(k * k)) ||
+        !(eta_io = alloc<double>(1)) || !(PtPO = alloc<double>(2 * rr * rr)) ||
+        !(OtO = alloc<double>(rr * rr)) || !(Gn = alloc<double>(rr * rr)) || !(lam = alloc<double>(rr)) ||
+        !(cut = alloc<double>(rr + 1)) || !(sc = alloc<double>(SC_COUNT)) || !(q0 = alloc<double>(n)) ||
+        !(sg = alloc<signed char>((size_t)n * rr)))
+      return USBS_ERR_OOM;
+    int rc = make_lanczos<T>(c, pool, *P, mL, k, p.lanczos_max_restarts, p.subprob_eps,
+                             sizeof(T) == 4 ? (const T*)op32 : (const T*)op64, lz);
+    if (rc) return rc;
+    rc = make_lanczos<T>(c, pool, *P, k, k, 1, 0.0, (const T*)nullptr, qr);
+    if (rc) return rc;
+    if (lz.ldq != ld || qr.ldq != ld) return set_err(c, USBS_ERR_INVALID, "leading dimension mismatch");
+    if (p.exec_mode == USBS_EXEC_PERSISTENT) {
+      nb = std::min(lz.ntiles, std::max(1, c->coresident));
+      lz.nblocks = nb;
+      if (!(d_lz_k = alloc<LzInst<T>>(1)) || !(d_lz_kc = alloc<LzInst<T>>(1)) || !(d_map = alloc<int>(2 * nb)))
+        return USBS_ERR_OOM;
+      std::vector<int2> map(nb);
+      for (int i = 0; i < nb; ++i) map[i] = make_int2(0, i);
+      LzInst<T> a = lz, b = lz;
+      a.nd = k;
+      b.nd = kc;
+      if (hipMemcpy(d_lz_k, &a, sizeof(a), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(d_lz_kc, &b, sizeof(b), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(d_map, map.data(), sizeof(int2) * nb, hipMemcpyHostToDevice) != hipSuccess)
+        return set_err(c, USBS_ERR_HIP, "upload of Lanczos descriptors failed");
+    }
+    return USBS_OK;
+  }
+
+#define CK(expr)                                                                    \
+  do {                                                                              \
+    int _rc = (expr);                                                               \
+    if (_rc) return set_err(c, USBS_ERR_HIP, std::string(#expr) + " -> " +         \
+                                                 hipGetErrorString((hipError_t)_rc)); \
+  } while (0)
+
+  int fill(double* ptr, long long count, double v) {
+    k_fill<double><<<(unsigned)((count + 255) / 256), 256, 0, s>>>(ptr, count, v);
+    return (int)hipGetLastError();
+  }
+
+  // Lanczos on the assembled operator (op64/op32 already hold -C + A*(z)).
+  int lanczos(int nd) {
+    LzInst<T> L = lz;
+    L.nd = nd;
+    k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
+    CK((int)hipGetLastError());
+    if (p.exec_mode == USBS_EXEC_PEROP) {
+      CK(lanczos_perop<T>(L, s, host_ctrl));
+      lz_restarts = host_ctrl[LZ_RESTART];
+    } else {
+      CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, &nb, mL, s, d_map, 1ull << 26));
+    }
+    return USBS_OK;
+  }
+
+  int assemble_op(double coefC, const double* z1, double a1, const double* z2, double a2,
+                  const double* z3, double a3, bool want32) {
+    return launch_assemble(*P, coefC, z1, a1, z2, a2, z3, a3, op64, want32 ? op32 : nullptr, s);
+  }
+
+  // out (k x k) = V^T M V for the operator currently in op64.
+  int project(double* out) {
+    CK((launch_spmm<double, T>(n, P->row_ptr, P->col, op64, V(), ld, k, W, ld, s)));
+    CK((launch_tsgram<T, double>(n, P->tile, P->ntiles, V(), ld, k, W, ld, k, partial, out, s)));
+    return USBS_OK;
+  }
+
+  int rounding() {
+    if (r <= 0 || !p.rounding) return USBS_OK;
+    CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, P_(), ld, r, PO, ld, 2 * r, partial, PtPO, s)));
+    CK(launch_nystrom_core(n, r, PtPO, OtO, Gn, lam, sc, s));
+    CK(launch_round<T>(*P, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, sg, ipart, dpart, cut, s));
+    k_copy1<<<1, 64, 0, s>>>(sc + SC_CB, cut + r);
+    CK((int)hipGetLastError());
+    return USBS_OK;
+  }
+
+  int body() {
+    const double tu = p.trace_ub, rho = p.rho;
+    // --- prox subproblem inputs (solver/usbs.py:53-61)
+    CK(launch_dots3(*P, zbar, y, P->has_ineq ? ups : nullptr, partial, sc + SC_ZZ, s));
+    CK(assemble_op(-1.0, y, 1.0, P->b, -1.0 / rho, P->has_ineq ? ups : nullptr, 1.0 / rho, false));
+    if (project(Gh)) return USBS_ERR_HIP;
+    CK(assemble_op(0.0, zbar, 1.0, nullptr, 0.0, nullptr, 0.0, false));
+    if (project(Gz)) return USBS_ERR_HIP;
+    CK(launch_qbase<T>(*P, k, V(), ld, partial, Gq, s));
+    IpmArgs ia{};
+    ia.k = k; ia.kc = kc; ia.max_iters = p.subprob_max_iters; ia.first = 1;
+    ia.tu = tu; ia.rho = rho; ia.eps = p.subprob_eps;
+    ia.Gq = Gq; ia.Gz = Gz; ia.Gh = Gh; ia.sc = sc; ia.eta_io = eta_io; ia.S_io = S_io;
+    ia.GF = GF; ia.Epast = Epast;
+    CK(launch_ipm_quad(ia, s));
+    CK((launch_tsmm<T, T>(n, V(), ld, k, GF, k, F, ld, 0.0, nullptr, 0, s)));
+    if (P->has_ineq) {
+      // upsilon alternation (solver/usbs.py:384-431)
+      double gap = 1.0;
+      int it = 0;
+      while (true) {
+        CK(launch_upsilon<T>(*P, k, F, ld, sc + SC_ETA, zbar, y, rho, ups, partial, sc + SC_UPSGAP, s));
+        ++it;
+        if (hipMemcpyAsync(&gap, sc + SC_UPSGAP, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+          return set_err(c, USBS_ERR_HIP, "reading upsilon gap");
+        if (!(gap > p.subprob_eps) || it >= p.subprob_max_iters) break;
+        CK(launch_dots3(*P, zbar, y, ups, partial, sc + SC_ZZ, s));
+        CK(assemble_op(-1.0, y, 1.0, P->b, -1.0 / rho, ups, 1.0 / rho, false));
+        if (project(Gh)) return USBS_ERR_HIP;
+        ia.first = 0;
+        CK(launch_ipm_quad(ia, s));
+        CK((launch_tsmm<T, T>(n, V(), ld, k, GF, k, F, ld, 0.0, nullptr, 0, s)));
+      }
+    }
+    // --- primal / sketch / dual candidate (solver/usbs.py:568-585)
+    CK(launch_constraint_update<T>(*P, k, kc, F, ld, sc + SC_ETA, zbar, y,
+                                   P->has_ineq ? ups : nullptr, rho, z, zbn, ycand, partial,
+                                   sc + SC_BYC, s));
+    if (r > 0) CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, F, ld, k, Om(), ld, r, partial, FO, s)));
+    CK(launch_sketch_update<T>(n, k, kc, r, F, ld, FO, sc + SC_ETA, Pbar, P_(), ld, s));
+    CK(launch_objective<T>(*P, k, kc, F, ld, partial, sc + SC_OBJF, s));
+    // --- candidate eigenvectors (solver/usbs.py:588-599)
+    CK(assemble_op(-1.0, ycand, 1.0, nullptr, 0.0, nullptr, 0.0, sizeof(T) == 4));
+    if (lanczos(kc)) return USBS_ERR_HIP;
+    // --- spectral lower bound with the old V (solver/usbs.py:606-621)
+    if (project(Glb)) return USBS_ERR_HIP;
+    CK(launch_ipm_lb(k, p.subprob_max_iters, tu, p.subprob_eps, Glb, sc, s));
+    ScalarArgs sa{};
+    sa.tu = tu; sa.tf = p.trace_factor; sa.beta = p.beta; sa.SC = p.SCALE_C; sa.SX = p.SCALE_X;
+    sa.lanczos_alpha = lz.alpha;
+    CK(launch_scalars(sa, sc, s));
+    CK(launch_select(m, sc + SC_DESCENT, y, ycand, s));
+    std::swap(zbar, zbn);
+    // --- V_next = qr([V E_S[:, kc:], cand_vecs]) (solver/usbs.py:640-642)
+    if (kp > 0) CK((launch_tsmm<T, T>(n, V(), ld, k, Epast, kp, Qb + ld, ld, 0.0, nullptr, 0, s)));
+    if (hipMemcpyAsync(Qb + ld * (1 + kp), lz.Q, sizeof(T) * ld * kc, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "copy of candidate eigenvectors");
+    LzInst<T> q = qr;
+    q.Q = Qb;
+    k_lz_start<T><<<1, 64, 0, s>>>(q, q0);  // resets control words only (column 0 set below)
+    CK(fill_T(Qb, ld));
+    CK(orth_perop<T>(q, s));
+    std::swap(Qa, Qb);
+    return USBS_OK;
+  }
+
+  int fill_T(T* ptr, long long count) {
+    k_fill<T><<<(unsigned)((count + 255) / 256), 256, 0, s>>>(ptr, count, (T)0);
+    return (int)hipGetLastError();
+  }
+
+  int read_record(usbs_record& rec, double t0) {
+    double h[SC_COUNT];
+    if (hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "reading the scalar block");
+    if (p.exec_mode == USBS_EXEC_PERSISTENT) {
+      int ctrl[LZ_NCTRL];
+      if (hipMemcpy(ctrl, lz.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_err(c, USBS_ERR_HIP, "reading Lanczos control");
+      if (ctrl[LZ_ERROR]) return set_err(c, USBS_ERR_TIMEOUT, "persistent Lanczos barrier wait expired");
+      lz_restarts = ctrl[LZ_RESTART];
+    }
+    const double scx = p.SCALE_C * p.SCALE_X;
+    rec.t = h[SC_T];
+    rec.time = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
+    rec.pen_dual_obj = h[SC_PEN];
+    rec.cand_pen_dual_obj = h[SC_CANDPEN];
+    rec.lb_spec_est = h[SC_LB];
+    rec.primal_obj = h[SC_P] / scx;
+    rec.dual_obj = h[SC_BY];
+    rec.obj_ub = h[SC_OBJUB];
+    rec.obj_gap = h[SC_GAP];
+    rec.infeas_gap = h[SC_INFGAP];
+    rec.max_infeas = h[SC_MAXINF];
+    rec.callback_val = (r > 0 && p.rounding) ? h[SC_CB] : NAN;
+    rec.ipm_iters = h[SC_IPM_IT];
+    rec.lb_iters = h[SC_LB_IT];
+    rec.lanczos_restarts = lz_restarts;
+    rec.descent = h[SC_DESCENT];
+    return USBS_OK;
+  }
+#undef CK
+};
+
+static std::vector<long long> geometric_schedule(long long maxiter, double base) {
+  // utils/loop.py:85-98
+  const double num = std::ceil(std::log((base - 1) * maxiter - 1) / std::log(base)) + 1;
+  std::vector<long long> steps;
+  long long total = 0;
+  for (long long i = 0; i < (long long)num; ++i) {
+    const long long st = (long long)std::pow(base, (double)i);
+    steps.push_back(st);
+  }
+  std::vector<long long> kept;
+  long long cum = 0;
+  for (long long st : steps) {
+    cum += st;
+    if (cum < maxiter) { kept.push_back(st); total += st; }
+  }
+  kept.push_back(maxiter - total);
+  return kept;
+}
+
+template <typename T>
+static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const double* z0, double trX0,
+                     double pobj0, const double* Omega, const double* P0, const double* q0h,
+                     double* y_out, double* z_out, double* P_out, double* scal_out,
+                     usbs_record* log, int log_cap, int* log_len) {
+  Solver<T> S;
+  int rc = S.setup(c, p);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  const int n = S.n, m = S.m, r = S.r;
+  const long long ld = S.ld;
+  std::vector<DevBuf> tmp;
+  struct G { std::vector<DevBuf>& t; ~G() { ctx_free_pool(t); } } guard{tmp};
+  auto up = [&](const double* h, size_t cnt) -> double* {
+    double* dptr = (double*)ctx_alloc(c, tmp, sizeof(double) * cnt);
+    if (dptr && hipMemcpy(dptr, h, sizeof(double) * cnt, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return dptr;
+  };
+  if (hipMemcpy(S.y, y0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(S.z, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(S.zbar, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(S.q0, q0h, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(c, USBS_ERR_HIP, "upload of the start point");
+  S.fill(S.ups, m, 0.0);
+  S.fill_T(S.Qa, ld * (S.k + 1));
+  S.fill_T(S.Qb, ld * (S.k + 1));
+  S.fill_T(S.PO, ld * 2 * (r > 0 ? r : 1));
+  if (r > 0) {
+    double* d_om = up(Omega, (size_t)n * r);
+    if (!d_om) return USBS_ERR_OOM;
+    k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_om, S.Om(), ld);
+    if (P0) {
+      double* d_p = up(P0, (size_t)n * r);
+      if (!d_p) return USBS_ERR_OOM;
+      k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_p, S.P_(), ld);
+      k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_p, S.Pbar, ld);
+    } else {
+      S.fill_T(S.Pbar, ld * r);
+    }
+    if (launch_tsgram<T, T>(n, c->P.tile, c->P.ntiles, S.Om(), ld, r, S.Om(), ld, r, S.partial, S.OtO, s))
+      return set_err(c, USBS_ERR_HIP, "Omega Gram");
+  } else {
+    if (P0) {
+      double* d_p = up(P0, (size_t)n);
+      if (!d_p) return USBS_ERR_OOM;
+      k_load_colmajor<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, 1, d_p, S.P_(), ld);
+      k_load_colmajor<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, 1, d_p, S.Pbar, ld);
+    } else {
+      S.fill_T(S.Pbar, ld);
+    }
+  }
+  // scalar block
+  std::vector<double> h(SC_COUNT, 0.0);
+  h[SC_TRX] = trX0;
+  h[SC_TRXB] = trX0;
+  h[SC_P] = pobj0;
+  h[SC_PB] = pobj0;
+  h[SC_OBJUB] = INFINITY;
+  h[SC_GAP] = 1.1 * p.obj_gap_eps;
+  h[SC_INFGAP] = 1.1 * p.infeas_gap_eps;
+  h[SC_MAXINF] = 1.1 * p.max_infeas_eps;
+  h[SC_CB] = NAN;
+  {
+    std::vector<double> bh(m), sah(m);
+    if (hipMemcpy(bh.data(), c->P.b, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(sah.data(), c->P.scale_a, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "reading b");
+    double nb2 = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double v = (bh[i] / sah[i]) / p.SCALE_X;
+      nb2 += v * v;
+    }
+    h[SC_BNORM] = std::sqrt(nb2);
+  }
+  if (hipMemcpy(S.sc, h.data(), sizeof(double) * SC_COUNT, hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(c, USBS_ERR_HIP, "upload of the scalar block");
+
+  const double t_origin =
+      std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  // initial bundle: k smallest eigenpairs of -C + A*(y0) (solver/usbs.py:693-706)
+  if (launch_assemble(c->P, -1.0, S.y, 1.0, nullptr, 0.0, nullptr, 0.0, S.op64,
+                      sizeof(T) == 4 ? S.op32 : nullptr, s))
+    return set_err(c, USBS_ERR_HIP, "assemble");
+  rc = S.lanczos(S.k);
+  if (rc) return rc;
+  if (hipMemcpyAsync(S.Qa + ld, S.lz.Q, sizeof(T) * ld * S.k, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return set_err(c, USBS_ERR_HIP, "copy of the initial bundle");
+  if (launch_dots3(c->P, S.y, c->P.b, nullptr, S.partial, S.tmp4, s)) return set_err(c, USBS_ERR_HIP, "dots");
+  k_init_pen<<<1, 64, 0, s>>>(S.sc, S.tmp4, S.lz.alpha, p.trace_ub);
+
+  int nlog = 0;
+  auto cond = [&](bool& go) -> int {
+    int rc2 = S.rounding();
+    if (rc2) return rc2;
+    if (launch_dots3(c->P, S.y, c->P.b, nullptr, S.partial, S.tmp4, s)) return set_err(c, USBS_ERR_HIP, "dots");
+    k_set_by<<<1, 64, 0, s>>>(S.sc, S.tmp4);
+    usbs_record rec{};
+    rc2 = S.read_record(rec, t_origin);
+    if (rc2) return rc2;
+    if (log && nlog < log_cap) log[nlog] = rec;
+    ++nlog;
+    go = (rec.t == 0.0) ||
+         (rec.time < p.max_time && (rec.obj_gap > p.obj_gap_eps || rec.infeas_gap > p.infeas_gap_eps ||
+                                    rec.max_infeas > p.max_infeas_eps));
+    return USBS_OK;
+  };
+  bool go = true;
+  rc = cond(go);
+  if (rc) return rc;
+  long long iters = 0;
+  if (p.cond_exp_base > 1.0) {
+    for (long long steps : geometric_schedule(p.max_iters, p.cond_exp_base)) {
+      if (!go) break;
+      for (long long i = 0; i < steps; ++i) {
+        rc = S.body();
+        if (rc) return rc;
+        ++iters;
+      }
+      rc = cond(go);
+      if (rc) return rc;
+    }
+  } else {
+    for (long long it = 0; it < p.max_iters && go; ++it) {
+      rc = S.body();
+      if (rc) return rc;
+      ++iters;
+      rc = cond(go);
+      if (rc) return rc;
+    }
+  }
+  if (log_len) *log_len = nlog;
+  // outputs
+  double* d_out = (double*)ctx_alloc(c, tmp, sizeof(double) * (size_t)n * (r > 0 ? r : 1));
+  if (!d_out) return USBS_ERR_OOM;
+  const int rr = r > 0 ? r : 1;
+  k_store_rowmajor<T><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.P_(), ld, d_out);
+  std::vector<double> hs(SC_COUNT);
+  if ((y_out && hipMemcpyAsync(y_out, S.y, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (z_out && hipMemcpyAsync(z_out, S.z, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (P_out && hipMemcpyAsync(P_out, d_out, sizeof(double) * (size_t)n * rr, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipMemcpyAsync(hs.data(), S.sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return set_err(c, USBS_ERR_HIP, "reading results");
+  if (scal_out) {
+    scal_out[0] = hs[SC_TRX];
+    scal_out[1] = hs[SC_P];
+    scal_out[2] = (double)iters;
+    scal_out[3] = hs[SC_GAP];
+    scal_out[4] = hs[SC_INFGAP];
+    scal_out[5] = hs[SC_MAXINF];
+  }
+  return USBS_OK;
+}
+
+}  // namespace usbs
+
+using namespace usbs;
+
+extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0, const double* z0,
+                          double tr_X0, double primal_obj0, const double* Omega, const double* P0,
+                          const double* q0, double* y_out, double* z_out, double* P_out,
+                          double* scal_out, usbs_record* log, int log_cap, int* log_len) {
+  if (!h || !prm) return USBS_ERR_INVALID;
+  Ctx* c = &h->c;
+  if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
+  const usbs_params& p = *prm;
+  if (!y0 || !z0 || !q0) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
+  const int k = p.k_curr + p.k_past;
+  if (p.k_curr < 1 || p.k_past < 0 || k > 11)
+    return set_err(c, USBS_ERR_INVALID, "need k_curr >= 1, k_past >= 0, k_curr + k_past <= 11");
+  if (p.lanczos_inner_iterations < k || p.lanczos_inner_iterations > 32 ||
+      p.lanczos_inner_iterations > c->P.n)
+    return set_err(c, USBS_ERR_INVALID, "lanczos_inner_iterations must be in [k, min(n, 32)]");
+  if (p.lanczos_max_restarts < 1 || p.subprob_max_iters < 1 || p.max_iters < 0)
+    return set_err(c, USBS_ERR_INVALID, "iteration limits must be positive");
+  if (p.sketch_dim < 0 || p.sketch_dim > 48) return set_err(c, USBS_ERR_INVALID, "sketch_dim in [0, 48]");
+  if (p.sketch_dim > 0 && !Omega) return set_err(c, USBS_ERR_INVALID, "Omega required when sketch_dim > 0");
+  if (p.rounding && p.sketch_dim == 0) return set_err(c, USBS_ERR_INVALID, "rounding needs a sketch");
+  if (!(p.cond_exp_base >= 1.0 && p.cond_exp_base <= 2.0))
+    return set_err(c, USBS_ERR_INVALID, "cond_exp_base must be in [1, 2]");
+  if (!(p.rho > 0.0) || !(p.trace_ub > 0.0) || !(p.trace_factor > 0.0))
+    return set_err(c, USBS_ERR_INVALID, "rho, trace_ub and trace_factor must be positive");
+  if (p.precision != USBS_PREC_FP64 && p.precision != USBS_PREC_MIXED)
+    return set_err(c, USBS_ERR_INVALID, "unknown precision");
+  if (p.exec_mode != USBS_EXEC_PEROP && p.exec_mode != USBS_EXEC_PERSISTENT)
+    return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
+  for (int i = 0; i < c->P.m; ++i)
+    if (!std::isfinite(y0[i]) || !std::isfinite(z0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite y0/z0");
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(c, USBS_ERR_HIP, "hipSetDevice");
+  if (p.precision == USBS_PREC_FP64)
+    return run_solve<double>(c, p, y0, z0, tr_X0, primal_obj0, Omega, P0, q0, y_out, z_out, P_out,
+                             scal_out, log, log_cap, log_len);
+  return run_solve<float>(c, p, y0, z0, tr_X0, primal_obj0, Omega, P0, q0, y_out, z_out, P_out,
+                          scal_out, log, log_cap, log_len);
+}

@@ -1,0 +1,119 @@
+"""usbs() with the reference's signature, run by the HIP library.
+
+Mirrors solver/usbs.py:436-789.  The whole loop (prox-bundle IPM, sketch,
+objective, Lanczos, lower bound, descent test, QR of the bundle, gaps,
+condition checks and the max-cut rounding callback) runs on the device; the
+host only reads one record per condition check.  Extra keyword arguments
+after the reference's select the execution:
+
+  backend   "hip" (the only backend in this package)
+  precision "fp64" (plain FP64) or "mixed" (FP32 Lanczos / sketch storage,
+            FP64 accumulation, Ritz solve, IPM, duals and convergence test)
+  exec_mode "perop" (a kernel per Lanczos phase) or "persistent" (one
+            cooperative launch per Lanczos call)
+  q0        Lanczos start vector (default: seeded normal, normalised; the
+            reference draws it from jax.random with PRNGKey(0))
+  log       a list that receives one dict per condition check
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib, problems
+from .common import SDPState
+from .sparse import coo_parts
+
+
+class usbs_params(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int32) for name in
+                ("k_curr", "k_past", "max_iters", "lanczos_inner_iterations",
+                 "lanczos_max_restarts", "subprob_max_iters", "sketch_dim", "rounding",
+                 "precision", "exec_mode")] + \
+               [(name, ctypes.c_double) for name in
+                ("trace_ub", "trace_factor", "rho", "beta", "max_time", "obj_gap_eps",
+                 "infeas_gap_eps", "max_infeas_eps", "subprob_eps", "cond_exp_base", "SCALE_C",
+                 "SCALE_X")]
+
+
+RECORD_FIELDS = ("t", "time", "pen_dual_obj", "cand_pen_dual_obj", "lb_spec_est", "primal_obj",
+                 "dual_obj", "obj_ub", "obj_gap", "infeas_gap", "max_infeas", "callback_val",
+                 "ipm_iters", "lb_iters", "lanczos_restarts", "descent")
+
+
+class usbs_record(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_double) for f in RECORD_FIELDS]
+
+
+_lib.EXTRA_SIGNATURES["usbs_solve"] = [
+    _lib._vp, ctypes.POINTER(usbs_params), _lib._dp, _lib._dp, ctypes.c_double, ctypes.c_double,
+    _lib._dp, _lib._dp, _lib._dp, _lib._dp, _lib._dp, _lib._dp, _lib._dp,
+    ctypes.POINTER(usbs_record), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+
+
+def _finite(x, default):
+    return float(x) if np.isfinite(x) else default
+
+
+def upload_problem(ctx, sdp_state, n, m):
+    rows, cols, vals, shape = coo_parts(sdp_state.C)
+    ctx.set_problem(n, m, rows, cols, vals, sdp_state.A_indices, sdp_state.A_data, sdp_state.b,
+                    sdp_state.b_ineq_mask, np.broadcast_to(np.asarray(sdp_state.SCALE_A, float), (m,)))
+
+
+def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max_iters, max_time,
+         obj_gap_eps, infeas_gap_eps, max_infeas_eps, lanczos_inner_iterations,
+         lanczos_max_restarts, subprob_eps, subprob_max_iters, cond_exp_base, callback_fn,
+         callback_static_args, callback_nonstatic_args, backend="hip", precision="fp64",
+         exec_mode="perop", q0=None, log=None, device=0, context=None, upload=True):
+    """Run USBS on the GPU and return the final SDPState (scaled units)."""
+    if backend != "hip":
+        raise ValueError("backend=%r is not available in usbs_amd (only 'hip')" % (backend,))
+    from .maxcut_helpers import compute_max_cut
+    if sdp_state.X is not None:
+        raise NotImplementedError("dense-X storage is not provided by the HIP backend")
+    rounding = 0
+    if callback_fn is not None and sdp_state.Omega is not None:
+        if callback_fn is not compute_max_cut:
+            raise NotImplementedError("only the max-cut rounding callback runs natively")
+        rounding = 1
+    ctx = context if context is not None else _lib.Context(device)
+    if upload:
+        upload_problem(ctx, sdp_state, n, m)
+    r = 0 if sdp_state.Omega is None else int(np.asarray(sdp_state.Omega).shape[1])
+    prm = usbs_params(k_curr=k_curr, k_past=k_past, max_iters=int(max_iters),
+                      lanczos_inner_iterations=lanczos_inner_iterations,
+                      lanczos_max_restarts=lanczos_max_restarts, subprob_max_iters=subprob_max_iters,
+                      sketch_dim=r, rounding=rounding, precision=_lib.PRECISION[precision],
+                      exec_mode=_lib.EXEC_MODE[exec_mode], trace_ub=trace_ub,
+                      trace_factor=trace_factor, rho=rho, beta=beta,
+                      max_time=_finite(max_time, 1e300), obj_gap_eps=_finite(obj_gap_eps, -1e300),
+                      infeas_gap_eps=_finite(infeas_gap_eps, -1e300),
+                      max_infeas_eps=_finite(max_infeas_eps, -1e300), subprob_eps=subprob_eps,
+                      cond_exp_base=cond_exp_base, SCALE_C=float(sdp_state.SCALE_C),
+                      SCALE_X=float(sdp_state.SCALE_X))
+    y0 = _lib.as_f64(sdp_state.y)
+    z0 = _lib.as_f64(sdp_state.z)
+    if q0 is None:
+        q0 = problems.unit_start_vector(n, 0)
+    q0 = _lib.as_f64(q0)
+    Om = None if r == 0 else _lib.as_f64(sdp_state.Omega)
+    P0 = None if sdp_state.P is None else _lib.as_f64(sdp_state.P)
+    y_out, z_out = np.zeros(m), np.zeros(m)
+    P_out = np.zeros((n, r)) if r > 0 else np.zeros(n)
+    scal = np.zeros(6)
+    cap = 1 << 16
+    recs = (usbs_record * cap)()
+    nlog = ctypes.c_int(0)
+    ctx.check(_lib.lib().usbs_solve(ctx.h, ctypes.byref(prm), _lib.dptr(y0), _lib.dptr(z0),
+                                    float(sdp_state.tr_X), float(sdp_state.primal_obj),
+                                    _lib.dptr(Om), _lib.dptr(P0), _lib.dptr(q0), _lib.dptr(y_out),
+                                    _lib.dptr(z_out), _lib.dptr(P_out), _lib.dptr(scal), recs, cap,
+                                    ctypes.byref(nlog)))
+    if log is not None:
+        for i in range(min(nlog.value, cap)):
+            log.append({f: getattr(recs[i], f) for f in RECORD_FIELDS})
+    P_ret = P_out if sdp_state.P is not None or r > 0 else None
+    return SDPState(C=sdp_state.C, A_indices=sdp_state.A_indices, A_data=sdp_state.A_data,
+                    b=sdp_state.b, b_ineq_mask=sdp_state.b_ineq_mask, X=None, P=P_ret,
+                    Omega=sdp_state.Omega, y=y_out, z=z_out, tr_X=scal[0], primal_obj=scal[1],
+                    SCALE_C=sdp_state.SCALE_C, SCALE_X=sdp_state.SCALE_X, SCALE_A=sdp_state.SCALE_A)
