@@ -12,93 +12,102 @@ namespace usbs {
 
 // ---------------------------------------------------------------------------
 // Symmetric eigendecomposition by parallel cyclic Jacobi (round-robin
-// ordering of disjoint rotation pairs).  A (m x m) is destroyed.  On exit
-// w[0..m) holds the eigenvalues in ascending order and column j of V the
-// eigenvector of w[j].  tmp: m*m doubles; coef: 2*m doubles; part: m ints.
-// Returns the number of sweeps used.
+// ordering of disjoint rotation pairs; a rotation is skipped when
+// |a_pq| <= 1e-17 sqrt|a_pp a_qq|, and the sweeps stop after a sweep that
+// rotates nothing).  A (m x m, m <= 64) is destroyed.  On exit w[0..m) holds
+// the eigenvalues in ascending order and column j of V the eigenvector of
+// w[j].  tmp: m*m doubles; coef: 2*m doubles; part: m + 2 ints.  Returns
+// the number of sweeps.
 __device__ int dense_eigh(double* A, int m, double* w, double* V, double* tmp, double* coef,
                           int* part, double* red) {
   const int t = threadIdx.x, nt = blockDim.x;
-  for (int idx = t; idx < m * m; idx += nt) V[idx] = (idx / m == idx % m) ? 1.0 : 0.0;
-  __syncthreads();
+  const int sh = (m <= 32) ? 5 : 6;          // threads per row of the 2-D loops
+  const int tx = t & ((1 << sh) - 1), ty = t >> sh, ny = nt >> sh;
+  int* nrot = part + m;                      // rotations in the current sweep
+  for (int i = ty; i < m; i += ny)
+    for (int j = tx; j < m; j += (1 << sh)) V[i * m + j] = (i == j) ? 1.0 : 0.0;
   const int M = m + (m & 1);
+  const int half = M / 2;
+  double* Ac = A;
+  double* An = tmp;
   int sweep = 0;
-  for (; sweep < 60 && m > 1; ++sweep) {
-    // convergence test on entry to the sweep
-    double off = 0.0, dg = 0.0;
-    for (int idx = t; idx < m * m; idx += nt) {
-      const int i = idx / m, j = idx % m;
-      const double v = A[idx] * A[idx];
-      if (i == j) dg += v; else off += v;
-    }
-    off = block_sum1(off, red);
-    dg = block_sum1(dg, red);
-    if (off <= 1e-34 * dg || off == 0.0 || !(off == off)) break;
+  __syncthreads();
+  for (; sweep < 40 && m > 1; ++sweep) {
+    if (t == 0) nrot[0] = 0;
+    __syncthreads();
     for (int r = 0; r < M - 1; ++r) {
-      for (int kp = t; kp < M / 2; kp += nt) {
-        int pa = (kp == 0) ? 0 : 1 + ((kp - 1 + r) % (M - 1));
-        int pb = 1 + ((M - 2 - kp + r) % (M - 1));
-        int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
+      for (int kp = t; kp < half; kp += nt) {
+        const int pa = (kp == 0) ? 0 : 1 + ((kp - 1 + r) % (M - 1));
+        const int pb = 1 + ((M - 2 - kp + r) % (M - 1));
+        const int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
         if (q >= m) {
-          if (p < m) { part[p] = p; coef[2 * p] = 1.0; coef[2 * p + 1] = 0.0; }
+          part[p] = p; coef[2 * p] = 1.0; coef[2 * p + 1] = 0.0;
           continue;
         }
-        const double apq = A[p * m + q], app = A[p * m + p], aqq = A[q * m + q];
+        const double apq = Ac[p * m + q], app = Ac[p * m + p], aqq = Ac[q * m + q];
         double c = 1.0, s = 0.0;
-        if (fabs(apq) > 1e-300) {
+        if (apq != 0.0 && fabs(apq) > 1e-17 * sqrt(fabs(app) * fabs(aqq))) {
           const double theta = (aqq - app) / (2.0 * apq);
           const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
           c = 1.0 / sqrt(tt * tt + 1.0);
           s = tt * c;
+          atomicAdd(nrot, 1);
         }
         part[p] = q; part[q] = p;
         coef[2 * p] = c; coef[2 * p + 1] = -s;
         coef[2 * q] = c; coef[2 * q + 1] = s;
       }
       __syncthreads();
-      for (int idx = t; idx < m * m; idx += nt) {
-        const int i = idx / m, j = idx % m, pi = part[i], pj = part[j];
-        const double ai = coef[2 * i], bi = coef[2 * i + 1], aj = coef[2 * j], bj = coef[2 * j + 1];
-        double v = ai * aj * A[i * m + j] + ai * bj * A[i * m + pj] + bi * aj * A[pi * m + j] +
-                   bi * bj * A[pi * m + pj];
-        if (pi == j && i != j) v = 0.0;
-        tmp[idx] = v;
+      for (int i = ty; i < m; i += ny) {
+        const int pi = part[i];
+        const double ai = coef[2 * i], bi = coef[2 * i + 1];
+        for (int j = tx; j < m; j += (1 << sh)) {
+          const int pj = part[j];
+          const double aj = coef[2 * j], bj = coef[2 * j + 1];
+          double v = ai * aj * Ac[i * m + j] + ai * bj * Ac[i * m + pj] + bi * aj * Ac[pi * m + j] +
+                     bi * bj * Ac[pi * m + pj];
+          if (pi == j && i != j) v = 0.0;
+          An[i * m + j] = v;
+        }
       }
       // V <- V J, in place per (row, pair)
-      for (int idx = t; idx < m * (M / 2); idx += nt) {
-        const int i = idx / (M / 2), kp = idx % (M / 2);
-        int pa = (kp == 0) ? 0 : 1 + ((kp - 1 + r) % (M - 1));
-        int pb = 1 + ((M - 2 - kp + r) % (M - 1));
-        int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
-        if (q >= m) continue;
-        const double vp = V[i * m + p], vq = V[i * m + q];
-        V[i * m + p] = coef[2 * p] * vp + coef[2 * p + 1] * vq;
-        V[i * m + q] = coef[2 * q] * vq + coef[2 * q + 1] * vp;
+      for (int i = ty; i < m; i += ny) {
+        for (int kp = tx; kp < half; kp += (1 << sh)) {
+          const int pa = (kp == 0) ? 0 : 1 + ((kp - 1 + r) % (M - 1));
+          const int pb = 1 + ((M - 2 - kp + r) % (M - 1));
+          const int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
+          if (q >= m) continue;
+          const double vp = V[i * m + p], vq = V[i * m + q];
+          V[i * m + p] = coef[2 * p] * vp + coef[2 * p + 1] * vq;
+          V[i * m + q] = coef[2 * q] * vq + coef[2 * q + 1] * vp;
+        }
       }
       __syncthreads();
-      for (int idx = t; idx < m * m; idx += nt) A[idx] = tmp[idx];
-      __syncthreads();
+      double* sw = Ac; Ac = An; An = sw;
     }
+    const int done = (nrot[0] == 0);
+    __syncthreads();
+    if (done) break;
   }
-  // sort ascending (stable in index), permute eigenvector columns via tmp
+  // sort ascending (stable in index), permute eigenvector columns via An
   for (int i = t; i < m; i += nt) {
-    const double di = A[i * m + i];
+    const double di = Ac[i * m + i];
     int rank = 0;
     for (int j = 0; j < m; ++j) {
-      const double dj = A[j * m + j];
+      const double dj = Ac[j * m + j];
       rank += (dj < di) || (dj == di && j < i);
     }
     part[i] = rank;
   }
   __syncthreads();
-  for (int i = t; i < m; i += nt) w[part[i]] = A[i * m + i];
-  for (int idx = t; idx < m * m; idx += nt) {
-    const int r = idx / m, c = idx % m;
-    tmp[r * m + part[c]] = V[idx];
-  }
+  for (int i = t; i < m; i += nt) w[part[i]] = Ac[i * m + i];
+  for (int i = ty; i < m; i += ny)
+    for (int j = tx; j < m; j += (1 << sh)) An[i * m + part[j]] = V[i * m + j];
   __syncthreads();
-  for (int idx = t; idx < m * m; idx += nt) V[idx] = tmp[idx];
+  for (int i = ty; i < m; i += ny)
+    for (int j = tx; j < m; j += (1 << sh)) V[i * m + j] = An[i * m + j];
   __syncthreads();
+  (void)red;
   return sweep;
 }
 
@@ -133,32 +142,26 @@ __device__ void dense_lu_solve(double* A, int d, double* B, int nrhs, double* re
       for (int j = t; j < nrhs; j += nt) {
         const double a = B[k * nrhs + j]; B[k * nrhs + j] = B[piv * nrhs + j]; B[piv * nrhs + j] = a;
       }
+      __syncthreads();
     }
-    __syncthreads();
     const double akk = A[k * d + k];
-    for (int i = k + 1 + t; i < d; i += nt) A[i * d + k] = A[i * d + k] / akk;
-    __syncthreads();
-    const int rows = d - k - 1, cols = d - k - 1 + nrhs;
-    for (int idx = t; idx < rows * cols; idx += nt) {
-      const int i = k + 1 + idx / cols, jj = idx % cols;
-      const double l = A[i * d + k];
-      if (jj < d - k - 1) {
-        const int j = k + 1 + jj;
-        A[i * d + j] -= l * A[k * d + j];
-      } else {
-        const int j = jj - (d - k - 1);
-        B[i * nrhs + j] -= l * B[k * nrhs + j];
-      }
+    // each thread eliminates its own rows below the pivot
+    for (int i = k + 1 + t; i < d; i += nt) {
+      const double l = A[i * d + k] / akk;
+      A[i * d + k] = l;
+      double* Ai = A + i * d;
+      const double* Ak = A + k * d;
+      for (int j = k + 1; j < d; ++j) Ai[j] -= l * Ak[j];
+      for (int j = 0; j < nrhs; ++j) B[i * nrhs + j] -= l * B[k * nrhs + j];
     }
     __syncthreads();
   }
   for (int k = d - 1; k >= 0; --k) {
     for (int j = t; j < nrhs; j += nt) B[k * nrhs + j] /= A[k * d + k];
     __syncthreads();
-    const int cnt = k * nrhs;
-    for (int idx = t; idx < cnt; idx += nt) {
-      const int i = idx / nrhs, j = idx % nrhs;
-      B[i * nrhs + j] -= A[i * d + k] * B[k * nrhs + j];
+    for (int i = t; i < k; i += nt) {
+      const double a = A[i * d + k];
+      for (int j = 0; j < nrhs; ++j) B[i * nrhs + j] -= a * B[k * nrhs + j];
     }
     __syncthreads();
   }

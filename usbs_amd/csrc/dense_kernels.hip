@@ -37,26 +37,70 @@ __device__ __forceinline__ double symkron(const double* T, const double* Si, int
   return 0.5 * s;
 }
 
-// inverse of a k x k matrix by LU with partial pivoting (as jnp.linalg.inv).
-__device__ void dense_inv(const double* S, int k, double* Sinv, double* tmp, double* red, int* ired) {
-  for (int idx = threadIdx.x; idx < k * k; idx += blockDim.x) {
-    tmp[idx] = S[idx];
-    Sinv[idx] = (idx / k == idx % k) ? 1.0 : 0.0;
+// ---------------------------------------------------------------------------
+// Per-lane K x K Cholesky in registers (K is a compile-time constant so the
+// packed factor stays in VGPRs).  Returns false on a non-positive pivot.
+template <int K>
+__device__ __forceinline__ bool lane_chol(const double* M, const double* dM, double s, double* L) {
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    double d = M[c * K + c] + s * dM[c * K + c];
+#pragma unroll
+    for (int p = 0; p < c; ++p) d -= L[c * (c + 1) / 2 + p] * L[c * (c + 1) / 2 + p];
+    if (!(d > 0.0)) return false;
+    d = sqrt(d);
+    L[c * (c + 1) / 2 + c] = d;
+#pragma unroll
+    for (int i = c + 1; i < K; ++i) {
+      double v = M[i * K + c] + s * dM[i * K + c];
+#pragma unroll
+      for (int p = 0; p < c; ++p) v -= L[i * (i + 1) / 2 + p] * L[c * (c + 1) / 2 + p];
+      L[i * (i + 1) / 2 + c] = v / d;
+    }
+  }
+  return true;
+}
+
+// S^{-1} for SPD S: lane j < K solves L L^T x = e_j for column j.
+template <int K>
+__device__ void lane_spd_inverse(const double* S, double* Si) {
+  const int j = threadIdx.x;
+  if (j < K) {
+    double L[K * (K + 1) / 2];
+    lane_chol<K>(S, S, 0.0, L);
+    double y[K], x[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int p = 0; p < i; ++p) v -= L[i * (i + 1) / 2 + p] * y[p];
+      y[i] = v / L[i * (i + 1) / 2 + i];
+    }
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) {
+      double v = y[i];
+#pragma unroll
+      for (int p = i + 1; p < K; ++p) v -= L[p * (p + 1) / 2 + i] * x[p];
+      x[i] = v / L[i * (i + 1) / 2 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) Si[i * K + j] = x[i];
   }
   __syncthreads();
-  dense_lu_solve(tmp, k, Sinv, k, red, ired);
 }
 
 // Largest 0.9^j (j < max_iters) with M + 0.9^j dM positive definite, or 0:
 // the vmapped eigvalsh test of solver/usbs.py:135-144 with a Cholesky test
-// per candidate (lanes of the wavefront test 32 candidates at a time).
-__device__ double pd_step(const double* M, const double* dM, int k, int max_iters, double* scratch) {
+// per candidate (the 64 lanes test 64 candidates at a time).
+template <int K>
+__device__ double pd_step(const double* M, const double* dM, int max_iters) {
   const int lane = threadIdx.x;
-  for (int j0 = 0; j0 < max_iters; j0 += 32) {
+  for (int j0 = 0; j0 < max_iters; j0 += 64) {
     double v = 0.0;
-    if (lane < 32 && j0 + lane < max_iters) {
+    if (j0 + lane < max_iters) {
       const double s = pow(0.9, (double)(j0 + lane));
-      if (thread_is_pd(M, dM, s, k, scratch + lane * k * k)) v = s;
+      double L[K * (K + 1) / 2];
+      if (lane_chol<K>(M, dM, s, L)) v = s;
     }
     v = wave_max(v);
     if (v > 0.0) return v;
@@ -73,26 +117,60 @@ struct IpmLds {
   double *coeff, *q12, *h1, *sI, *sS, *sT, *F1, *ordv, *dsS, *dsT, *tmpv;
   double *S, *T, *Si, *dS, *dT, *tk, *red;
   int* ired;
+  int *pb, *pc;  // svec index -> (b, c)
 };
 
 __device__ IpmLds ipm_layout(double* lds, int k) {
   const int d = k * (k + 1) / 2, kk = k * k;
   IpmLds L;
   double* p = lds;
-  L.coeff = p; p += (d * d > 32 * kk ? d * d : 32 * kk);
+  L.coeff = p; p += (d * d > 4 * kk ? d * d : 4 * kk);
   L.q12 = p; p += d; L.h1 = p; p += d; L.sI = p; p += d; L.sS = p; p += d; L.sT = p; p += d;
   L.F1 = p; p += d; L.ordv = p; p += d; L.dsS = p; p += d; L.dsT = p; p += d; L.tmpv = p; p += d;
   L.S = p; p += kk; L.T = p; p += kk; L.Si = p; p += kk; L.dS = p; p += kk; L.dT = p; p += kk;
   L.tk = p; p += kk;
   L.red = p; p += 16;
-  L.ired = (int*)p;
+  L.ired = (int*)p; p += 16;
+  L.pb = (int*)p; p += (d + 1) / 2 + 1;
+  L.pc = (int*)p;
   return L;
 }
 
 static size_t ipm_lds_bytes(int k) {
   const int d = k * (k + 1) / 2, kk = k * k;
-  const int big = d * d > 32 * kk ? d * d : 32 * kk;
-  return sizeof(double) * (big + 10 * d + 6 * kk + 16 + 16);
+  const int big = d * d > 4 * kk ? d * d : 4 * kk;
+  return sizeof(double) * (big + 10 * d + 6 * kk + 16 + 16 + d + 4);
+}
+
+__device__ void ipm_tables(const IpmLds& L, int k) {
+  const int d = k * (k + 1) / 2;
+  for (int a = threadIdx.x; a < d; a += blockDim.x) {
+    int b, c;
+    svec_pair(a, b, c);
+    L.pb[a] = b;
+    L.pc[a] = c;
+    L.sI[a] = (b == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+}
+
+// 0.5 * U (T (x) Sinv + Sinv (x) T) U^T, entry (a, b)  (solver/usbs.py:92-93)
+__device__ __forceinline__ double symkron(const IpmLds& L, int k, int a, int b) {
+  const int b1 = L.pb[a], c1 = L.pc[a], b2 = L.pb[b], c2 = L.pc[b];
+  const double* T = L.T;
+  const double* Si = L.Si;
+  const double u = 0.70710678118654752440;
+  const int na = (b1 == c1) ? 1 : 2, nb = (b2 == c2) ? 1 : 2;
+  const double ua = (na == 1) ? 1.0 : u, ub = (nb == 1) ? 1.0 : u;
+  double s = 0.0;
+  for (int x = 0; x < na; ++x) {
+    const int p = x == 0 ? b1 : c1, q = x == 0 ? c1 : b1;
+    for (int y = 0; y < nb; ++y) {
+      const int pp = y == 0 ? b2 : c2, qq = y == 0 ? c2 : b2;
+      s += ua * ub * (T[p * k + pp] * Si[q * k + qq] + Si[p * k + pp] * T[q * k + qq]);
+    }
+  }
+  return 0.5 * s;
 }
 
 // mu of the current point (solver/usbs.py:75-77, 163-165)
@@ -117,17 +195,20 @@ __device__ void ipm_center(const IpmLds& L, int k, double& zeta, double& omega) 
   width = width < 100.0 ? 100.0 : width;
   omega = -1.01 * mn;
   omega = omega < width ? width : omega;
+  __syncthreads();
   for (int i = threadIdx.x; i < k; i += blockDim.x) L.T[i * k + i] += omega;
   zeta += omega;
   __syncthreads();
 }
 
 // Shared step: sizes, line search, update; returns the new mu and step.
-__device__ void ipm_step(const IpmLds& L, int k, int d, int max_iters, double& eta, double& zeta,
+template <int K>
+__device__ void ipm_step(const IpmLds& L, int max_iters, double& eta, double& zeta,
                          double& omega, double& mu, double& step, double d_eta, double d_zeta,
                          double d_omega) {
-  mat_from_svec(L.dsS, k, L.dS);
-  mat_from_svec(L.dsT, k, L.dT);
+  constexpr int D = K * (K + 1) / 2;
+  mat_from_svec(L.dsS, K, L.dS);
+  mat_from_svec(L.dsT, K, L.dT);
   const double r1 = d_eta / eta, r2 = d_zeta / zeta, r3 = d_omega / omega;
   double rmin = r1;
   if (r2 < rmin) rmin = r2;
@@ -136,12 +217,12 @@ __device__ void ipm_step(const IpmLds& L, int k, int d, int max_iters, double& e
   double st = -1.0 / rmin;
   st = st > 1.0 ? 1.0 : st;
   st = 0.99 * ((st <= 0.0) ? 1.0 : st);
-  const double cS = pd_step(L.S, L.dS, k, max_iters, L.coeff);
-  const double cT = pd_step(L.T, L.dT, k, max_iters, L.coeff);
+  const double cS = pd_step<K>(L.S, L.dS, max_iters);
+  const double cT = pd_step<K>(L.T, L.dT, max_iters);
   const double cand = cS < cT ? cS : cT;
   st = st < cand ? st : cand;
   __syncthreads();
-  for (int idx = threadIdx.x; idx < k * k; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < K * K; idx += blockDim.x) {
     L.S[idx] += st * L.dS[idx];
     L.T[idx] += st * L.dT[idx];
   }
@@ -149,7 +230,7 @@ __device__ void ipm_step(const IpmLds& L, int k, int d, int max_iters, double& e
   eta += st * d_eta;
   zeta += st * d_zeta;
   omega += st * d_omega;
-  double mun = ipm_mu(L, k, d, eta, zeta, omega);
+  double mun = ipm_mu(L, K, D, eta, zeta, omega);
   mun *= (st > 0.2) ? (0.5 - 0.4 * st * st) : 1.0;
   mu = mun > mu ? mu : mun;
   step = st;
@@ -164,10 +245,14 @@ __device__ void qmatvec(const double* Gq, double cq, const double* v, int d, dou
   __syncthreads();
 }
 
+template <int K>
 __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
+  constexpr int D = K * (K + 1) / 2;
+  const int k = K, d = D;
   extern __shared__ double lds[];
-  const int k = A.k, d = k * (k + 1) / 2, t = threadIdx.x;
+  const int t = threadIdx.x;
   IpmLds L = ipm_layout(lds, k);
+  ipm_tables(L, k);
   double* sc = A.sc;
   const double tu = A.tu, rho = A.rho;
   const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
@@ -177,11 +262,6 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
   const double h2 = (tu / trXb) * (sc[SC_ZY] - pb - (sc[SC_ZBU] / rho));
   dense_svec(A.Gz, k, L.q12);
   dense_svec(A.Gh, k, L.h1);
-  for (int a = t; a < d; a += IPM_NT) {
-    int b, c;
-    svec_pair(a, b, c);
-    L.sI[a] = (b == c) ? 1.0 : 0.0;
-  }
   __syncthreads();
   for (int a = t; a < d; a += IPM_NT) {
     L.q12[a] *= tu * tu / (rho * trXb);
@@ -225,13 +305,15 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
     __syncthreads();
     double F2 = dotd(L.q12, L.sS, d, L.red) + eta * q22 + h2;
     F2 = F2 + (-zeta + omega);
-    dense_inv(L.S, k, L.Si, L.tk, L.red, L.ired);
+    lane_spd_inverse<K>(L.S, L.Si);
     const double den = k1 * k2 + 1;
-    for (int idx = t; idx < d * d; idx += IPM_NT) {
-      const int a = idx / d, b = idx % d;
-      double v = cq * A.Gq[idx] + symkron(L.T, L.Si, k, a, b);
-      v -= (L.q12[a] * (k1 * L.q12[b] + L.sI[b]) + L.sI[a] * (L.q12[b] - k2 * L.sI[b])) / den;
-      L.coeff[idx] = v;
+    for (int a = t; a < d; a += IPM_NT) {
+      const double qa = L.q12[a], ia = L.sI[a];
+      for (int b = 0; b < d; ++b) {
+        double v = cq * A.Gq[a * d + b] + symkron(L, k, a, b);
+        v -= (qa * (k1 * L.q12[b] + L.sI[b]) + ia * (L.q12[b] - k2 * L.sI[b])) / den;
+        L.coeff[a * d + b] = v;
+      }
     }
     dense_svec(L.Si, k, L.tmpv);
     __syncthreads();
@@ -259,7 +341,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
       L.dsT[a] = L.dsT[a] + d_eta * L.q12[a] + d_omega * L.sI[a] + L.F1[a];
     __syncthreads();
     const double d_zeta = dotd(L.q12, L.dsS, d, L.red) + d_eta * q22 + d_omega + F2;
-    ipm_step(L, k, d, A.max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ipm_step<K>(L, A.max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
     ++it;
   }
   // outputs (usbs.py:191-192) and the bundle factor (usbs.py:568-570)
@@ -296,27 +378,20 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
   }
 }
 
-int launch_ipm_quad(const IpmArgs& a, hipStream_t s) {
-  if (a.k < 1 || a.k > IPM_MAXK) return (int)hipErrorInvalidValue;
-  k_ipm_quad<<<1, IPM_NT, ipm_lds_bytes(a.k), s>>>(a);
-  return (int)hipGetLastError();
-}
-
 // Spectral lower bound (solver/usbs.py:195-356); writes sc[SC_LB].
+template <int K>
 __global__ void __launch_bounds__(IPM_NT)
-k_ipm_lb(int k, int max_iters, double tu, double eps, const double* Glb, double* sc) {
+k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
+  constexpr int D = K * (K + 1) / 2;
+  const int k = K, d = D;
   extern __shared__ double lds[];
-  const int d = k * (k + 1) / 2, t = threadIdx.x;
+  const int t = threadIdx.x;
   IpmLds L = ipm_layout(lds, k);
+  ipm_tables(L, k);
   const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
   const double pb = sc[SC_PB];
   double* g1 = L.h1;
   dense_svec(Glb, k, g1);
-  for (int a = t; a < d; a += IPM_NT) {
-    int b, c;
-    svec_pair(a, b, c);
-    L.sI[a] = (b == c) ? 1.0 : 0.0;
-  }
   __syncthreads();
   for (int a = t; a < d; a += IPM_NT) g1[a] *= tu;
   const double g2 = (tu / trXb) * (sc[SC_ZYC] - pb);
@@ -333,12 +408,12 @@ k_ipm_lb(int k, int max_iters, double tu, double eps, const double* Glb, double*
   while (it < max_iters && mu > eps) {
     const double trS = dotd(L.sI, L.sS, d, L.red);
     const double k1 = (1 - trS - eta) / omega;
-    dense_inv(L.S, k, L.Si, L.tk, L.red, L.ired);
+    lane_spd_inverse<K>(L.S, L.Si);
     const double ze = zeta / eta;
     const double cI = ze / (k1 * ze + 1);
-    for (int idx = t; idx < d * d; idx += IPM_NT) {
-      const int a = idx / d, b = idx % d;
-      L.coeff[idx] = symkron(L.T, L.Si, k, a, b) + cI * (L.sI[a] * L.sI[b]);
+    for (int a = t; a < d; a += IPM_NT) {
+      const double ia = L.sI[a];
+      for (int b = 0; b < d; ++b) L.coeff[a * d + b] = symkron(L, k, a, b) + cI * (ia * L.sI[b]);
     }
     dense_svec(L.Si, k, L.tmpv);
     __syncthreads();
@@ -359,7 +434,7 @@ k_ipm_lb(int k, int max_iters, double tu, double eps, const double* Glb, double*
       L.dsT[a] = d_omega * L.sI[a] - L.sT[a] + g1[a] + omega * L.sI[a];
     __syncthreads();
     const double d_zeta = d_omega - zeta + g2 + omega;
-    ipm_step(L, k, d, max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ipm_step<K>(L, max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
     ++it;
   }
   dense_svec(L.S, k, L.sS);
@@ -371,10 +446,34 @@ k_ipm_lb(int k, int max_iters, double tu, double eps, const double* Glb, double*
   }
 }
 
+#define USBS_K_SWITCH(k, CALL) \
+  switch (k) {                 \
+    case 1: CALL(1); break;    \
+    case 2: CALL(2); break;    \
+    case 3: CALL(3); break;    \
+    case 4: CALL(4); break;    \
+    case 5: CALL(5); break;    \
+    case 6: CALL(6); break;    \
+    case 7: CALL(7); break;    \
+    case 8: CALL(8); break;    \
+    case 9: CALL(9); break;    \
+    case 10: CALL(10); break;  \
+    case 11: CALL(11); break;  \
+    default: return (int)hipErrorInvalidValue; \
+  }
+
+int launch_ipm_quad(const IpmArgs& a, hipStream_t s) {
+#define Q_CALL(KK) k_ipm_quad<KK><<<1, IPM_NT, ipm_lds_bytes(KK), s>>>(a)
+  USBS_K_SWITCH(a.k, Q_CALL)
+#undef Q_CALL
+  return (int)hipGetLastError();
+}
+
 int launch_ipm_lb(int k, int max_iters, double tu, double eps, const double* Glb, double* sc,
                   hipStream_t s) {
-  if (k < 1 || k > IPM_MAXK) return (int)hipErrorInvalidValue;
-  k_ipm_lb<<<1, IPM_NT, ipm_lds_bytes(k), s>>>(k, max_iters, tu, eps, Glb, sc);
+#define L_CALL(KK) k_ipm_lb<KK><<<1, IPM_NT, ipm_lds_bytes(KK), s>>>(max_iters, tu, eps, Glb, sc)
+  USBS_K_SWITCH(k, L_CALL)
+#undef L_CALL
   return (int)hipGetLastError();
 }
 

@@ -160,7 +160,7 @@ __device__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alp
   double* coef = w + m;
   double* red = coef + 2 * m;
   int* part = (int*)(red + 16);
-  int* sh = part + m;  // small int scratch
+  int* sh = part + m + 2;  // small int scratch (part[m..m+1] is eigh's)
   const int k = ctrl[LZ_K];
   for (int idx = t; idx < m * m; idx += nt) {
     const int i = idx / m, j = idx % m;
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_breakdown(LzInst<T> L, int i) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(USBS_DENSE_NT) k_lz_ritz(LzInst<T> L) {
+__global__ void __launch_bounds__(USBS_NT) k_lz_ritz(LzInst<T> L) {
   extern __shared__ double lds[];
   lz_ritz(L.m, L.nd, L.tol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
 }
@@ -313,7 +313,7 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
       k_lz_phaseD<T><<<g, b, lds, s>>>(L, i);
       k_lz_breakdown<T, MAXC><<<1, b, lds, s>>>(L, i);
     }
-    k_lz_ritz<T><<<1, USBS_DENSE_NT, lds, s>>>(L);
+    k_lz_ritz<T><<<1, USBS_NT, lds, s>>>(L);
     k_lz_rotate<T, MAXC><<<g, b, lds, s>>>(L);
     hipError_t e = hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(int) * LZ_NCTRL,
                                   hipMemcpyDeviceToHost, s);
