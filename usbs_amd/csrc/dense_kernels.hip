@@ -9,7 +9,7 @@
 
 namespace usbs {
 
-#define IPM_NT 64
+#define IPM_NT 256
 #define IPM_MAXK 11
 
 __device__ __forceinline__ double dotd(const double* x, const double* y, int d, double* red) {
@@ -93,17 +93,23 @@ __device__ void lane_spd_inverse(const double* S, double* Si) {
 // the vmapped eigvalsh test of solver/usbs.py:135-144 with a Cholesky test
 // per candidate (the 64 lanes test 64 candidates at a time).
 template <int K>
-__device__ double pd_step(const double* M, const double* dM, int max_iters) {
-  const int lane = threadIdx.x;
-  for (int j0 = 0; j0 < max_iters; j0 += 64) {
+__device__ double pd_step(const double* M, const double* dM, int max_iters, double* red) {
+  const int t = threadIdx.x;
+  for (int j0 = 0; j0 < max_iters; j0 += blockDim.x) {
     double v = 0.0;
-    if (j0 + lane < max_iters) {
-      const double s = pow(0.9, (double)(j0 + lane));
+    if (j0 + t < max_iters) {
+      const double s = pow(0.9, (double)(j0 + t));
       double L[K * (K + 1) / 2];
       if (lane_chol<K>(M, dM, s, L)) v = s;
     }
     v = wave_max(v);
-    if (v > 0.0) return v;
+    __syncthreads();
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    double b = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = red[w] > b ? red[w] : b;
+    __syncthreads();
+    if (b > 0.0) return b;
   }
   return 0.0;
 }
@@ -116,6 +122,7 @@ __device__ void mat_from_svec(const double* v, int k, double* M) {
 struct IpmLds {
   double *coeff, *q12, *h1, *sI, *sS, *sT, *F1, *ordv, *dsS, *dsT, *tmpv;
   double *S, *T, *Si, *dS, *dT, *tk, *red;
+  double* R;     // d x 3 right-hand sides of the Woodbury solve
   int* ired;
   int *pb, *pc;  // svec index -> (b, c)
 };
@@ -124,11 +131,12 @@ __device__ IpmLds ipm_layout(double* lds, int k) {
   const int d = k * (k + 1) / 2, kk = k * k;
   IpmLds L;
   double* p = lds;
-  L.coeff = p; p += (d * d > 4 * kk ? d * d : 4 * kk);
+  L.coeff = p; p += (d * d > 5 * kk + 4 * k + 64 ? d * d : 5 * kk + 4 * k + 64);
   L.q12 = p; p += d; L.h1 = p; p += d; L.sI = p; p += d; L.sS = p; p += d; L.sT = p; p += d;
   L.F1 = p; p += d; L.ordv = p; p += d; L.dsS = p; p += d; L.dsT = p; p += d; L.tmpv = p; p += d;
   L.S = p; p += kk; L.T = p; p += kk; L.Si = p; p += kk; L.dS = p; p += kk; L.dT = p; p += kk;
   L.tk = p; p += kk;
+  L.R = p; p += 3 * d;
   L.red = p; p += 16;
   L.ired = (int*)p; p += 16;
   L.pb = (int*)p; p += (d + 1) / 2 + 1;
@@ -138,8 +146,8 @@ __device__ IpmLds ipm_layout(double* lds, int k) {
 
 static size_t ipm_lds_bytes(int k) {
   const int d = k * (k + 1) / 2, kk = k * k;
-  const int big = d * d > 4 * kk ? d * d : 4 * kk;
-  return sizeof(double) * (big + 10 * d + 6 * kk + 16 + 16 + d + 4);
+  const int big = d * d > 5 * kk + 4 * k + 64 ? d * d : 5 * kk + 4 * k + 64;
+  return sizeof(double) * (big + 13 * d + 6 * kk + 16 + 16 + d + 4);
 }
 
 __device__ void ipm_tables(const IpmLds& L, int k) {
@@ -171,6 +179,47 @@ __device__ __forceinline__ double symkron(const IpmLds& L, int k, int a, int b) 
     }
   }
   return 0.5 * s;
+}
+
+
+// Solve (B - U W^T) x = r where B is SPD (factored in place by dense_ldlt),
+// U = [u1, u2], W = [w1, w2] (nu = 1 or 2 columns): the Sherman-Morrison-
+// Woodbury identity x = z0 + Z (I - W^T Z)^{-1} W^T z0 with z0 = B^{-1} r,
+// Z = B^{-1} U.  Same solution as a dense solve of the full matrix, up to
+// rounding.  r in/out; R: d x 3 LDS scratch.
+__device__ void woodbury_solve(double* B, int d, double* r, const double* u1, const double* u2,
+                               const double* w1, const double* w2, int nu, double* R,
+                               double* red) {
+  const int t = threadIdx.x;
+  for (int a = t; a < d; a += blockDim.x) {
+    R[a * 3 + 0] = r[a];
+    R[a * 3 + 1] = u1[a];
+    R[a * 3 + 2] = (nu > 1) ? u2[a] : 0.0;
+  }
+  __syncthreads();
+  dense_ldlt(B, d);
+  dense_ldlt_solve(B, d, R, 3);
+  double g0 = 0.0, g1 = 0.0, m00 = 0.0, m01 = 0.0, m10 = 0.0, m11 = 0.0;
+  for (int a = t; a < d; a += blockDim.x) {
+    const double z0 = R[a * 3], z1 = R[a * 3 + 1], z2 = R[a * 3 + 2];
+    const double wa = w1[a], wb = (nu > 1) ? w2[a] : 0.0;
+    g0 += wa * z0; g1 += wb * z0;
+    m00 += wa * z1; m01 += wa * z2; m10 += wb * z1; m11 += wb * z2;
+  }
+  g0 = block_sum1(g0, red); g1 = block_sum1(g1, red);
+  m00 = block_sum1(m00, red); m01 = block_sum1(m01, red);
+  m10 = block_sum1(m10, red); m11 = block_sum1(m11, red);
+  double c0, c1 = 0.0;
+  if (nu > 1) {
+    const double a00 = 1.0 - m00, a01 = -m01, a10 = -m10, a11 = 1.0 - m11;
+    const double det = a00 * a11 - a01 * a10;
+    c0 = (a11 * g0 - a01 * g1) / det;
+    c1 = (a00 * g1 - a10 * g0) / det;
+  } else {
+    c0 = g0 / (1.0 - m00);
+  }
+  for (int a = t; a < d; a += blockDim.x) r[a] = R[a * 3] + R[a * 3 + 1] * c0 + R[a * 3 + 2] * c1;
+  __syncthreads();
 }
 
 // mu of the current point (solver/usbs.py:75-77, 163-165)
@@ -217,8 +266,8 @@ __device__ void ipm_step(const IpmLds& L, int max_iters, double& eta, double& ze
   double st = -1.0 / rmin;
   st = st > 1.0 ? 1.0 : st;
   st = 0.99 * ((st <= 0.0) ? 1.0 : st);
-  const double cS = pd_step<K>(L.S, L.dS, max_iters);
-  const double cT = pd_step<K>(L.T, L.dT, max_iters);
+  const double cS = pd_step<K>(L.S, L.dS, max_iters, L.red);
+  const double cT = pd_step<K>(L.T, L.dT, max_iters, L.red);
   const double cand = cS < cT ? cS : cT;
   st = st < cand ? st : cand;
   __syncthreads();
@@ -307,15 +356,16 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
     F2 = F2 + (-zeta + omega);
     lane_spd_inverse<K>(L.S, L.Si);
     const double den = k1 * k2 + 1;
-    for (int a = t; a < d; a += IPM_NT) {
-      const double qa = L.q12[a], ia = L.sI[a];
-      for (int b = 0; b < d; ++b) {
-        double v = cq * A.Gq[a * d + b] + symkron(L, k, a, b);
-        v -= (qa * (k1 * L.q12[b] + L.sI[b]) + ia * (L.q12[b] - k2 * L.sI[b])) / den;
-        L.coeff[a * d + b] = v;
-      }
+    // coeff = B - U W^T, B = Q11 + sym-kron(T, S^-1) (SPD),
+    // U = [q12, svec(I)], W = [k1 q12 + svec(I), q12 - k2 svec(I)] / den
+    for (int idx = t; idx < d * d; idx += IPM_NT) {
+      const int a = idx / D, b = idx - (idx / D) * D;
+      L.coeff[idx] = cq * A.Gq[idx] + symkron(L, k, a, b);
     }
     dense_svec(L.Si, k, L.tmpv);
+    for (int a = t; a < d; a += IPM_NT) {
+      L.dsT[a] = (k1 * L.q12[a] + L.sI[a]) / den;
+    }
     __syncthreads();
     const double c_a = (mu / omega + trS + eta - 1 + k1 * (F2 - mu / eta + zeta)) / den;
     const double c_b = (F2 - mu / eta + zeta - k2 * (mu / omega + trS + eta - 1)) / den;
@@ -324,9 +374,10 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
       v += c_a * L.q12[a];
       v += c_b * L.sI[a];
       L.dsS[a] = v;
+      L.ordv[a] = (L.q12[a] - k2 * L.sI[a]) / den;
     }
     __syncthreads();
-    dense_lu_solve(L.coeff, d, L.dsS, 1, L.red, L.ired);
+    woodbury_solve(L.coeff, d, L.dsS, L.q12, L.sI, L.dsT, L.ordv, 2, L.R, L.red);
     // back-substitution for the other directions (usbs.py:111-122)
     for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = k1 * L.q12[a] + L.sI[a];
     __syncthreads();
@@ -358,7 +409,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
   double* tmp = Ve + k * k;
   double* w = tmp + k * k;
   double* coef = w + k;
-  int* part = L.ired + 4;
+  int* part = (int*)(coef + 2 * k + 2);
   for (int idx = t; idx < k * k; idx += IPM_NT) Ae[idx] = tu * L.S[idx];
   __syncthreads();
   double trs = 0.0;
@@ -411,11 +462,13 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
     lane_spd_inverse<K>(L.S, L.Si);
     const double ze = zeta / eta;
     const double cI = ze / (k1 * ze + 1);
-    for (int a = t; a < d; a += IPM_NT) {
-      const double ia = L.sI[a];
-      for (int b = 0; b < d; ++b) L.coeff[a * d + b] = symkron(L, k, a, b) + cI * (ia * L.sI[b]);
+    // coeff = sym-kron(T, S^-1) + cI svec(I) svec(I)^T  (B - U W^T with W = -cI svec(I))
+    for (int idx = t; idx < d * d; idx += IPM_NT) {
+      const int a = idx / D, b = idx - (idx / D) * D;
+      L.coeff[idx] = symkron(L, k, a, b);
     }
     dense_svec(L.Si, k, L.tmpv);
+    for (int a = t; a < d; a += IPM_NT) L.ordv[a] = -cI * L.sI[a];
     __syncthreads();
     const double cf = ze * (-k1 * (mu / eta - g2 - omega) + mu / omega + trS + eta - 1) / (-k1 * ze - 1);
     for (int a = t; a < d; a += IPM_NT) {
@@ -425,7 +478,7 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
       L.dsS[a] = v;
     }
     __syncthreads();
-    dense_lu_solve(L.coeff, d, L.dsS, 1, L.red, L.ired);
+    woodbury_solve(L.coeff, d, L.dsS, L.sI, nullptr, L.ordv, nullptr, 1, L.R, L.red);
     const double p1 = dotd(L.sI, L.dsS, d, L.red);
     const double d_eta = (p1 - k1 * (mu / eta - g2 - omega) + mu / omega + trS + eta - 1) / (-k1 * ze - 1);
     double d_omega = -ze * d_eta + mu / eta;
@@ -499,7 +552,7 @@ k_nystrom_core(int n, int r, const double* PtPO, const double* OtO, double* G, d
   double* M = B + rr;
   double* X = M + rr;
   int* part = (int*)(X + rr);
-  int* flag = part + r + 4;
+  int* flag = part + r * r + 2 * r + 8;
   // PtPO = P^T [P | Omega] (r x 2r row-major): P^T P and P^T Omega
 #define PtP(i, j) PtPO[(i) * 2 * r + (j)]
 #define OtP(i, j) PtPO[(j) * 2 * r + r + (i)]
@@ -569,7 +622,7 @@ k_nystrom_core(int n, int r, const double* PtPO, const double* OtO, double* G, d
 int launch_nystrom_core(int n, int r, const double* PtPO, const double* OtO, double* G,
                         double* lam, double* sc, hipStream_t s) {
   if (r < 1 || r > 48) return (int)hipErrorInvalidValue;
-  const size_t lds = sizeof(double) * (7 * r * r + 4 * r + 32);
+  const size_t lds = sizeof(double) * (8 * r * r + 6 * r + 64);
   k_nystrom_core<<<1, IPM_NT, lds, s>>>(n, r, PtPO, OtO, G, lam, sc);
   return (int)hipGetLastError();
 }

@@ -6,7 +6,8 @@
 //    accumulate in double.
 //  * Rows are split into fixed tiles of `tile` rows (a function of n only).
 //    Every reduction over rows first produces one partial per tile, then sums
-//    the tile partials in tile order.  Results therefore do not depend on the
+//    the tile partials in an order fixed by the tile count alone.  Results
+//    therefore do not depend on the
 //    launch shape, on how many workgroups a persistent launch has, or on which
 //    other problems share the GPU: a solve is bitwise the same alone and in a
 //    batch.
@@ -84,15 +85,38 @@ __device__ __forceinline__ double block_sum1(double v, double* lds /* >= NT/64 d
   return s;
 }
 
-// Sum tile partials part[tile * stride + j] over tiles in order; j < c.
-// Every thread of the block computes the sums for its j (j = t, t+NT, ...)
-// and stores them in out (LDS).
+// Sum tile partials part[tile * stride + j] over tiles, j < c <= blockDim,
+// into out (LDS).  P = blockDim / c threads share a column: thread (j, q)
+// sums tiles q, q + P, ... in order, then thread j adds the P sums in order.
+// The order depends only on (c, ntiles, blockDim), so it is deterministic.
+// scratch: LDS of blockDim doubles (must not alias out).
 __device__ __forceinline__ void sum_tile_partials(const double* __restrict__ part, int ntiles,
-                                                  int stride, int c, double* out) {
-  for (int j = threadIdx.x; j < c; j += blockDim.x) {
+                                                  int stride, int c, double* out,
+                                                  double* scratch) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  if (c <= 0) { __syncthreads(); return; }
+  int P = nt / c;
+  if (P < 1) P = 1;
+  if (P > ntiles) P = ntiles;
+  if (t < c * P) {
+    const int j = t / P, q = t - (t / P) * P;
     double s = 0.0;
-    for (int tl = 0; tl < ntiles; ++tl) s += part[(size_t)tl * stride + j];
-    out[j] = s;
+    int tl = q;
+    for (; tl + 3 * P < ntiles; tl += 4 * P) {
+      const double a0 = part[(size_t)tl * stride + j];
+      const double a1 = part[(size_t)(tl + P) * stride + j];
+      const double a2 = part[(size_t)(tl + 2 * P) * stride + j];
+      const double a3 = part[(size_t)(tl + 3 * P) * stride + j];
+      s += a0; s += a1; s += a2; s += a3;
+    }
+    for (; tl < ntiles; tl += P) s += part[(size_t)tl * stride + j];
+    scratch[t] = s;
+  }
+  __syncthreads();
+  if (t < c) {
+    double s = 0.0;
+    for (int q = 0; q < P; ++q) s += scratch[t * P + q];
+    out[t] = s;
   }
   __syncthreads();
 }
@@ -131,7 +155,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned nblocks
     if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
       ok = 0;
     } else {
-      __atomic_thread_fence(__ATOMIC_RELEASE);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       unsigned long long polls = 0;
       while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -147,7 +171,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned nblocks
           break;
         }
       }
-      __atomic_thread_fence(__ATOMIC_ACQUIRE);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     s_ok = ok;
   }

@@ -188,7 +188,8 @@ int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int
   if (!(L.beta = (double*)ctx_alloc(c, pool, sizeof(double) * m))) return USBS_ERR_OOM;
   if (!(L.Yr = (double*)ctx_alloc(c, pool, sizeof(double) * m * m))) return USBS_ERR_OOM;
   if (!(L.ctrl = (int*)ctx_alloc(c, pool, sizeof(int) * LZ_NCTRL))) return USBS_ERR_OOM;
-  if (!(L.bar = (unsigned*)ctx_alloc(c, pool, sizeof(unsigned) * 64))) return USBS_ERR_OOM;
+  L.nbar = max_restarts + 1 > 64 ? max_restarts + 1 : 64;
+  if (!(L.bar = (unsigned*)ctx_alloc(c, pool, sizeof(unsigned) * L.nbar))) return USBS_ERR_OOM;
   return USBS_OK;
 }
 
@@ -208,7 +209,7 @@ __global__ void k_lz_reset(LzInst<T> L, const double* q0) {
   }
   if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
   if (i < LZ_NCTRL) L.ctrl[i] = 0;
-  if (i < 64) L.bar[i] = 0u;
+  for (long long x = i; x < L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
 }
 
 template <typename T>
@@ -256,13 +257,15 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
     int nb = std::min(L.ntiles, std::max(1, c->coresident));
     L.nblocks = nb;
     LzInst<T>* d_inst = (LzInst<T>*)ctx_alloc(c, pool, sizeof(LzInst<T>));
-    int* d_map = (int*)ctx_alloc(c, pool, sizeof(int2) * nb);
+    int* d_map = (int*)ctx_alloc(c, pool, sizeof(int2) * (nb + L.ntiles));
     if (!d_inst || !d_map) return USBS_ERR_OOM;
-    std::vector<int2> map(nb);
+    std::vector<int2> map(nb + L.ntiles);
     for (int i = 0; i < nb; ++i) map[i] = make_int2(0, i);
+    for (int i = 0; i < L.ntiles; ++i) map[nb + i] = make_int2(0, i);
     USBS_HIP(c, hipMemcpyAsync(d_inst, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
-    USBS_HIP(c, hipMemcpyAsync(d_map, map.data(), sizeof(int2) * nb, hipMemcpyHostToDevice, c->stream));
-    rc = lanczos_persistent<T>(d_inst, 1, &nb, m, c->stream, d_map, 1ull << 26);
+    USBS_HIP(c, hipMemcpyAsync(d_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, c->stream));
+    rc = lanczos_persistent<T>(d_inst, 1, nb, m, max_restarts, c->stream, d_map, d_map + 2 * nb,
+                               L.ntiles, 1ull << 26);
     if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos persistent: ") + hipGetErrorString((hipError_t)rc));
     USBS_HIP(c, hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, c->stream));
     USBS_HIP(c, hipStreamSynchronize(c->stream));

@@ -21,14 +21,14 @@
 namespace usbs {
 
 int tile_rows_for(int n) {
-  int t = 1024;
+  int t = 256;
   while ((n + t - 1) / t > 256) t *= 2;
   return t;
 }
 
 static inline int lds_doubles_for(int m, int maxc) {
   const int red = USBS_RCH * USBS_NT + USBS_RCH * USBS_NPART + maxc + 32;
-  const int ritz = 3 * m * m + 4 * m + 64;
+  const int ritz = 4 * m * m + 4 * m + 64;
   return red > ritz ? red : ritz;
 }
 
@@ -76,7 +76,7 @@ __device__ void tile_update(const LzInst<T>& L, int tl, int i, const double* src
   double* red = lds;
   double* red2 = lds + USBS_RCH * USBS_NT;
   double* h = red2 + USBS_RCH * USBS_NPART;
-  sum_tile_partials(src, L.ntiles, L.pstride, i + 1, h);
+  sum_tile_partials(src, L.ntiles, L.pstride, i + 1, h, red);
   if (save != nullptr && tl == 0)
     for (int j = threadIdx.x; j <= i; j += USBS_NT) save[j] = h[j];
   const int r0 = tl * L.tile;
@@ -119,7 +119,7 @@ __device__ void tile_normalize(const LzInst<T>& L, int tl, int i, double beta) {
 template <typename T>
 __device__ double lz_beta(const LzInst<T>& L, double* lds) {
   double* s = lds + USBS_RCH * USBS_NT;
-  sum_tile_partials(L.part3, L.ntiles, L.pstride, 1, s);
+  sum_tile_partials(L.part3, L.ntiles, L.pstride, 1, s, lds);
   const double b = sqrt(s[0]);
   __syncthreads();
   return b;
@@ -159,8 +159,8 @@ __device__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alp
   double* w = tmp + m * m;
   double* coef = w + m;
   double* red = coef + 2 * m;
-  int* part = (int*)(red + 16);
-  int* sh = part + m + 2;  // small int scratch (part[m..m+1] is eigh's)
+  int* sh = (int*)(red + 16);  // small int scratch
+  int* part = sh + 8;          // eigh's scratch: m*m + 2m + 4 ints
   const int k = ctrl[LZ_K];
   for (int idx = t; idx < m * m; idx += nt) {
     const int i = idx / m, j = idx % m;
@@ -284,9 +284,20 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_breakdown(LzInst<T> L, int i) {
   for (int tl = 0; tl < L.ntiles; ++tl) tile_normalize(L, tl, i, b);
 }
 
+#define LZ_RITZ_NT 1024
 template <typename T>
-__global__ void __launch_bounds__(USBS_NT) k_lz_ritz(LzInst<T> L) {
+__global__ void __launch_bounds__(LZ_RITZ_NT) k_lz_ritz(LzInst<T> L) {
   extern __shared__ double lds[];
+  lz_ritz(L.m, L.nd, L.tol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
+}
+
+// Ritz step of restart r for every instance still running (block = instance).
+template <typename T>
+__global__ void __launch_bounds__(LZ_RITZ_NT) k_lz_ritz_multi(const LzInst<T>* __restrict__ insts,
+                                                              int r) {
+  extern __shared__ double lds[];
+  const LzInst<T>& L = insts[blockIdx.x];
+  if (L.ctrl[LZ_DONE] || L.ctrl[LZ_RESTART] != r || L.ctrl[LZ_ERROR]) return;
   lz_ritz(L.m, L.nd, L.tol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
 }
 
@@ -298,6 +309,21 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate(LzInst<T> L) {
   for (int idx = threadIdx.x; idx < mm; idx += USBS_NT) lds[idx] = L.Yr[idx];
   __syncthreads();
   tile_rotate<T, MAXC>(L, blockIdx.x, knext, lds);
+}
+
+// Rotation after the Ritz step of restart r (block = (instance, tile)).
+template <typename T, int MAXC>
+__global__ void __launch_bounds__(USBS_NT) k_lz_rotate_multi(const LzInst<T>* __restrict__ insts,
+                                                             const int2* __restrict__ tilemap, int r) {
+  extern __shared__ double lds[];
+  const int2 mp = tilemap[blockIdx.x];
+  const LzInst<T>& L = insts[mp.x];
+  if (L.ctrl[LZ_RESTART] != r + 1 || L.ctrl[LZ_ERROR]) return;
+  const int knext = L.ctrl[LZ_K];
+  const int mm = L.m * L.m;
+  for (int idx = threadIdx.x; idx < mm; idx += USBS_NT) lds[idx] = L.Yr[idx];
+  __syncthreads();
+  tile_rotate<T, MAXC>(L, mp.y, knext, lds);
 }
 
 template <typename T, int MAXC>
@@ -313,7 +339,7 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
       k_lz_phaseD<T><<<g, b, lds, s>>>(L, i);
       k_lz_breakdown<T, MAXC><<<1, b, lds, s>>>(L, i);
     }
-    k_lz_ritz<T><<<1, USBS_NT, lds, s>>>(L);
+    k_lz_ritz<T><<<1, LZ_RITZ_NT, lds, s>>>(L);
     k_lz_rotate<T, MAXC><<<g, b, lds, s>>>(L);
     hipError_t e = hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(int) * LZ_NCTRL,
                                   hipMemcpyDeviceToHost, s);
@@ -361,93 +387,88 @@ int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
 }
 
 // ----------------------------------------------------------------------------
-// persistent form: wgmap[blockIdx.x] = (instance, local workgroup index)
+// persistent form: one cooperative launch runs every Lanczos step of restart
+// r for all instances; wgmap[blockIdx.x] = (instance, local workgroup index).
 
 template <typename T, int MAXC>
-__global__ void __launch_bounds__(USBS_NT) k_lz_persistent(const LzInst<T>* __restrict__ insts,
-                                                           const int2* __restrict__ wgmap,
+__global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __restrict__ insts,
+                                                           const int2* __restrict__ wgmap, int r,
                                                            unsigned long long spin_limit) {
   extern __shared__ double lds[];
   const int2 mp = wgmap[blockIdx.x];
   const LzInst<T> L = insts[mp.x];
+  if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;  // uniform over the instance's workgroups
   const int lw = mp.y, G = L.nblocks;
   int* err = L.ctrl + LZ_ERROR;
+  unsigned* bar = L.bar + (r < L.nbar ? r : L.nbar - 1);
   unsigned target = 0;
-  int k = L.ctrl[LZ_K];
+  const int k = L.ctrl[LZ_K];
   bool ok = true;
-  for (int restart = 0; restart < L.max_restarts && ok; ++restart) {
-    for (int i = k; i < L.m && ok; ++i) {
-      for (int tl = lw; tl < L.ntiles; tl += G) tile_phaseA<T, MAXC>(L, tl, i, 0, lds);
-      ok = grid_barrier(L.bar, G, target, err, spin_limit);
-      if (!ok) break;
-      for (int tl = lw; tl < L.ntiles; tl += G)
-        tile_update<T, MAXC>(L, tl, i, L.part1, L.part2, L.h1, 0, lds);
-      ok = grid_barrier(L.bar, G, target, err, spin_limit);
-      if (!ok) break;
-      for (int tl = lw; tl < L.ntiles; tl += G)
-        tile_update<T, MAXC>(L, tl, i, L.part2, L.part3, L.h2, 1, lds);
-      ok = grid_barrier(L.bar, G, target, err, spin_limit);
-      if (!ok) break;
-      double beta = lz_beta(L, lds);
-      const bool brk = beta < 1e-15;
-      if (lw == 0 && threadIdx.x == 0) {
-        L.alpha[i] = L.h1[i] + L.h2[i];
-        L.beta[i] = beta;
-        L.ctrl[LZ_BREAKDOWN] = brk ? 1 : 0;
-        if (brk) L.ctrl[LZ_NBREAK] += 1;
-        L.ctrl[LZ_STEPS] += 1;
-      }
-      if (brk) {
-        for (int tl = lw; tl < L.ntiles; tl += G) tile_phaseA<T, MAXC>(L, tl, i, 1, lds);
-        ok = grid_barrier(L.bar, G, target, err, spin_limit);
-        if (!ok) break;
-        for (int tl = lw; tl < L.ntiles; tl += G)
-          tile_update<T, MAXC>(L, tl, i, L.part1, L.part2, nullptr, 0, lds);
-        ok = grid_barrier(L.bar, G, target, err, spin_limit);
-        if (!ok) break;
-        for (int tl = lw; tl < L.ntiles; tl += G)
-          tile_update<T, MAXC>(L, tl, i, L.part2, L.part3, nullptr, 1, lds);
-        ok = grid_barrier(L.bar, G, target, err, spin_limit);
-        if (!ok) break;
-        beta = lz_beta(L, lds);
-      }
-      for (int tl = lw; tl < L.ntiles; tl += G) tile_normalize(L, tl, i, beta);
-      ok = grid_barrier(L.bar, G, target, err, spin_limit);
+  for (int i = k; i < L.m && ok; ++i) {
+    for (int tl = lw; tl < L.ntiles; tl += G) tile_phaseA<T, MAXC>(L, tl, i, 0, lds);
+    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+    for (int tl = lw; tl < L.ntiles; tl += G)
+      tile_update<T, MAXC>(L, tl, i, L.part1, L.part2, L.h1, 0, lds);
+    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+    for (int tl = lw; tl < L.ntiles; tl += G)
+      tile_update<T, MAXC>(L, tl, i, L.part2, L.part3, L.h2, 1, lds);
+    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+    double beta = lz_beta(L, lds);
+    const bool brk = beta < 1e-15;
+    if (lw == 0 && threadIdx.x == 0) {
+      L.alpha[i] = L.h1[i] + L.h2[i];
+      L.beta[i] = beta;
+      L.ctrl[LZ_BREAKDOWN] = brk ? 1 : 0;
+      if (brk) L.ctrl[LZ_NBREAK] += 1;
+      L.ctrl[LZ_STEPS] += 1;
     }
-    if (!ok) break;
-    if (lw == 0) lz_ritz(L.m, L.nd, L.tol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
-    ok = grid_barrier(L.bar, G, target, err, spin_limit);
-    if (!ok) break;
-    const int knext = __hip_atomic_load(L.ctrl + LZ_K, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int done = __hip_atomic_load(L.ctrl + LZ_DONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int mm = L.m * L.m;
-    for (int idx = threadIdx.x; idx < mm; idx += USBS_NT) lds[idx] = L.Yr[idx];
-    __syncthreads();
-    for (int tl = lw; tl < L.ntiles; tl += G) tile_rotate<T, MAXC>(L, tl, knext, lds);
-    ok = grid_barrier(L.bar, G, target, err, spin_limit);
-    k = knext;
-    if (done) break;
+    if (brk) {
+      for (int tl = lw; tl < L.ntiles; tl += G) tile_phaseA<T, MAXC>(L, tl, i, 1, lds);
+      if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+      for (int tl = lw; tl < L.ntiles; tl += G)
+        tile_update<T, MAXC>(L, tl, i, L.part1, L.part2, nullptr, 0, lds);
+      if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+      for (int tl = lw; tl < L.ntiles; tl += G)
+        tile_update<T, MAXC>(L, tl, i, L.part2, L.part3, nullptr, 1, lds);
+      if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+      beta = lz_beta(L, lds);
+    }
+    for (int tl = lw; tl < L.ntiles; tl += G) tile_normalize(L, tl, i, beta);
+    ok = grid_barrier(bar, G, target, err, spin_limit);
   }
 }
 
 template <typename T, int MAXC>
-static int persistent_impl(const LzInst<T>* dev_insts, int ninst, const int* host_nblocks, int m,
-                           hipStream_t s, int* dev_wgmap, unsigned long long spin_limit) {
-  int total = 0;
-  for (int i = 0; i < ninst; ++i) total += host_nblocks[i];
+static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
+                           int max_restarts, hipStream_t s, const int* dev_wgmap,
+                           const int* dev_tilemap, int total_tiles, unsigned long long spin_limit) {
   const size_t lds = sizeof(double) * lds_doubles_for(m, MAXC);
-  void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&spin_limit};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_lz_persistent<T, MAXC>, dim3(total),
-                                            dim3(USBS_NT), args, lds, s);
-  return (int)e;
+  for (int r = 0; r < max_restarts; ++r) {
+    void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&r, (void*)&spin_limit};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, MAXC>, dim3(total_wgs),
+                                              dim3(USBS_NT), args, lds, s);
+    if (e != hipSuccess) return (int)e;
+    k_lz_ritz_multi<T><<<ninst, LZ_RITZ_NT, lds, s>>>(dev_insts, r);
+    k_lz_rotate_multi<T, MAXC><<<total_tiles, USBS_NT, lds, s>>>(dev_insts, (const int2*)dev_tilemap, r);
+    e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+  }
+  return 0;
 }
 
 template <typename T>
-int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, const int* host_nblocks, int m,
-                       hipStream_t s, int* dev_wgmap, unsigned long long spin_limit) {
-  if (m + 1 <= 17) return persistent_impl<T, 17>(dev_insts, ninst, host_nblocks, m, s, dev_wgmap, spin_limit);
-  if (m + 1 <= 33) return persistent_impl<T, 33>(dev_insts, ninst, host_nblocks, m, s, dev_wgmap, spin_limit);
-  if (m + 1 <= 49) return persistent_impl<T, 49>(dev_insts, ninst, host_nblocks, m, s, dev_wgmap, spin_limit);
+int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
+                       int max_restarts, hipStream_t s, const int* dev_wgmap,
+                       const int* dev_tilemap, int total_tiles, unsigned long long spin_limit) {
+  if (m + 1 <= 17)
+    return persistent_impl<T, 17>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
+                                  dev_tilemap, total_tiles, spin_limit);
+  if (m + 1 <= 33)
+    return persistent_impl<T, 33>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
+                                  dev_tilemap, total_tiles, spin_limit);
+  if (m + 1 <= 49)
+    return persistent_impl<T, 49>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
+                                  dev_tilemap, total_tiles, spin_limit);
   return (int)hipErrorInvalidValue;
 }
 
@@ -457,18 +478,18 @@ int lanczos_max_coresident_blocks(int m, int device) {
   int per_cu = 0;
   hipError_t e;
   if (maxc == 17)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_persistent<float, 17>, USBS_NT, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, 17>, USBS_NT, lds);
   else if (maxc == 33)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_persistent<float, 33>, USBS_NT, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, 33>, USBS_NT, lds);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_persistent<float, 49>, USBS_NT, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, 49>, USBS_NT, lds);
   int per_cu64 = 0;
   if (maxc == 17)
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_persistent<double, 17>, USBS_NT, lds);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, 17>, USBS_NT, lds);
   else if (maxc == 33)
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_persistent<double, 33>, USBS_NT, lds);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, 33>, USBS_NT, lds);
   else
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_persistent<double, 49>, USBS_NT, lds);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, 49>, USBS_NT, lds);
   if (e != hipSuccess) return 0;
   if (per_cu64 < per_cu) per_cu = per_cu64;
   hipDeviceProp_t prop;
@@ -478,9 +499,9 @@ int lanczos_max_coresident_blocks(int m, int device) {
 
 template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*);
 template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*);
-template int lanczos_persistent<double>(const LzInst<double>*, int, const int*, int, hipStream_t, int*,
-                                        unsigned long long);
-template int lanczos_persistent<float>(const LzInst<float>*, int, const int*, int, hipStream_t, int*,
-                                       unsigned long long);
+template int lanczos_persistent<double>(const LzInst<double>*, int, int, int, int, hipStream_t,
+                                        const int*, const int*, int, unsigned long long);
+template int lanczos_persistent<float>(const LzInst<float>*, int, int, int, int, hipStream_t,
+                                       const int*, const int*, int, unsigned long long);
 
 }  // namespace usbs

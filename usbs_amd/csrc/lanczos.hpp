@@ -40,7 +40,8 @@ struct LzInst {
   double* beta;  // m
   double* Yr;    // m x m row-major: saved Ritz vectors (columns) of T
   int* ctrl;     // LZ_NCTRL
-  unsigned* bar; // barrier counter (persistent launch)
+  unsigned* bar; // barrier counters, one per restart (persistent launch)
+  int nbar;      // number of counters in bar (>= max_restarts)
 };
 
 int tile_rows_for(int n);
@@ -48,9 +49,14 @@ int tile_rows_for(int n);
 // Host launchers (lanczos.hip).  All return hipError_t as int.
 template <typename T>
 int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl);
+// Persistent form for ninst instances sharing the basis size m: per restart
+// one cooperative launch of all Lanczos steps (dev_wgmap: int2 per workgroup
+// = instance, local workgroup), then the Ritz kernel (one block per
+// instance) and the rotation (dev_tilemap: int2 per tile = instance, tile).
 template <typename T>
-int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, const int* host_nblocks, int m,
-                       hipStream_t s, int* dev_wgmap, unsigned long long spin_limit);
+int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
+                       int max_restarts, hipStream_t s, const int* dev_wgmap,
+                       const int* dev_tilemap, int total_tiles, unsigned long long spin_limit);
 int lanczos_max_coresident_blocks(int m, int device);
 template <typename T>
 int orth_perop(const LzInst<T>& L, hipStream_t s);

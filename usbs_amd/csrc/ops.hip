@@ -46,6 +46,28 @@ int launch_assemble(const Problem& P, double coefC, const double* z1, double a1,
 
 namespace usbs {
 
+// Sum over tiles of partial[tile * stride + j] by one wavefront: lane q sums
+// tiles q, q + 64, ... in order, then a fixed butterfly; lane 0 holds the
+// deterministic total.
+__device__ __forceinline__ double wave_tile_sum(const double* __restrict__ partial, int ntiles,
+                                                long long stride, long long j) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int t = lane; t < ntiles; t += 64) s += partial[(long long)t * stride + j];
+  return wave_sum(s);
+}
+
+__device__ __forceinline__ double wave_tile_max(const double* __restrict__ partial, int ntiles,
+                                                long long stride, long long j) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int t = lane; t < ntiles; t += 64) {
+    const double v = partial[(long long)t * stride + j];
+    s = v > s ? v : s;
+  }
+  return wave_max(s);
+}
+
 // ---------------------------------------------------------------------------
 // Y (double, ldy) = M X for p <= 16 columns; M in CSR with values `val`.
 template <typename TV, typename TX>
@@ -131,19 +153,19 @@ k_tsgram(int n, int tile, const TA* __restrict__ A, long long lda, int p, const 
   }
 }
 
-// out[e] = scale * sum_tiles partial[tile][e], tiles in order.
-__global__ void k_reduce_partials(const double* __restrict__ partial, int ntiles, int count,
-                                  double scale, double* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// out[e] = scale * sum over tiles of partial[tile][e]; one wavefront per entry.
+__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ partial,
+                                                          int ntiles, int count, double scale,
+                                                          double* __restrict__ out) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= count) return;
-  double s = 0.0;
-  for (int t = 0; t < ntiles; ++t) s += partial[(long long)t * count + e];
-  out[e] = scale * s;
+  const double s = wave_tile_sum(partial, ntiles, count, e);
+  if ((threadIdx.x & 63) == 0) out[e] = scale * s;
 }
 
 int launch_reduce(const double* partial, int ntiles, int count, double scale, double* out,
                   hipStream_t s) {
-  k_reduce_partials<<<(count + 255) / 256, 256, 0, s>>>(partial, ntiles, count, scale, out);
+  k_reduce_partials<<<(count + 3) / 4, 256, 0, s>>>(partial, ntiles, count, scale, out);
   return (int)hipGetLastError();
 }
 
@@ -214,8 +236,10 @@ template int launch_tsmm<double, float>(int, const double*, long long, int, cons
 
 // ---------------------------------------------------------------------------
 // Q_base Gram (solver/utils.py:53-65): G row c = svec(V^T A_c V) built in LDS
-// per chunk of constraints, then partial[tile] += G_chunk^T G_chunk (upper
-// triangle a <= b stored at a*d + b; the reduce kernel mirrors it).
+// per chunk of GRAM_CH constraints, then partial[tile] += G_chunk^T G_chunk.
+// Each thread owns 4 x 4 output blocks (upper block triangle only); the
+// reduce kernel mirrors the strictly lower blocks.
+#define QB_BS 4
 template <typename TV>
 __global__ void __launch_bounds__(USBS_NT)
 k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
@@ -224,24 +248,54 @@ k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
         double* __restrict__ partial) {
   extern __shared__ double lds[];
   const int d = k * (k + 1) / 2;
-  double* G = lds;  // GRAM_CH x d
+  const int dp = (d + QB_BS - 1) / QB_BS * QB_BS;  // padded width
+  double* G = lds;  // GRAM_CH x dp
+  int* pb = (int*)(G + GRAM_CH * dp);
+  int* pc = pb + dp;
+  const int nbk = dp / QB_BS;
+  const int nblk = nbk * (nbk + 1) / 2;  // upper block triangle
   const int tl = blockIdx.x;
   const int c0t = tl * mtile, c1t = min(m, c0t + mtile);
-  const int dd = d * d;
-  double acc[24];
+  for (int a = threadIdx.x; a < dp; a += USBS_NT) {
+    int bb = 0, cc = 0;
+    if (a < d) {
+      bb = (int)((sqrt(8.0 * a + 1.0) - 1.0) * 0.5);
+      while ((bb + 1) * (bb + 2) / 2 <= a) ++bb;
+      while (bb * (bb + 1) / 2 > a) --bb;
+      cc = a - bb * (bb + 1) / 2;
+    }
+    pb[a] = bb;
+    pc[a] = cc;
+  }
+  // this thread's blocks: bi <= bj in the upper triangle, indices s*NT + t
+  constexpr int MAXB = 2;  // nblk <= 2*256 covers d <= 124
+  int bi[MAXB], bj[MAXB];
 #pragma unroll
-  for (int s = 0; s < 24; ++s) acc[s] = 0.0;
+  for (int s = 0; s < MAXB; ++s) {
+    int id = threadIdx.x + s * USBS_NT, r = 0;
+    bi[s] = -1; bj[s] = -1;
+    if (id < nblk) {
+      while (id >= nbk - r) { id -= nbk - r; ++r; }
+      bi[s] = r;
+      bj[s] = r + id;
+    }
+  }
+  double acc[MAXB][QB_BS][QB_BS];
+#pragma unroll
+  for (int s = 0; s < MAXB; ++s)
+#pragma unroll
+    for (int x = 0; x < QB_BS; ++x)
+#pragma unroll
+      for (int y = 0; y < QB_BS; ++y) acc[s][x][y] = 0.0;
   const double u = 0.70710678118654752440;
+  __syncthreads();
   for (int c0 = c0t; c0 < c1t; c0 += GRAM_CH) {
     const int rows = min(GRAM_CH, c1t - c0);
-    for (int idx = threadIdx.x; idx < GRAM_CH * d; idx += USBS_NT) {
-      const int rr = idx / d, a = idx % d;
+    for (int idx = threadIdx.x; idx < GRAM_CH * dp; idx += USBS_NT) {
+      const int rr = idx / dp, a = idx - (idx / dp) * dp;
       double v = 0.0;
-      if (rr < rows) {
-        int bb = (int)((sqrt(8.0 * a + 1.0) - 1.0) * 0.5);
-        while ((bb + 1) * (bb + 2) / 2 <= a) ++bb;
-        while (bb * (bb + 1) / 2 > a) --bb;
-        const int cc = a - bb * (bb + 1) / 2;
+      if (rr < rows && a < d) {
+        const int bb = pb[a], cc = pc[a];
         const int c = c0 + rr;
         for (int e = con_ptr[c]; e < con_ptr[c + 1]; ++e) {
           const int t = con_trip[e];
@@ -255,51 +309,68 @@ k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
           }
         }
       }
-      G[rr * d + a] = v;
+      G[idx] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 24; ++s) {
-      const int e = threadIdx.x + s * USBS_NT;
-      if (e < dd) {
-        const int a = e / d, b = e % d;
-        if (a <= b) {
-          double v = acc[s];
-          for (int rr = 0; rr < GRAM_CH; ++rr) v += G[rr * d + a] * G[rr * d + b];
-          acc[s] = v;
+    for (int s = 0; s < MAXB; ++s) {
+      if (bi[s] >= 0) {
+        const int a0 = bi[s] * QB_BS, b0 = bj[s] * QB_BS;
+        for (int rr = 0; rr < GRAM_CH; ++rr) {
+          double ga[QB_BS], gb[QB_BS];
+#pragma unroll
+          for (int x = 0; x < QB_BS; ++x) {
+            ga[x] = G[rr * dp + a0 + x];
+            gb[x] = G[rr * dp + b0 + x];
+          }
+#pragma unroll
+          for (int x = 0; x < QB_BS; ++x)
+#pragma unroll
+            for (int y = 0; y < QB_BS; ++y) acc[s][x][y] += ga[x] * gb[y];
         }
       }
     }
     __syncthreads();
   }
+  double* out = partial + (long long)tl * d * d;
 #pragma unroll
-  for (int s = 0; s < 24; ++s) {
-    const int e = threadIdx.x + s * USBS_NT;
-    if (e < dd) partial[(long long)tl * dd + e] = acc[s];
+  for (int s = 0; s < MAXB; ++s) {
+    if (bi[s] >= 0) {
+      const int a0 = bi[s] * QB_BS, b0 = bj[s] * QB_BS;
+#pragma unroll
+      for (int x = 0; x < QB_BS; ++x)
+#pragma unroll
+        for (int y = 0; y < QB_BS; ++y) {
+          const int a = a0 + x, b = b0 + y;
+          if (a < d && b < d && a <= b) out[a * d + b] = acc[s][x][y];
+        }
+    }
   }
 }
 
-__global__ void k_reduce_sym(const double* __restrict__ partial, int ntiles, int d,
-                             double* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_reduce_sym(const double* __restrict__ partial, int ntiles,
+                                                     int d, double* __restrict__ out) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= d * d) return;
   const int a = e / d, b = e % d;
   const int src = a <= b ? a * d + b : b * d + a;
-  double s = 0.0;
-  for (int t = 0; t < ntiles; ++t) s += partial[(long long)t * d * d + src];
-  out[e] = s;
+  const double s = wave_tile_sum(partial, ntiles, (long long)d * d, src);
+  if ((threadIdx.x & 63) == 0) out[e] = s;
 }
 
 template <typename TV>
 int launch_qbase(const Problem& P, int k, const TV* V, long long ldv, double* partial, double* out,
                  hipStream_t s) {
   const int d = k * (k + 1) / 2;
-  if (d * d > 24 * USBS_NT) return (int)hipErrorInvalidValue;
-  k_qbase<TV><<<P.mntiles, USBS_NT, sizeof(double) * GRAM_CH * d, s>>>(
-      P.m, P.mtile, k, V, ldv, P.con_ptr, P.con_trip, P.ti, P.tj, P.tdata, partial);
+  const int dp = (d + QB_BS - 1) / QB_BS * QB_BS;
+  const int nbk = dp / QB_BS;
+  if (nbk * (nbk + 1) / 2 > 2 * USBS_NT) return (int)hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * GRAM_CH * dp + sizeof(int) * 2 * dp;
+  k_qbase<TV><<<P.mntiles, USBS_NT, lds, s>>>(P.m, P.mtile, k, V, ldv, P.con_ptr, P.con_trip,
+                                              P.ti, P.tj, P.tdata, partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
-  k_reduce_sym<<<(d * d + 255) / 256, 256, 0, s>>>(partial, P.mntiles, d, out);
+  k_reduce_sym<<<(d * d + 3) / 4, 256, 0, s>>>(partial, P.mntiles, d, out);
   return (int)hipGetLastError();
 }
 template int launch_qbase<double>(const Problem&, int, const double*, long long, double*, double*,
@@ -378,17 +449,13 @@ k_constraint_update(int m, int mtile, int k, int kc, const TF* __restrict__ F, l
   }
 }
 
-// Sum the first three partial columns over tiles, max the fourth; write to out[0..3].
-__global__ void k_reduce_cu(const double* __restrict__ partial, int ntiles, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
-  for (int t = 0; t < ntiles; ++t) {
-    a += partial[t * 4];
-    b += partial[t * 4 + 1];
-    c += partial[t * 4 + 2];
-    d = partial[t * 4 + 3] > d ? partial[t * 4 + 3] : d;
-  }
-  out[0] = a; out[1] = b; out[2] = c; out[3] = d;
+// Sum the first three partial columns over tiles, max the fourth; write to
+// out[0..3].  Wavefront w handles column w.
+__global__ void __launch_bounds__(256) k_reduce_cu(const double* __restrict__ partial, int ntiles,
+                                                    double* __restrict__ out) {
+  const int w = threadIdx.x >> 6;
+  const double v = (w < 3) ? wave_tile_sum(partial, ntiles, 4, w) : wave_tile_max(partial, ntiles, 4, 3);
+  if ((threadIdx.x & 63) == 0) out[w] = v;
 }
 
 template <typename TF>
@@ -401,7 +468,7 @@ int launch_constraint_update(const Problem& P, int k, int kc, const TF* F, long 
       ups, P.mask, P.scale_a, rho, z, zbn, ycand, partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
-  k_reduce_cu<<<1, 64, 0, s>>>(partial, P.mntiles, out4);
+  k_reduce_cu<<<1, 256, 0, s>>>(partial, P.mntiles, out4);
   return (int)hipGetLastError();
 }
 template int launch_constraint_update<double>(const Problem&, int, int, const double*, long long,
@@ -511,18 +578,25 @@ k_objective(int n, int tile, int k, const int* __restrict__ row_ptr, const int* 
 }
 
 // out[0] = sum over all k columns, out[1] = over the first kc columns.
-__global__ void k_reduce_objective(const double* __restrict__ partial, int ntiles, int k, int kc,
-                                   double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double a = 0.0, c = 0.0;
-  for (int x = 0; x < k; ++x) {
-    double s = 0.0;
-    for (int t = 0; t < ntiles; ++t) s += partial[t * 16 + x];
-    a += s;
-    if (x < kc) c += s;
+__global__ void __launch_bounds__(1024) k_reduce_objective(const double* __restrict__ partial,
+                                                           int ntiles, int k, int kc,
+                                                           double* __restrict__ out) {
+  __shared__ double col[16];
+  const int w = threadIdx.x >> 6;
+  if (w < k) {
+    const double v = wave_tile_sum(partial, ntiles, 16, w);
+    if ((threadIdx.x & 63) == 0) col[w] = v;
   }
-  out[0] = a;
-  out[1] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    for (int x = 0; x < k; ++x) {
+      a += col[x];
+      if (x < kc) c += col[x];
+    }
+    out[0] = a;
+    out[1] = c;
+  }
 }
 
 template <typename T>
@@ -534,7 +608,7 @@ int launch_objective(const Problem& P, int k, int kc, const T* F, long long ldf,
                                                 partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
-  k_reduce_objective<<<1, 64, 0, s>>>(partial, P.ntiles, k, kc, out2);
+  k_reduce_objective<<<1, 1024, 0, s>>>(partial, P.ntiles, k, kc, out2);
   return (int)hipGetLastError();
 }
 template int launch_objective<double>(const Problem&, int, int, const double*, long long, double*,
@@ -574,7 +648,7 @@ int launch_dots3(const Problem& P, const double* z, const double* y, const doubl
   k_dots3<<<P.mntiles, USBS_NT, 0, s>>>(P.m, P.mtile, z, y, P.b, ups, partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
-  k_reduce_cu<<<1, 64, 0, s>>>(partial, P.mntiles, out4);
+  k_reduce_cu<<<1, 256, 0, s>>>(partial, P.mntiles, out4);
   return (int)hipGetLastError();
 }
 
@@ -634,11 +708,10 @@ k_upsilon(int m, int mtile, int k, const TF* __restrict__ F, long long ldf,
   }
 }
 
-__global__ void k_reduce_max(const double* __restrict__ partial, int ntiles, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double v = 0.0;
-  for (int t = 0; t < ntiles; ++t) v = partial[t] > v ? partial[t] : v;
-  *out = v;
+__global__ void __launch_bounds__(64) k_reduce_max(const double* __restrict__ partial, int ntiles,
+                                                    double* __restrict__ out) {
+  const double v = wave_tile_max(partial, ntiles, 1, 0);
+  if (threadIdx.x == 0) *out = v;
 }
 
 template <typename TF>

@@ -50,7 +50,7 @@ __global__ void k_lz_start(LzInst<T> L, const double* q0) {
   if (i < L.n) L.Q[i] = (T)q0[i];
   if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
   if (i < LZ_NCTRL) L.ctrl[i] = 0;
-  if (i < 64) L.bar[i] = 0u;
+  for (long long x = i; x < L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
 }
 
 // Penalised dual objective of the initial point (solver/usbs.py:704-706).
@@ -99,8 +99,15 @@ struct Solver {
   int nb = 1;
   int host_ctrl[LZ_NCTRL] = {0};
   int lz_restarts = 0;
+  hipStream_t s2 = nullptr;          // side stream: the lower-bound IPM overlaps the Lanczos call
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
-  ~Solver() { ctx_free_pool(pool); }
+  ~Solver() {
+    if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    ctx_free_pool(pool);
+  }
 
   template <typename X>
   X* alloc(size_t count) {
@@ -154,19 +161,24 @@ struct Solver {
     rc = make_lanczos<T>(c, pool, *P, k, k, 1, 0.0, (const T*)nullptr, qr);
     if (rc) return rc;
     if (lz.ldq != ld || qr.ldq != ld) return set_err(c, USBS_ERR_INVALID, "leading dimension mismatch");
+    if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
     if (p.exec_mode == USBS_EXEC_PERSISTENT) {
-      nb = std::min(lz.ntiles, std::max(1, c->coresident));
+      nb = std::min(std::min(lz.ntiles, 64), std::max(1, c->coresident));
       lz.nblocks = nb;
-      if (!(d_lz_k = alloc<LzInst<T>>(1)) || !(d_lz_kc = alloc<LzInst<T>>(1)) || !(d_map = alloc<int>(2 * nb)))
+      if (!(d_lz_k = alloc<LzInst<T>>(1)) || !(d_lz_kc = alloc<LzInst<T>>(1)) || !(d_map = alloc<int>(2 * (nb + lz.ntiles))))
         return USBS_ERR_OOM;
-      std::vector<int2> map(nb);
+      std::vector<int2> map(nb + lz.ntiles);
       for (int i = 0; i < nb; ++i) map[i] = make_int2(0, i);
+      for (int i = 0; i < lz.ntiles; ++i) map[nb + i] = make_int2(0, i);
       LzInst<T> a = lz, b = lz;
       a.nd = k;
       b.nd = kc;
       if (hipMemcpy(d_lz_k, &a, sizeof(a), hipMemcpyHostToDevice) != hipSuccess ||
           hipMemcpy(d_lz_kc, &b, sizeof(b), hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(d_map, map.data(), sizeof(int2) * nb, hipMemcpyHostToDevice) != hipSuccess)
+          hipMemcpy(d_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice) != hipSuccess)
         return set_err(c, USBS_ERR_HIP, "upload of Lanczos descriptors failed");
     }
     return USBS_OK;
@@ -194,7 +206,8 @@ struct Solver {
       CK(lanczos_perop<T>(L, s, host_ctrl));
       lz_restarts = host_ctrl[LZ_RESTART];
     } else {
-      CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, &nb, mL, s, d_map, 1ull << 26));
+      CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
+                               d_map + 2 * nb, lz.ntiles, 1ull << 26));
     }
     return USBS_OK;
   }
@@ -205,11 +218,12 @@ struct Solver {
   }
 
   // out (k x k) = V^T M V for the operator currently in op64.
-  int project(double* out) {
-    CK((launch_spmm<double, T>(n, P->row_ptr, P->col, op64, V(), ld, k, W, ld, s)));
-    CK((launch_tsgram<T, double>(n, P->tile, P->ntiles, V(), ld, k, W, ld, k, partial, out, s)));
+  int project(double* out, hipStream_t st) {
+    CK((launch_spmm<double, T>(n, P->row_ptr, P->col, op64, V(), ld, k, W, ld, st)));
+    CK((launch_tsgram<T, double>(n, P->tile, P->ntiles, V(), ld, k, W, ld, k, partial, out, st)));
     return USBS_OK;
   }
+  int project(double* out) { return project(out, s); }
 
   int rounding() {
     if (r <= 0 || !p.rounding) return USBS_OK;
@@ -265,10 +279,15 @@ struct Solver {
     CK(launch_objective<T>(*P, k, kc, F, ld, partial, sc + SC_OBJF, s));
     // --- candidate eigenvectors (solver/usbs.py:588-599)
     CK(assemble_op(-1.0, ycand, 1.0, nullptr, 0.0, nullptr, 0.0, sizeof(T) == 4));
+    // --- spectral lower bound with the old V (solver/usbs.py:606-621) on the
+    // side stream, overlapping the Lanczos call (it needs only ycand)
+    CK((int)hipEventRecord(ev_fork, s));
+    CK((int)hipStreamWaitEvent(s2, ev_fork, 0));
+    if (project(Glb, s2)) return USBS_ERR_HIP;
+    CK(launch_ipm_lb(k, p.subprob_max_iters, tu, p.subprob_eps, Glb, sc, s2));
+    CK((int)hipEventRecord(ev_join, s2));
     if (lanczos(kc)) return USBS_ERR_HIP;
-    // --- spectral lower bound with the old V (solver/usbs.py:606-621)
-    if (project(Glb)) return USBS_ERR_HIP;
-    CK(launch_ipm_lb(k, p.subprob_max_iters, tu, p.subprob_eps, Glb, sc, s));
+    CK((int)hipStreamWaitEvent(s, ev_join, 0));
     ScalarArgs sa{};
     sa.tu = tu; sa.tf = p.trace_factor; sa.beta = p.beta; sa.SC = p.SCALE_C; sa.SX = p.SCALE_X;
     sa.lanczos_alpha = lz.alpha;
