@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the USBS hot path on one MI355X.
+
+Workload: the max-cut SDP of a 2-D toroidal grid 100 x 200 (n = 20 000
+vertices, 40 000 edges, weights uniform on {-1, +1} from seed 0), solved by
+usbs() with k_curr = 10, k_past = 1, sketch_dim = 10, trace_factor = 2,
+rho = 0.01, beta = 0.25, Lanczos 32 inner iterations x 10 restarts at
+tolerance 1e-7, subproblem eps 1e-7 / 100 iterations.
+
+A step is one USBS iteration (prox-bundle IPM, sketch update, objective and
+infeasibility, Lanczos oracle on -C + A*(ycand), lower-bound IPM, descent
+test, bundle QR, max-cut rounding and the stop test).  The timed region is
+one usbs() call of K iterations on a context whose problem already sits in
+device memory; W warm-up iterations run first in a separate call.
+
+Printed (one JSON line): USBS iterations per second of the mixed-precision
+persistent form ("value"), ms per iteration, the plain FP64 per-op form on
+the same K iterations and the speed-up, the objective gap reached, the
+roofline of the Lanczos step kernel measured with HIP events on the
+solver's stream during the timed call, and the FP64 CPU restatement (the
+oracle) timed on the host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "time-to-1e-5 rel gap: mixed/batched vs plain FP64 on GPU; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lz_steps_coop_torus100x200_traffic.txt")
+
+SOLVER = dict(trace_factor=2.0, rho=0.01, beta=0.25, k_curr=10, k_past=1, lanczos_inner=32,
+              lanczos_restarts=10, subprob_eps=1e-7, subprob_max_iters=100, sketch_dim=10)
+
+
+def workload():
+    from usbs_amd import problems
+    W = problems.torus(100, 200, seed=0, signed=True)
+    n = W.shape[0]
+    Om = problems.gaussian_matrix(n, SOLVER["sketch_dim"], 1)
+    q0 = problems.unit_start_vector(n, 2)
+    return W, n, Om, q0
+
+
+def run_gpu(ctx, state, n, q0, iters, precision, exec_mode, timing=False):
+    from usbs_amd import maxcut_helpers
+    from usbs_amd.usbs import usbs
+    s = SOLVER
+    log = []
+    if timing:
+        ctx.set_timing(True)
+    t0 = time.perf_counter()
+    usbs(state, n, n, s["trace_factor"] * n, s["trace_factor"], s["rho"], s["beta"], s["k_curr"],
+         s["k_past"], iters, float("inf"), -float("inf"), -float("inf"), -float("inf"),
+         s["lanczos_inner"], s["lanczos_restarts"], s["subprob_eps"], s["subprob_max_iters"], 1.0,
+         maxcut_helpers.compute_max_cut, None, state.C, precision=precision, exec_mode=exec_mode,
+         q0=q0, log=log, context=ctx, upload=False)
+    el = time.perf_counter() - t0
+    stats = ctx.stats() if timing else None
+    if timing:
+        ctx.set_timing(False)
+    return el, log, stats
+
+
+def gap_marks(log):
+    marks = {}
+    for thr in (1e-1, 1e-2, 1e-3, 1e-4, 1e-5):
+        hit = [r for r in log if r["t"] > 0 and r["obj_gap"] <= thr]
+        marks["%g" % thr] = ({"iteration": int(hit[0]["t"]), "seconds": round(hit[0]["time"], 4),
+                              "infeas_gap": hit[0]["infeas_gap"]} if hit else None)
+    return marks
+
+
+def roofline(stats):
+    """Algorithmic bytes per launch of the persistent Lanczos step kernel:
+    per step the CSR SpMV nnz*(s+4) + 4(n+1) + 2 n s plus CGS2 over c valid
+    columns (4c + 6) n s, with s the bytes per stored value."""
+    L = stats["lz_launches"]
+    if not L or stats["lz_kernel_ms"] <= 0:
+        return None
+    n, nnz, s = stats["n"], stats["nnz"], stats["value_bytes"]
+    steps, sumc = stats["lz_steps"], stats["lz_sum_cols"]
+    spmv = nnz * (s + 4) + 4 * (n + 1) + 2 * n * s
+    total = steps * spmv + (4 * sumc + 6 * steps) * n * s
+    per_launch = total / L
+    avg_s = stats["lz_kernel_ms"] / L / 1e3
+    achieved = per_launch / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "kernel": "k_lz_steps_coop (all Lanczos steps of one restart)",
+            "avg_launch_us": round(avg_s * 1e6, 2), "launches": L,
+            "steps_per_launch": round(steps / L, 3), "algorithmic_bytes_per_launch": int(per_launch)}
+
+
+def cpu_baseline(iters):
+    """The FP64 CPU restatement (oracle) on the same workload, `iters`
+    iterations including the initial Lanczos call, BLAS threads capped."""
+    import threadpoolctl
+    from oracle import maxcut as omc, usbs as ousbs
+    W, n, Om, q0 = workload()
+    cores = min(16, len(os.sched_getaffinity(0)))
+    s = SOLVER
+    with threadpoolctl.threadpool_limits(limits=cores):
+        st = omc.initialize_state(W, s["sketch_dim"], Om)
+        log = []
+        t0 = time.perf_counter()
+        ousbs.usbs(st, n, n, s["trace_factor"] * n, s["trace_factor"], s["rho"], s["beta"],
+                   s["k_curr"], s["k_past"], iters, 1e300, -1e300, -1e300, -1e300,
+                   s["lanczos_inner"], s["lanczos_restarts"], s["subprob_eps"],
+                   s["subprob_max_iters"], 1.0, omc.compute_max_cut, None, st.C, q0, log=log)
+        el = time.perf_counter() - t0
+    return {"value": round(iters / el, 4), "unit": "USBS iterations/s", "cores": cores,
+            "kind": "port",
+            "sample": "FP64 NumPy/SciPy restatement, torus 100x200, %d iterations incl. the "
+                      "initial Lanczos call (%.1f s), BLAS threads %d" % (iters, el, cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300, help="timed USBS iterations")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed USBS iterations")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--exec-mode", default="persistent", choices=["persistent", "perop"])
+    ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--cpu-iters", type=int, default=15, help="oracle iterations (0 = skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 20 device-to-device copies of MIB MiB (the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 20}))
+        return
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py measures one GPU: the USBS hot path here runs one solve per GPU and "
+              "no multi-GPU form is built (requested --gpus %d, WORLD_SIZE %d)"
+              % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    from usbs_amd import _lib, maxcut_helpers
+    from usbs_amd.usbs import upload_problem
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except ImportError:
+        sync = lambda: None  # noqa: E731
+
+    W, n, Om, q0 = workload()
+    ctx = _lib.Context(0)
+    state = maxcut_helpers.initialize_state(W, SOLVER["sketch_dim"], Omega=Om)
+    upload_problem(ctx, state, n, n)
+
+    if args.warmup > 0:
+        run_gpu(ctx, state, n, q0, args.warmup, args.precision, args.exec_mode)
+    sync()
+    el, log, stats = run_gpu(ctx, state, n, q0, args.steps, args.precision, args.exec_mode,
+                             timing=(args.exec_mode == "persistent"))
+    sync()
+    K = int(log[-1]["t"])
+    ms = 1e3 * el / max(K, 1)
+    out = {"metric": METRIC, "value": round(K / el, 3), "unit": "USBS iterations/s",
+           "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 storage / f64 accumulate" if args.precision == "mixed" else "f64",
+           "data": "synthetic (seeded torus graph, seeded Omega and q0)",
+           "config": {"workload": "max-cut SDP, torus 100x200 (n=20000, 40000 edges, "
+                                  "w in {-1,+1}, seed 0)",
+                      "precision": args.precision, "exec_mode": args.exec_mode,
+                      "step": "one USBS iteration", **SOLVER}}
+    last = log[-1]
+    out["gap"] = {"obj_gap_end": last["obj_gap"], "infeas_gap_end": last["infeas_gap"],
+                  "max_infeas_end": last["max_infeas"], "cut_value_end": last["callback_val"],
+                  "time_to_gap": gap_marks(log), "target_1e-5_reached":
+                      gap_marks(log)["1e-05"] is not None}
+    if not args.no_fp64_leg:
+        el64, log64, _ = run_gpu(ctx, state, n, q0, args.steps, "fp64", "perop")
+        K64 = int(log64[-1]["t"])
+        ms64 = 1e3 * el64 / max(K64, 1)
+        out["plain_fp64"] = {"exec_mode": "perop", "ms_per_step": round(ms64, 4),
+                             "speedup_of_value": round(ms64 / ms, 3),
+                             "obj_gap_end": log64[-1]["obj_gap"],
+                             "primal_rel_diff_end": abs(log64[-1]["primal_obj"] - last["primal_obj"])
+                             / max(1.0, abs(log64[-1]["primal_obj"]))}
+    out["roofline"] = roofline(stats) if stats else None
+    out["cpu_baseline"] = cpu_baseline(args.cpu_iters) if args.cpu_iters > 0 else None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -151,6 +151,28 @@ int usbs_objective_infeas(usbs_ctx* ctx, int32_t k, int32_t kc, const double* F,
                           const double* zbar, const double* y, const double* upsilon, double rho,
                           double* z_out, double* zbar_next_out, double* ycand_out, double* out6);
 
+/*
+ * Live kernel timing (measurement support; no counterpart in the reference).
+ * While enabled, usbs_solve brackets every persistent Lanczos step launch
+ * with HIP events on the context's stream and accumulates the duration of
+ * the launches that ran at least one step, with their step counts, so that a
+ * caller can divide algorithmic bytes by measured kernel time.
+ */
+typedef struct usbs_stats {
+  double lz_kernel_ms;  /* summed duration of those launches                        */
+  int64_t lz_launches;  /* number of launches that ran steps                         */
+  int64_t lz_steps;     /* Lanczos steps they ran                                    */
+  int64_t lz_sum_cols;  /* sum over those steps of the valid basis column count i+1  */
+  int64_t n, nnz;       /* operator size of the uploaded problem (merged pattern)     */
+  int32_t value_bytes;  /* bytes per stored basis / operator value of the last solve */
+  int32_t reserved;
+} usbs_stats;
+
+/* Enable (1) or disable (0) the timing; enabling resets the counters. */
+int usbs_set_timing(usbs_ctx* ctx, int enable);
+/* Copy the counters accumulated since timing was enabled. */
+int usbs_get_stats(const usbs_ctx* ctx, usbs_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,14 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
 
 
+class usbs_stats(ctypes.Structure):
+    """Timing counters of the persistent Lanczos step kernel (usbs_get_stats)."""
+    _fields_ = [("lz_kernel_ms", ctypes.c_double), ("lz_launches", ctypes.c_int64),
+                ("lz_steps", ctypes.c_int64), ("lz_sum_cols", ctypes.c_int64),
+                ("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("value_bytes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class USBSError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("usbs-hip error %d (%s): %s" % (code, ERRORS.get(code, "?"), msg))
@@ -44,6 +52,8 @@ def _declare(lib):
     lib.usbs_lanczos_smallest.argtypes = [_vp, ctypes.c_double, _dp, _dp, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
+    lib.usbs_set_timing.argtypes = [_vp, ctypes.c_int]
+    lib.usbs_get_stats.argtypes = [_vp, ctypes.POINTER(usbs_stats)]
     for name, args in EXTRA_SIGNATURES.items():
         if hasattr(lib, name):
             getattr(lib, name).argtypes = args
@@ -115,6 +125,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_timing(self, enable):
+        """Bracket persistent Lanczos launches with HIP events (resets counters)."""
+        self.check(self._lib.usbs_set_timing(self.h, 1 if enable else 0))
+
+    def stats(self):
+        st = usbs_stats()
+        self.check(self._lib.usbs_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in usbs_stats._fields_ if f != "reserved"}
 
     def set_problem(self, n, m, C_rows, C_cols, C_vals, A_indices, A_data, b, b_ineq_mask, SCALE_A):
         C_rows, C_cols, C_vals = as_i32(C_rows), as_i32(C_cols), as_f64(C_vals)

@@ -142,9 +142,14 @@ __device__ __forceinline__ double counter_normal(uint64_t key, uint64_t idx) {
 // Device-wide barrier among `nblocks` workgroups that share `counter`
 // (monotonic, zeroed before the launch).  The caller's workgroups must all be
 // resident (cooperative launch).  `target` is kept by the caller and grows by
-// nblocks each call.  A bounded wait: after `spin_limit` polls the barrier
-// records an error in *err and every later barrier returns at once, so a
-// wait that can never complete ends instead of hanging the device.
+// nblocks each call.  One release fence before the arrival and one acquire
+// fence after the wait: the L2 of an XCD is not coherent with the others, so
+// each agent-scope fence writes back / invalidates it, and the polling loop
+// itself uses relaxed loads so that waiting does not flush the cache that the
+// XCD's other workgroups are still working from.  A bounded wait: after
+// `spin_limit` polls the barrier records an error in *err and every later
+// barrier returns at once, so a wait that can never complete ends instead of
+// hanging the device.
 __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned nblocks, unsigned& target,
                                              int* err, unsigned long long spin_limit) {
   __syncthreads();
@@ -156,9 +161,9 @@ __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned nblocks
       ok = 0;
     } else {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       unsigned long long polls = 0;
-      while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         __builtin_amdgcn_s_sleep(1);
         if (++polls > spin_limit) {
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

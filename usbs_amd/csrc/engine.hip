@@ -190,6 +190,7 @@ int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int
   if (!(L.ctrl = (int*)ctx_alloc(c, pool, sizeof(int) * LZ_NCTRL))) return USBS_ERR_OOM;
   L.nbar = max_restarts + 1 > 64 ? max_restarts + 1 : 64;
   if (!(L.bar = (unsigned*)ctx_alloc(c, pool, sizeof(unsigned) * L.nbar))) return USBS_ERR_OOM;
+  if (!(L.lstat = (int*)ctx_alloc(c, pool, sizeof(int) * 2 * L.nbar))) return USBS_ERR_OOM;
   return USBS_OK;
 }
 
@@ -210,6 +211,7 @@ __global__ void k_lz_reset(LzInst<T> L, const double* q0) {
   if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
   if (i < LZ_NCTRL) L.ctrl[i] = 0;
   for (long long x = i; x < L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
+  for (long long x = i; x < 2 * L.nbar; x += (long long)gridDim.x * blockDim.x) L.lstat[x] = 0;
 }
 
 template <typename T>
@@ -254,7 +256,8 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
     rc = lanczos_perop<T>(L, c->stream, host_ctrl);
     if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos per-op: ") + hipGetErrorString((hipError_t)rc));
   } else {
-    int nb = std::min(L.ntiles, std::max(1, c->coresident));
+    int nb = std::min(std::min(L.ntiles, c->persist_wgs > 0 ? c->persist_wgs : L.ntiles),
+                      std::max(1, c->coresident));
     L.nblocks = nb;
     LzInst<T>* d_inst = (LzInst<T>*)ctx_alloc(c, pool, sizeof(LzInst<T>));
     int* d_map = (int*)ctx_alloc(c, pool, sizeof(int2) * (nb + L.ntiles));
@@ -311,6 +314,7 @@ int usbs_create(usbs_ctx** out, int device) {
     return USBS_ERR_HIP;
   }
   h->c.coresident = lanczos_max_coresident_blocks(32, device);
+  if (const char* e = getenv("USBS_PERSIST_WGS")) h->c.persist_wgs = atoi(e);
   *out = h;
   return USBS_OK;
 }

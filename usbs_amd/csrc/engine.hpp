@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/usbs_hip.h"
 #include "lanczos.hpp"
 
 namespace usbs {
@@ -54,6 +55,9 @@ struct Ctx {
   std::vector<DevBuf> prob_bufs;  // freed on new problem / destroy
   std::vector<DevBuf> work_bufs;  // solve workspaces
   int coresident = 0;             // max co-resident Lanczos workgroups on this device
+  int persist_wgs = 0;            // workgroups per persistent Lanczos instance (0 = one per tile)
+  bool timing = false;            // bracket persistent Lanczos launches with events
+  usbs_stats stats{};
 };
 
 // error helpers

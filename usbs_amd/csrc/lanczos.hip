@@ -404,6 +404,12 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
   unsigned target = 0;
   const int k = L.ctrl[LZ_K];
   bool ok = true;
+  if (lw == 0 && threadIdx.x == 0 && r < L.nbar) {
+    int sumc = 0;
+    for (int i = k; i < L.m; ++i) sumc += i + 1;
+    L.lstat[2 * r] = L.m - k;
+    L.lstat[2 * r + 1] = sumc;
+  }
   for (int i = k; i < L.m && ok; ++i) {
     for (int tl = lw; tl < L.ntiles; tl += G) tile_phaseA<T, MAXC>(L, tl, i, 0, lds);
     if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
@@ -441,13 +447,16 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
 template <typename T, int MAXC>
 static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                            int max_restarts, hipStream_t s, const int* dev_wgmap,
-                           const int* dev_tilemap, int total_tiles, unsigned long long spin_limit) {
+                           const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
+                           hipEvent_t* evs) {
   const size_t lds = sizeof(double) * lds_doubles_for(m, MAXC);
   for (int r = 0; r < max_restarts; ++r) {
     void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&r, (void*)&spin_limit};
+    if (evs && hipEventRecord(evs[2 * r], s) != hipSuccess) return (int)hipErrorUnknown;
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, MAXC>, dim3(total_wgs),
                                               dim3(USBS_NT), args, lds, s);
     if (e != hipSuccess) return (int)e;
+    if (evs && hipEventRecord(evs[2 * r + 1], s) != hipSuccess) return (int)hipErrorUnknown;
     k_lz_ritz_multi<T><<<ninst, LZ_RITZ_NT, lds, s>>>(dev_insts, r);
     k_lz_rotate_multi<T, MAXC><<<total_tiles, USBS_NT, lds, s>>>(dev_insts, (const int2*)dev_tilemap, r);
     e = hipGetLastError();
@@ -459,16 +468,17 @@ static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs,
 template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
-                       const int* dev_tilemap, int total_tiles, unsigned long long spin_limit) {
+                       const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
+                       hipEvent_t* evs) {
   if (m + 1 <= 17)
     return persistent_impl<T, 17>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit);
+                                  dev_tilemap, total_tiles, spin_limit, evs);
   if (m + 1 <= 33)
     return persistent_impl<T, 33>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit);
+                                  dev_tilemap, total_tiles, spin_limit, evs);
   if (m + 1 <= 49)
     return persistent_impl<T, 49>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit);
+                                  dev_tilemap, total_tiles, spin_limit, evs);
   return (int)hipErrorInvalidValue;
 }
 
@@ -500,8 +510,8 @@ int lanczos_max_coresident_blocks(int m, int device) {
 template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*);
 template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*);
 template int lanczos_persistent<double>(const LzInst<double>*, int, int, int, int, hipStream_t,
-                                        const int*, const int*, int, unsigned long long);
+                                        const int*, const int*, int, unsigned long long, hipEvent_t*);
 template int lanczos_persistent<float>(const LzInst<float>*, int, int, int, int, hipStream_t,
-                                       const int*, const int*, int, unsigned long long);
+                                       const int*, const int*, int, unsigned long long, hipEvent_t*);
 
 }  // namespace usbs

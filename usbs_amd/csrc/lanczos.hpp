@@ -42,6 +42,8 @@ struct LzInst {
   int* ctrl;     // LZ_NCTRL
   unsigned* bar; // barrier counters, one per restart (persistent launch)
   int nbar;      // number of counters in bar (>= max_restarts)
+  int* lstat;    // 2 x nbar: per persistent launch r, steps done and the sum
+                 // over those steps of the valid column count i+1
 };
 
 int tile_rows_for(int n);
@@ -53,10 +55,12 @@ int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl);
 // one cooperative launch of all Lanczos steps (dev_wgmap: int2 per workgroup
 // = instance, local workgroup), then the Ritz kernel (one block per
 // instance) and the rotation (dev_tilemap: int2 per tile = instance, tile).
+// evs (optional, 2 x max_restarts events) brackets each cooperative launch.
 template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
-                       const int* dev_tilemap, int total_tiles, unsigned long long spin_limit);
+                       const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
+                       hipEvent_t* evs = nullptr);
 int lanczos_max_coresident_blocks(int m, int device);
 template <typename T>
 int orth_perop(const LzInst<T>& L, hipStream_t s);

@@ -51,6 +51,7 @@ __global__ void k_lz_start(LzInst<T> L, const double* q0) {
   if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
   if (i < LZ_NCTRL) L.ctrl[i] = 0;
   for (long long x = i; x < L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
+  for (long long x = i; x < 2 * L.nbar; x += (long long)gridDim.x * blockDim.x) L.lstat[x] = 0;
 }
 
 // Penalised dual objective of the initial point (solver/usbs.py:704-706).
@@ -101,12 +102,38 @@ struct Solver {
   int lz_restarts = 0;
   hipStream_t s2 = nullptr;          // side stream: the lower-bound IPM overlaps the Lanczos call
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  std::vector<hipEvent_t> tev;       // timing: 2 events per persistent launch
+  bool tev_pending = false;
+  std::vector<int> lstat_h;
 
   ~Solver() {
     if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
     ctx_free_pool(pool);
+  }
+
+  // Fold the timed launches of the last persistent Lanczos call into the
+  // context's counters (waits for that call to finish).
+  int collect_timing() {
+    if (!tev_pending) return USBS_OK;
+    tev_pending = false;
+    const int R = lz.max_restarts;
+    if (hipEventSynchronize(tev[2 * R - 1]) != hipSuccess ||
+        hipMemcpy(lstat_h.data(), lz.lstat, sizeof(int) * 2 * R, hipMemcpyDeviceToHost) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "reading Lanczos timing");
+    for (int q = 0; q < R; ++q) {
+      if (lstat_h[2 * q] <= 0) continue;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, tev[2 * q], tev[2 * q + 1]) != hipSuccess)
+        return set_err(c, USBS_ERR_HIP, "event timing");
+      c->stats.lz_kernel_ms += ms;
+      c->stats.lz_launches += 1;
+      c->stats.lz_steps += lstat_h[2 * q];
+      c->stats.lz_sum_cols += lstat_h[2 * q + 1];
+    }
+    return USBS_OK;
   }
 
   template <typename X>
@@ -165,8 +192,18 @@ struct Solver {
         hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
+    if (p.exec_mode == USBS_EXEC_PERSISTENT && c->timing) {
+      tev.assign(2 * lz.max_restarts, nullptr);
+      lstat_h.assign(2 * lz.max_restarts, 0);
+      for (auto& e : tev)
+        if (hipEventCreate(&e) != hipSuccess) return set_err(c, USBS_ERR_HIP, "event creation failed");
+      c->stats.n = n;
+      c->stats.nnz = P->nnz;
+      c->stats.value_bytes = (int)sizeof(T);
+    }
     if (p.exec_mode == USBS_EXEC_PERSISTENT) {
-      nb = std::min(std::min(lz.ntiles, 64), std::max(1, c->coresident));
+      nb = std::min(std::min(lz.ntiles, c->persist_wgs > 0 ? c->persist_wgs : lz.ntiles),
+                    std::max(1, c->coresident));
       lz.nblocks = nb;
       if (!(d_lz_k = alloc<LzInst<T>>(1)) || !(d_lz_kc = alloc<LzInst<T>>(1)) || !(d_map = alloc<int>(2 * (nb + lz.ntiles))))
         return USBS_ERR_OOM;
@@ -206,8 +243,13 @@ struct Solver {
       CK(lanczos_perop<T>(L, s, host_ctrl));
       lz_restarts = host_ctrl[LZ_RESTART];
     } else {
+      if (tev_pending) {
+        const int rc = collect_timing();
+        if (rc) return rc;
+      }
       CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
-                               d_map + 2 * nb, lz.ntiles, 1ull << 26));
+                               d_map + 2 * nb, lz.ntiles, 1ull << 26, tev.empty() ? nullptr : tev.data()));
+      tev_pending = !tev.empty();
     }
     return USBS_OK;
   }
@@ -317,6 +359,10 @@ struct Solver {
     if (hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "reading the scalar block");
+    if (tev_pending) {
+      const int rc = collect_timing();
+      if (rc) return rc;
+    }
     if (p.exec_mode == USBS_EXEC_PERSISTENT) {
       int ctrl[LZ_NCTRL];
       if (hipMemcpy(ctrl, lz.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost) != hipSuccess)
@@ -522,6 +568,19 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
 }  // namespace usbs
 
 using namespace usbs;
+
+extern "C" int usbs_set_timing(usbs_ctx* h, int enable) {
+  if (!h) return USBS_ERR_INVALID;
+  h->c.timing = enable != 0;
+  if (enable) h->c.stats = usbs_stats{};
+  return USBS_OK;
+}
+
+extern "C" int usbs_get_stats(const usbs_ctx* h, usbs_stats* out) {
+  if (!h || !out) return USBS_ERR_INVALID;
+  *out = h->c.stats;
+  return USBS_OK;
+}
 
 extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0, const double* z0,
                           double tr_X0, double primal_obj0, const double* Omega, const double* P0,
