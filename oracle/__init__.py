@@ -13,10 +13,8 @@ Module map (reference file:line each function follows is in its docstring):
 * ``loop``       -- utils/loop.py:21-115 (bounded while-loop semantics)
 * ``ipm``        -- solver/usbs.py:25-433 (prox-bundle IPM subproblems)
 * ``usbs``       -- solver/usbs.py:436-789 (the USBS outer loop)
-* ``cgal``       -- solver/cgal.py:21-243 (CGAL baseline)
 * ``common``     -- utils/common.py:11-121 (SDPState, scaling, Nystrom)
 * ``maxcut``     -- utils/maxcut_helpers.py:26-92, 292-302
-* ``qap``        -- utils/qap_helpers.py:157-381 (lifted QAP operator)
 
 Pinning status of each function (what checks it besides its own formula) is
 listed in ``PINS`` below; tests/test_oracle_*.py implement those checks.
@@ -36,12 +34,9 @@ PINS = {
     "lanczos.eigsh_smallest": "numpy.linalg.eigh of the dense operator; torus closed form",
     "lanczos._build_arrowhead_matrix": "hand-worked m<=6 arrowhead cases",
     "lanczos._pick_saved_ritz_values": "hand-worked cases of the gamma-ratio rule",
-    "ipm.solve_quad_subprob_ipm": "scipy SLSQP on the same convex QP over (S>=0, eta>=0, trS+eta<=1), k=1 closed form",
+    "ipm.solve_quad_subprob_ipm": "scipy SLSQP on the same convex QP over (S>=0, eta>=0, trS+eta<=1), k=2",
     "ipm.compute_lb_spec_est_ipm": "closed form: lower bound of a linear objective over the spectraplex-with-eta",
     "usbs.usbs": "SDP closed forms: bipartite |E|, K_n n^2/4, odd cycle (n/2)(1+cos(pi/n))",
-    "cgal.cgal": "SDP closed forms (bipartite, K_n) at coarse tolerance",
     "common.reconstruct_from_sketch": "exact recovery of a rank<=r PSD matrix (Tropp et al. 2017)",
     "maxcut.compute_max_cut": "brute force over all 2^n cuts on n<=14; integer edge-sum identity",
-    "qap.build_objective_matrix": "dense numpy.kron(W, D) on the padded index set",
-    "qap.get_all_problem_data": "feasibility of permutation matrices: A(X_perm) = b for every family",
 }
