@@ -45,14 +45,18 @@ def test_lanczos_fp64_matches_oracle(gpu_ctx, exec_mode):
     np.testing.assert_allclose(V_g.T @ V_g, np.eye(10), atol=1e-10)
 
 
-def test_persistent_bitwise_equals_perop(gpu_ctx):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_persistent_bitwise_equals_perop(gpu_ctx, precision):
+    """The persistent launch (FP32 basis: tiles cached in LDS; FP64: read
+    from global memory) reproduces the per-op kernels bit for bit."""
     W = problems.torus(20, 30, seed=1)
     negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 11)
     n = negC.shape[0]
+    print("case: torus 20x30 bitwise perop == persistent, %s" % precision, flush=True)
     a = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 10, 32, 10, 1e-7, exec_mode="perop",
-                       context=gpu_ctx)
+                       precision=precision, context=gpu_ctx)
     b = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 10, 32, 10, 1e-7,
-                       exec_mode="persistent", context=gpu_ctx)
+                       exec_mode="persistent", precision=precision, context=gpu_ctx)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 

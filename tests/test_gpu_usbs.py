@@ -51,12 +51,25 @@ def test_usbs_fp64_trajectory_matches_oracle(gpu_ctx, exec_mode):
 
 def test_usbs_reference_config_er200(gpu_ctx):
     """k_curr=10, k_past=1, sketch 10, rho=0.01 (the reference's max-cut
-    settings): the first two iterations agree with the oracle."""
+    settings): the first two iterations agree with the oracle to 1e-6
+    relative, or to 20x the oracle's own response to a rounding-level
+    perturbation of q0 when that is larger.  At rho = 0.01 the dual step
+    (b - z)/rho amplifies rounding: a 1e-15 relative change of q0 moves the
+    oracle's infeasibility gap by ~1e-5 at the second iteration."""
     W = problems.erdos_renyi(200, 0.06, seed=0)
+    n = W.shape[0]
     log_o, log_g, _ = _run_both(W, 2, 10, 1, 10, 0.01, ctx=gpu_ctx)
-    for a, b in zip(log_o[1:], log_g[1:]):
+    q0 = problems.unit_start_vector(n, 2)
+    q1 = q0 + 1e-15 * np.random.default_rng(5).standard_normal(n)
+    so = omc.initialize_state(W, 10, problems.gaussian_matrix(n, 10, 1))
+    log_p = []
+    ousbs.usbs(so, n, n, 2.0 * n, HP["trace_factor"], 0.01, HP["beta"], 10, 1, 2, HP["max_time"],
+               -1.0, -1.0, -1.0, 32, 10, 1e-7, 100, 1.0, omc.compute_max_cut, None, so.C,
+               q1 / np.linalg.norm(q1), log=log_p)
+    for a, b, c in zip(log_o[1:], log_g[1:], log_p[1:]):
         for key in ("primal_obj", "obj_ub", "obj_gap", "infeas_gap", "max_infeas"):
-            np.testing.assert_allclose(b[key], a[key], rtol=1e-5, atol=1e-9, err_msg=key)
+            tol = max(1e-6 * abs(a[key]), 20.0 * abs(a[key] - c[key]), 1e-12)
+            assert abs(b[key] - a[key]) <= tol, (key, int(a["t"]), b[key], a[key], tol)
         assert b["callback_val"] == a["callback_val"]
 
 

@@ -121,108 +121,177 @@ __device__ int dense_eigh(double* A, int m, double* w, double* V, double* tmp, d
 }
 
 // ---------------------------------------------------------------------------
-// Solve A x = b by Gaussian elimination with partial pivoting (first maximal
-// |pivot|, as LAPACK's idamax; same pivots and elimination as getrf/getrs).
-// A (d x d) and b are destroyed; b returns x.  Column k is eliminated with
-// the pivot row in place, then wave 0 swaps it into position k and searches
-// the next pivot, so each column costs two block barriers.  A zero pivot
-// yields inf/NaN, as getrs does.
-__device__ void dense_lu_solve1(double* A, int d, double* b, int* ired) {
+// The same parallel cyclic Jacobi for m <= 64 with the data laid out for a
+// whole block: each round, threads t < M/2 compute the rotations of the
+// round-robin pairing, then every thread applies the two-sided rotation to
+// whole 2x2 blocks (pair a rows x pair b columns) and the one-sided rotation
+// to (row, pair) entries of V, with one barrier after each half.  Rows are
+// padded (lda = M + 1) so that threads reading one column of consecutive rows
+// hit different banks.
+//   A : M x lda in LDS, M = m rounded up to even; when m is odd row/column m
+//       must be zero.  Destroyed.
+//   Vw: M x lda LDS scratch.  cs: M doubles.  sched: (M-1) * M ints.
+//   w : m eigenvalues ascending;  Y: m x m row-major, column j = eigenvector
+//       of w[j] (stable order for equal eigenvalues).
+// Threshold and stopping rule as dense_eigh (|a_pq| <= 1e-16 |A|_F is set to
+// zero; stop after a sweep without a rotation).  Called by all threads of the
+// block (blockDim a multiple of 64, at least M/2); returns the sweeps.
+__device__ int block_eigh(double* A, int lda, int m, double* w, double* Y, double* Vw, double* cs,
+                          int* sched, double* red) {
   const int t = threadIdx.x, nt = blockDim.x;
-  auto search = [&](int k) {  // wave 0 only
-    double best = -1.0;
-    int bi = k;
-    for (int i = k + t; i < d; i += 64) {
-      const double v = fabs(A[i * d + k]);
-      if (v > best) { best = v; bi = i; }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-    }
-    if (t == 0) ired[0] = bi;
-  };
-  if (t < 64) search(0);
-  __syncthreads();
-  for (int k = 0; k < d; ++k) {
-    const int piv = ired[0];
-    const double* Ap = A + piv * d;
-    const double akk = Ap[k];
-    const double bp = b[piv];
-    // eliminate column k from every remaining row except the pivot row
-    // rows k..d-1 over groups of 32 threads; columns k+1..d (column d = rhs)
-    for (int i = k + (t >> 5); i < d; i += (nt >> 5)) {
-      if (i == piv) continue;
-      const double l = A[i * d + k] / akk;
-      for (int j = k + 1 + (t & 31); j <= d; j += 32) {
-        if (j < d) A[i * d + j] -= l * Ap[j];
-        else b[i] -= l * bp;
-      }
-    }
+  const int M = m + (m & 1), half = M / 2;
+  if (m == 1) {
+    if (t == 0) { w[0] = A[0]; Y[0] = 1.0; }
     __syncthreads();
-    if (t < 64) {
-      // zero the eliminated column entries and swap rows k <-> piv
-      for (int i = k + t; i < d; i += 64)
-        if (i != piv) A[i * d + k] = 0.0;
-      if (piv != k) {
-        for (int j = k + t; j < d; j += 64) {
-          const double a = A[k * d + j]; A[k * d + j] = A[piv * d + j]; A[piv * d + j] = a;
+    return 0;
+  }
+  for (int idx = t; idx < (M - 1) * half; idx += nt) {
+    const int r = idx / half, kp = idx - (idx / half) * half;
+    const int pa = (kp == 0) ? 0 : 1 + ((kp - 1 + r) % (M - 1));
+    const int pb = 1 + ((M - 2 - kp + r) % (M - 1));
+    sched[2 * idx] = pa < pb ? pa : pb;
+    sched[2 * idx + 1] = pa < pb ? pb : pa;
+  }
+  // thread decomposition: column pair bl, row pairs al0, al0 + astep, ...
+  const int astep = nt / half;
+  const int bl = t % half, al0 = t / half;
+  const bool act = t < astep * half;
+  double fro = 0.0;
+  if (act)
+    for (int i = al0; i < M; i += astep)
+      for (int j = bl; j < M; j += half) {
+        Vw[i * lda + j] = (i == j) ? 1.0 : 0.0;
+        fro += A[i * lda + j] * A[i * lda + j];
+      }
+  fro = block_sum1(fro, red);
+  const double thr = 1e-16 * sqrt(fro);
+  __syncthreads();
+  int sweep = 0;
+  for (; sweep < 40; ++sweep) {
+    int any = 0;
+    for (int r = 0; r < M - 1; ++r) {
+      const int* pr = sched + 2 * r * half;
+      int rot = 0;
+      if (t < half) {
+        const int p = pr[2 * t], q = pr[2 * t + 1];
+        double c = 1.0, s = 0.0;
+        if (q < m) {
+          const double apq = A[p * lda + q];
+          if (fabs(apq) > thr) {
+            const double app = A[p * lda + p], aqq = A[q * lda + q];
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(tt * tt + 1.0);
+            s = tt * c;
+            rot = 1;
+          }
         }
-        if (t == 0) { const double x = b[k]; b[k] = b[piv]; b[piv] = x; }
+        cs[2 * t] = c;
+        cs[2 * t + 1] = s;
       }
-      if (k + 1 < d) search(k + 1);
+      any |= __syncthreads_or(rot);
+      if (act) {
+        const int pb = pr[2 * bl], qb = pr[2 * bl + 1];
+        const double cb = cs[2 * bl], sb = cs[2 * bl + 1];
+        for (int a = al0; a < half; a += astep) {
+          const int pa = pr[2 * a], qa = pr[2 * a + 1];
+          const double ca = cs[2 * a], sa = cs[2 * a + 1];
+          double* rp = A + pa * lda;
+          double* rq = A + qa * lda;
+          const double x00 = rp[pb], x01 = rp[qb], x10 = rq[pb], x11 = rq[qb];
+          const double y00 = ca * x00 - sa * x10, y01 = ca * x01 - sa * x11;
+          const double y10 = sa * x00 + ca * x10, y11 = sa * x01 + ca * x11;
+          double z00 = cb * y00 - sb * y01, z01 = sb * y00 + cb * y01;
+          double z10 = cb * y10 - sb * y11, z11 = sb * y10 + cb * y11;
+          if (a == bl && qa < m) { z01 = 0.0; z10 = 0.0; }
+          rp[pb] = z00; rp[qb] = z01; rq[pb] = z10; rq[qb] = z11;
+        }
+        if (qb < m)
+          for (int i = al0; i < m; i += astep) {
+            double* vi = Vw + i * lda;
+            const double vp = vi[pb], vq = vi[qb];
+            vi[pb] = cb * vp - sb * vq;
+            vi[qb] = cb * vq + sb * vp;
+          }
+      }
+      __syncthreads();
     }
-    __syncthreads();
+    if (!any) { ++sweep; break; }
   }
-  // back substitution by wave 0
-  if (t < 64) {
-    for (int k = d - 1; k >= 0; --k) {
-      const double xk = b[k] / A[k * d + k];
-      for (int i = t; i < k; i += 64) b[i] -= A[i * d + k] * xk;
-      if (t == 0) b[k] = xk;
+  // ascending order, ties by index; column permutation of V into Y
+  for (int i = t; i < m; i += nt) {
+    const double di = A[i * lda + i];
+    int rank = 0;
+    for (int j = 0; j < m; ++j) {
+      const double dj = A[j * lda + j];
+      rank += (dj < di) || (dj == di && j < i);
     }
+    w[rank] = di;
+    for (int x = 0; x < m; ++x) Y[x * m + rank] = Vw[x * lda + i];
   }
   __syncthreads();
+  return sweep;
 }
 
 // ---------------------------------------------------------------------------
-// In-place LDL^T of a symmetric positive definite d x d matrix (lower
-// triangle used, no pivoting): one block barrier per column.  On exit
-// A[i][k] (i > k) holds the unscaled column (L[i][k] * D[k]) and A[k][k] = D[k].
-__device__ void dense_ldlt(double* A, int d) {
+// LDL^T of a symmetric d x d matrix without pivoting (the IPM's SPD Newton
+// block), in place: on return the strict lower triangle holds the unit factor
+// L and rD[k] = 1 / D[k].  Right-looking: step k updates the trailing lower
+// triangle with column k (16 threads per row).  A zero pivot yields inf/NaN,
+// as an LU solve of a singular matrix does.
+__device__ void dense_ldlt(double* A, int d, double* rD) {
   const int t = threadIdx.x, nt = blockDim.x;
   const int tr = t >> 4, nr = nt >> 4, tc = t & 15;
   for (int k = 0; k < d - 1; ++k) {
-    const double dk = A[k * d + k];
-    // trailing update A[i][j] -= A[i][k] A[j][k] / dk for k < j <= i
+    const double rdk = 1.0 / A[k * d + k];
+    // trailing update A[i][j] -= (A[i][k] / D_k) A[j][k] for k < j <= i
     for (int i = k + 1 + tr; i < d; i += nr) {
-      const double lik = A[i * d + k] / dk;
+      const double lik = A[i * d + k] * rdk;
       for (int j = k + 1 + tc; j <= i; j += 16) A[i * d + j] -= lik * A[j * d + k];
     }
     __syncthreads();
   }
+  for (int k = t; k < d; k += nt) rD[k] = 1.0 / A[k * d + k];
+  __syncthreads();
+  for (int i = tr; i < d; i += nr)
+    for (int j = tc; j < i; j += 16) A[i * d + j] *= rD[j];
+  __syncthreads();
 }
 
-// Solve (L D L^T) X = B for nrhs right-hand sides with the dense_ldlt factor
-// (B row-major d x nrhs, overwritten).  One wavefront per right-hand side.
-__device__ void dense_ldlt_solve(const double* A, int d, double* B, int nrhs) {
+// Broadcast lane `src` (uniform) of a double to the whole wavefront.
+__device__ __forceinline__ double readlane_d(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Solve (L D L^T) X = B with the dense_ldlt factor for nrhs right-hand sides
+// (B row-major d x ldb, columns 0..nrhs-1 overwritten), d <= 128.  One
+// wavefront per right-hand side holds it in registers (rows lane and
+// lane + 64); forward and backward substitution are column-oriented, so each
+// step is one broadcast and one FMA per row.
+__device__ void dense_ldlt_solve(const double* A, int d, const double* rD, double* B, int ldb,
+                                 int nrhs) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (w < nrhs) {
-    // forward: y_i = b_i - sum_{k<i} (A[i][k] / D[k]) y_k  (column-oriented)
+    const int i0 = lane, i1 = lane + 64;
+    double y0 = (i0 < d) ? B[i0 * ldb + w] : 0.0;
+    double y1 = (i1 < d) ? B[i1 * ldb + w] : 0.0;
     for (int k = 0; k < d; ++k) {
-      const double yk = B[k * nrhs + w];
-      const double dk = A[k * d + k];
-      for (int i = k + 1 + lane; i < d; i += 64) B[i * nrhs + w] -= (A[i * d + k] / dk) * yk;
+      const double yk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
+      if (i0 > k && i0 < d) y0 -= A[i0 * d + k] * yk;
+      if (i1 > k && i1 < d) y1 -= A[i1 * d + k] * yk;
     }
-    // diagonal and backward: x_k = y_k / D[k] - sum_{i>k} (A[i][k] / D[k]) x_i
-    for (int k = d - 1; k >= 0; --k) {
-      const double dk = A[k * d + k];
-      double s = 0.0;
-      for (int i = k + 1 + lane; i < d; i += 64) s += (A[i * d + k] / dk) * B[i * nrhs + w];
-      s = wave_sum(s);
-      if (lane == 0) B[k * nrhs + w] = B[k * nrhs + w] / dk - s;
+    if (i0 < d) y0 *= rD[i0];
+    if (i1 < d) y1 *= rD[i1];
+    for (int k = d - 1; k > 0; --k) {
+      const double xk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
+      if (i0 < k) y0 -= A[k * d + i0] * xk;
+      if (i1 < k) y1 -= A[k * d + i1] * xk;
     }
+    if (i0 < d) B[i0 * ldb + w] = y0;
+    if (i1 < d) B[i1 * ldb + w] = y1;
   }
   __syncthreads();
 }

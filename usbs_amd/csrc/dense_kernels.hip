@@ -41,7 +41,8 @@ __device__ __forceinline__ double symkron(const double* T, const double* Si, int
 // Per-lane K x K Cholesky in registers (K is a compile-time constant so the
 // packed factor stays in VGPRs).  Returns false on a non-positive pivot.
 template <int K>
-__device__ __forceinline__ bool lane_chol(const double* M, const double* dM, double s, double* L) {
+__device__ __forceinline__ bool lane_chol(const double* M, const double* dM, double s, double* L,
+                                          double* rdiag) {
 #pragma unroll
   for (int c = 0; c < K; ++c) {
     double d = M[c * K + c] + s * dM[c * K + c];
@@ -49,13 +50,15 @@ __device__ __forceinline__ bool lane_chol(const double* M, const double* dM, dou
     for (int p = 0; p < c; ++p) d -= L[c * (c + 1) / 2 + p] * L[c * (c + 1) / 2 + p];
     if (!(d > 0.0)) return false;
     d = sqrt(d);
+    const double rd = 1.0 / d;
     L[c * (c + 1) / 2 + c] = d;
+    rdiag[c] = rd;
 #pragma unroll
     for (int i = c + 1; i < K; ++i) {
       double v = M[i * K + c] + s * dM[i * K + c];
 #pragma unroll
       for (int p = 0; p < c; ++p) v -= L[i * (i + 1) / 2 + p] * L[c * (c + 1) / 2 + p];
-      L[i * (i + 1) / 2 + c] = v / d;
+      L[i * (i + 1) / 2 + c] = v * rd;
     }
   }
   return true;
@@ -66,22 +69,22 @@ template <int K>
 __device__ void lane_spd_inverse(const double* S, double* Si) {
   const int j = threadIdx.x;
   if (j < K) {
-    double L[K * (K + 1) / 2];
-    lane_chol<K>(S, S, 0.0, L);
+    double L[K * (K + 1) / 2], rd[K];
+    lane_chol<K>(S, S, 0.0, L, rd);
     double y[K], x[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       double v = (i == j) ? 1.0 : 0.0;
 #pragma unroll
       for (int p = 0; p < i; ++p) v -= L[i * (i + 1) / 2 + p] * y[p];
-      y[i] = v / L[i * (i + 1) / 2 + i];
+      y[i] = v * rd[i];
     }
 #pragma unroll
     for (int i = K - 1; i >= 0; --i) {
       double v = y[i];
 #pragma unroll
       for (int p = i + 1; p < K; ++p) v -= L[p * (p + 1) / 2 + i] * x[p];
-      x[i] = v / L[i * (i + 1) / 2 + i];
+      x[i] = v * rd[i];
     }
 #pragma unroll
     for (int i = 0; i < K; ++i) Si[i * K + j] = x[i];
@@ -91,16 +94,18 @@ __device__ void lane_spd_inverse(const double* S, double* Si) {
 
 // Largest 0.9^j (j < max_iters) with M + 0.9^j dM positive definite, or 0:
 // the vmapped eigvalsh test of solver/usbs.py:135-144 with a Cholesky test
-// per candidate (the 64 lanes test 64 candidates at a time).
+// per candidate (each thread tests one candidate of a batch of blockDim).
+// pw0 = 0.9^threadIdx.x, the first batch's candidate of this thread.
 template <int K>
-__device__ double pd_step(const double* M, const double* dM, int max_iters, double* red) {
+__device__ double pd_step(const double* M, const double* dM, int max_iters, double pw0,
+                          double* red) {
   const int t = threadIdx.x;
   for (int j0 = 0; j0 < max_iters; j0 += blockDim.x) {
     double v = 0.0;
     if (j0 + t < max_iters) {
-      const double s = pow(0.9, (double)(j0 + t));
-      double L[K * (K + 1) / 2];
-      if (lane_chol<K>(M, dM, s, L)) v = s;
+      const double s = (j0 == 0) ? pw0 : pow(0.9, (double)(j0 + t));
+      double L[K * (K + 1) / 2], rd[K];
+      if (lane_chol<K>(M, dM, s, L, rd)) v = s;
     }
     v = wave_max(v);
     __syncthreads();
@@ -123,6 +128,8 @@ struct IpmLds {
   double *coeff, *q12, *h1, *sI, *sS, *sT, *F1, *ordv, *dsS, *dsT, *tmpv;
   double *S, *T, *Si, *dS, *dT, *tk, *red;
   double* R;     // d x 3 right-hand sides of the Woodbury solve
+  double* rD;    // d reciprocal pivots of the LDL^T factor
+  double* cb;    // 2 x 128 published columns of the LDL^T
   int* ired;
   int *pb, *pc;  // svec index -> (b, c)
 };
@@ -137,6 +144,8 @@ __device__ IpmLds ipm_layout(double* lds, int k) {
   L.S = p; p += kk; L.T = p; p += kk; L.Si = p; p += kk; L.dS = p; p += kk; L.dT = p; p += kk;
   L.tk = p; p += kk;
   L.R = p; p += 3 * d;
+  L.rD = p; p += d;
+  L.cb = p; p += 256;
   L.red = p; p += 16;
   L.ired = (int*)p; p += 16;
   L.pb = (int*)p; p += (d + 1) / 2 + 1;
@@ -147,7 +156,7 @@ __device__ IpmLds ipm_layout(double* lds, int k) {
 static size_t ipm_lds_bytes(int k) {
   const int d = k * (k + 1) / 2, kk = k * k;
   const int big = d * d > 5 * kk + 4 * k + 64 ? d * d : 5 * kk + 4 * k + 64;
-  return sizeof(double) * (big + 13 * d + 6 * kk + 16 + 16 + d + 4);
+  return sizeof(double) * (big + 14 * d + 256 + 6 * kk + 16 + 16 + d + 4);
 }
 
 __device__ void ipm_tables(const IpmLds& L, int k) {
@@ -182,43 +191,160 @@ __device__ __forceinline__ double symkron(const IpmLds& L, int k, int a, int b) 
 }
 
 
-// Solve (B - U W^T) x = r where B is SPD (factored in place by dense_ldlt),
-// U = [u1, u2], W = [w1, w2] (nu = 1 or 2 columns): the Sherman-Morrison-
-// Woodbury identity x = z0 + Z (I - W^T Z)^{-1} W^T z0 with z0 = B^{-1} r,
-// Z = B^{-1} U.  Same solution as a dense solve of the full matrix, up to
-// rounding.  r in/out; R: d x 3 LDS scratch.
-__device__ void woodbury_solve(double* B, int d, double* r, const double* u1, const double* u2,
-                               const double* w1, const double* w2, int nu, double* R,
-                               double* red) {
+// LDL^T (no pivoting) of the SPD Newton block B (D x D row-major in LDS),
+// held in registers by the four wavefronts of the block: lane l owns rows l
+// and l + 64, wave w owns the columns j = w (mod 4).  Step k: the owner of
+// column k publishes it (double-buffered LDS column, one barrier per step),
+// every lane scales its entry by 1/D_k and updates its columns j > k.  On
+// return the strict lower triangle of B holds the unit factor L (stored once
+// at the end, scaled by the reciprocal pivots) and rD[k] = 1/D_k.  A zero pivot yields inf/NaN, as a dense solve of a singular
+// matrix does.  cb: 256 doubles of LDS.
+template <int D>
+__device__ void ipm_ldlt(double* B, double* rD, double* cb) {
+  constexpr int NC = (D + 3) / 4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r0 = lane, r1 = lane + 64;
+  const int r0c = r0 < D ? r0 : D - 1, r1c = r1 < D ? r1 : D - 1;
+  double a0[NC], a1[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int j = w + 4 * c;
+    a0[c] = (j < D && r0 < D) ? B[r0 * D + j] : 0.0;
+    a1[c] = (D > 64 && j < D && r1 < D) ? B[r1 * D + j] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < D; ++k) {
+    double* col = cb + (k & 1) * 128;
+    if (w == (k & 3)) {
+      // a0[k >> 2] without dynamic register indexing (which would spill)
+      double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c == (k >> 2)) { v0 = a0[c]; v1 = a1[c]; }
+      if (r0 < D) col[r0] = v0;
+      if (D > 64 && r1 < D) col[r1] = v1;
+    }
+    __syncthreads();
+    // every load of the step is issued before the first use, and the
+    // updates are selects rather than branches
+    const double piv = col[k];
+    const double c0 = col[r0c];
+    const double c1 = (D > 64) ? col[r1c] : 0.0;
+    double cv[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int j = w + 4 * c;
+      cv[c] = col[j < D ? j : D - 1];
+    }
+    const double rk = 1.0 / piv;
+    if (threadIdx.x == 0) rD[k] = rk;
+    const double l0 = c0 * rk;
+    const double l1 = c1 * rk;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int j = w + 4 * c;
+      const bool live = (j > k) && (j < D);
+      const double n0 = a0[c] - l0 * cv[c];
+      a0[c] = (live && j <= r0) ? n0 : a0[c];
+      if (D > 64) {
+        const double n1 = a1[c] - l1 * cv[c];
+        a1[c] = (live && j <= r1) ? n1 : a1[c];
+      }
+    }
+  }
+  __syncthreads();
+  // column j of the lower triangle is final after step j - 1: scale and store
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int j = w + 4 * c;
+    if (j < D && j < r0 && r0 < D) B[r0 * D + j] = a0[c] * rD[j];
+    if (D > 64 && j < D && j < r1 && r1 < D) B[r1 * D + j] = a1[c] * rD[j];
+  }
+  __syncthreads();
+}
+
+// Solve (L D L^T) X = R for nrhs <= 4 right-hand sides (R row-major D x 3)
+// with the ipm_ldlt factor.  One wavefront per right-hand side keeps it in
+// registers (rows lane, lane + 64); both substitutions are column-oriented:
+// one broadcast (readlane) and one FMA per row and step.
+template <int D>
+__device__ void ipm_ldlt_solve(const double* Lm, const double* rD, double* R, int nrhs) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w < nrhs) {
+    const int i0 = lane, i1 = lane + 64;
+    const int i0c = i0 < D ? i0 : D - 1, i1c = i1 < D ? i1 : D - 1;
+    double y0 = (i0 < D) ? R[i0 * 3 + w] : 0.0;
+    double y1 = (D > 64 && i1 < D) ? R[i1 * 3 + w] : 0.0;
+#pragma unroll
+    for (int k = 0; k < D - 1; ++k) {
+      const double yk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
+      const double la = (i0 > k && i0 < D) ? Lm[i0c * D + k] : 0.0;
+      y0 -= la * yk;
+      if (D > 64) {
+        const double lb = (i1 > k && i1 < D) ? Lm[i1c * D + k] : 0.0;
+        y1 -= lb * yk;
+      }
+    }
+    y0 *= rD[i0c];
+    if (D > 64) y1 *= rD[i1c];
+#pragma unroll
+    for (int k = D - 1; k > 0; --k) {
+      const double xk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
+      const double la = (i0 < k) ? Lm[k * D + i0] : 0.0;
+      y0 -= la * xk;
+      if (D > 64) {
+        const double lb = (i1 < k) ? Lm[k * D + (i1 < D ? i1 : 0)] : 0.0;
+        y1 -= lb * xk;
+      }
+    }
+    if (i0 < D) R[i0 * 3 + w] = y0;
+    if (D > 64 && i1 < D) R[i1 * 3 + w] = y1;
+  }
+  __syncthreads();
+}
+
+// Solve (B - U W^T) x = r where B (D x D) is SPD: U = [u1, u2], W = [w1, w2]
+// (nu = 1 or 2 columns), by the Sherman-Morrison-Woodbury identity
+// x = z0 + Z (I - W^T Z)^{-1} W^T z0 with z0 = B^{-1} r, Z = B^{-1} U.  Same
+// solution as a dense solve of the full matrix, up to rounding.  r in/out;
+// R: D x 3 LDS scratch; B is destroyed.
+template <int D>
+__device__ void woodbury_solve(double* B, double* r, const double* u1, const double* u2,
+                               const double* w1, const double* w2, int nu, double* R, double* rD,
+                               double* cb, double* red) {
   const int t = threadIdx.x;
-  for (int a = t; a < d; a += blockDim.x) {
+  for (int a = t; a < D; a += blockDim.x) {
     R[a * 3 + 0] = r[a];
     R[a * 3 + 1] = u1[a];
     R[a * 3 + 2] = (nu > 1) ? u2[a] : 0.0;
   }
+  dense_ldlt(B, D, rD);
+  ipm_ldlt_solve<D>(B, rD, R, nu + 1);
+  if (t < 64) {
+    double g0 = 0.0, g1 = 0.0, m00 = 0.0, m01 = 0.0, m10 = 0.0, m11 = 0.0;
+    for (int a = t; a < D; a += 64) {
+      const double z0 = R[a * 3], z1 = R[a * 3 + 1], z2 = R[a * 3 + 2];
+      const double wa = w1[a], wb = (nu > 1) ? w2[a] : 0.0;
+      g0 += wa * z0; g1 += wb * z0;
+      m00 += wa * z1; m01 += wa * z2; m10 += wb * z1; m11 += wb * z2;
+    }
+    g0 = wave_sum(g0); g1 = wave_sum(g1);
+    m00 = wave_sum(m00); m01 = wave_sum(m01);
+    m10 = wave_sum(m10); m11 = wave_sum(m11);
+    double c0, c1 = 0.0;
+    if (nu > 1) {
+      const double a00 = 1.0 - m00, a01 = -m01, a10 = -m10, a11 = 1.0 - m11;
+      const double det = a00 * a11 - a01 * a10;
+      c0 = (a11 * g0 - a01 * g1) / det;
+      c1 = (a00 * g1 - a10 * g0) / det;
+    } else {
+      c0 = g0 / (1.0 - m00);
+    }
+    if (t == 0) { red[0] = c0; red[1] = c1; }
+  }
   __syncthreads();
-  dense_ldlt(B, d);
-  dense_ldlt_solve(B, d, R, 3);
-  double g0 = 0.0, g1 = 0.0, m00 = 0.0, m01 = 0.0, m10 = 0.0, m11 = 0.0;
-  for (int a = t; a < d; a += blockDim.x) {
-    const double z0 = R[a * 3], z1 = R[a * 3 + 1], z2 = R[a * 3 + 2];
-    const double wa = w1[a], wb = (nu > 1) ? w2[a] : 0.0;
-    g0 += wa * z0; g1 += wb * z0;
-    m00 += wa * z1; m01 += wa * z2; m10 += wb * z1; m11 += wb * z2;
-  }
-  g0 = block_sum1(g0, red); g1 = block_sum1(g1, red);
-  m00 = block_sum1(m00, red); m01 = block_sum1(m01, red);
-  m10 = block_sum1(m10, red); m11 = block_sum1(m11, red);
-  double c0, c1 = 0.0;
-  if (nu > 1) {
-    const double a00 = 1.0 - m00, a01 = -m01, a10 = -m10, a11 = 1.0 - m11;
-    const double det = a00 * a11 - a01 * a10;
-    c0 = (a11 * g0 - a01 * g1) / det;
-    c1 = (a00 * g1 - a10 * g0) / det;
-  } else {
-    c0 = g0 / (1.0 - m00);
-  }
-  for (int a = t; a < d; a += blockDim.x) r[a] = R[a * 3] + R[a * 3 + 1] * c0 + R[a * 3 + 2] * c1;
+  const double c0 = red[0], c1 = red[1];
+  for (int a = t; a < D; a += blockDim.x) r[a] = R[a * 3] + R[a * 3 + 1] * c0 + R[a * 3 + 2] * c1;
   __syncthreads();
 }
 
@@ -252,7 +378,7 @@ __device__ void ipm_center(const IpmLds& L, int k, double& zeta, double& omega) 
 
 // Shared step: sizes, line search, update; returns the new mu and step.
 template <int K>
-__device__ void ipm_step(const IpmLds& L, int max_iters, double& eta, double& zeta,
+__device__ void ipm_step(const IpmLds& L, int max_iters, double pw0, double& eta, double& zeta,
                          double& omega, double& mu, double& step, double d_eta, double d_zeta,
                          double d_omega) {
   constexpr int D = K * (K + 1) / 2;
@@ -266,8 +392,8 @@ __device__ void ipm_step(const IpmLds& L, int max_iters, double& eta, double& ze
   double st = -1.0 / rmin;
   st = st > 1.0 ? 1.0 : st;
   st = 0.99 * ((st <= 0.0) ? 1.0 : st);
-  const double cS = pd_step<K>(L.S, L.dS, max_iters, L.red);
-  const double cT = pd_step<K>(L.T, L.dT, max_iters, L.red);
+  const double cS = pd_step<K>(L.S, L.dS, max_iters, pw0, L.red);
+  const double cT = pd_step<K>(L.T, L.dT, max_iters, pw0, L.red);
   const double cand = cS < cT ? cS : cT;
   st = st < cand ? st : cand;
   __syncthreads();
@@ -285,11 +411,20 @@ __device__ void ipm_step(const IpmLds& L, int max_iters, double& eta, double& ze
   step = st;
 }
 
+// out = (cq Gq) v for the d x d matrix Gq in global memory: 4 threads per
+// row, each summing every 4th column, combined in a fixed order.
 __device__ void qmatvec(const double* Gq, double cq, const double* v, int d, double* out) {
-  for (int a = threadIdx.x; a < d; a += blockDim.x) {
+  const int t = threadIdx.x, q = t & 3;
+  for (int a0 = 0; a0 < d; a0 += IPM_NT / 4) {
+    const int a = a0 + (t >> 2);
     double s = 0.0;
-    for (int b = 0; b < d; ++b) s += (cq * Gq[a * d + b]) * v[b];
-    out[a] = s;
+    if (a < d) {
+      const double* g = Gq + (size_t)a * d;
+      for (int b = q; b < d; b += 4) s += (cq * g[b]) * v[b];
+    }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    if (a < d && q == 0) out[a] = s;
   }
   __syncthreads();
 }
@@ -302,6 +437,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
   const int t = threadIdx.x;
   IpmLds L = ipm_layout(lds, k);
   ipm_tables(L, k);
+  const double pw0 = pow(0.9, (double)t);
   double* sc = A.sc;
   const double tu = A.tu, rho = A.rho;
   const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
@@ -377,7 +513,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
       L.ordv[a] = (L.q12[a] - k2 * L.sI[a]) / den;
     }
     __syncthreads();
-    woodbury_solve(L.coeff, d, L.dsS, L.q12, L.sI, L.dsT, L.ordv, 2, L.R, L.red);
+    woodbury_solve<D>(L.coeff, L.dsS, L.q12, L.sI, L.dsT, L.ordv, 2, L.R, L.rD, L.cb, L.red);
     // back-substitution for the other directions (usbs.py:111-122)
     for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = k1 * L.q12[a] + L.sI[a];
     __syncthreads();
@@ -392,7 +528,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
       L.dsT[a] = L.dsT[a] + d_eta * L.q12[a] + d_omega * L.sI[a] + L.F1[a];
     __syncthreads();
     const double d_zeta = dotd(L.q12, L.dsS, d, L.red) + d_eta * q22 + d_omega + F2;
-    ipm_step<K>(L, A.max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ipm_step<K>(L, A.max_iters, pw0, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
     ++it;
   }
   // outputs (usbs.py:191-192) and the bundle factor (usbs.py:568-570)
@@ -439,6 +575,7 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
   const int t = threadIdx.x;
   IpmLds L = ipm_layout(lds, k);
   ipm_tables(L, k);
+  const double pw0 = pow(0.9, (double)t);
   const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
   const double pb = sc[SC_PB];
   double* g1 = L.h1;
@@ -478,7 +615,7 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
       L.dsS[a] = v;
     }
     __syncthreads();
-    woodbury_solve(L.coeff, d, L.dsS, L.sI, nullptr, L.ordv, nullptr, 1, L.R, L.red);
+    woodbury_solve<D>(L.coeff, L.dsS, L.sI, L.sI, L.ordv, L.ordv, 1, L.R, L.rD, L.cb, L.red);
     const double p1 = dotd(L.sI, L.dsS, d, L.red);
     const double d_eta = (p1 - k1 * (mu / eta - g2 - omega) + mu / omega + trS + eta - 1) / (-k1 * ze - 1);
     double d_omega = -ze * d_eta + mu / eta;
@@ -487,7 +624,7 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
       L.dsT[a] = d_omega * L.sI[a] - L.sT[a] + g1[a] + omega * L.sI[a];
     __syncthreads();
     const double d_zeta = d_omega - zeta + g2 + omega;
-    ipm_step<K>(L, max_iters, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ipm_step<K>(L, max_iters, pw0, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
     ++it;
   }
   dense_svec(L.S, k, L.sS);

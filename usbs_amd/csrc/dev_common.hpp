@@ -89,6 +89,8 @@ __device__ __forceinline__ double block_sum1(double v, double* lds /* >= NT/64 d
 // into out (LDS).  P = blockDim / c threads share a column: thread (j, q)
 // sums tiles q, q + P, ... in order, then thread j adds the P sums in order.
 // The order depends only on (c, ntiles, blockDim), so it is deterministic.
+// Each thread issues up to 16 loads before the first add, so the partials of
+// a typical reduction arrive in one memory round trip.
 // scratch: LDS of blockDim doubles (must not alias out).
 __device__ __forceinline__ void sum_tile_partials(const double* __restrict__ part, int ntiles,
                                                   int stride, int c, double* out,
@@ -101,15 +103,17 @@ __device__ __forceinline__ void sum_tile_partials(const double* __restrict__ par
   if (t < c * P) {
     const int j = t / P, q = t - (t / P) * P;
     double s = 0.0;
-    int tl = q;
-    for (; tl + 3 * P < ntiles; tl += 4 * P) {
-      const double a0 = part[(size_t)tl * stride + j];
-      const double a1 = part[(size_t)(tl + P) * stride + j];
-      const double a2 = part[(size_t)(tl + 2 * P) * stride + j];
-      const double a3 = part[(size_t)(tl + 3 * P) * stride + j];
-      s += a0; s += a1; s += a2; s += a3;
+    for (int tl0 = q; tl0 < ntiles; tl0 += 16 * P) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int tl = tl0 + u * P;
+        v[u] = (tl < ntiles) ? part[(size_t)tl * stride + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (tl0 + u * P < ntiles) s += v[u];
     }
-    for (; tl < ntiles; tl += P) s += part[(size_t)tl * stride + j];
     scratch[t] = s;
   }
   __syncthreads();

@@ -132,6 +132,10 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
   P.nA = nA;
   P.tile = tile_rows_for((int)n);
   P.ntiles = (int)((n + P.tile - 1) / P.tile);
+  for (int t = 0; t < P.ntiles; ++t) {
+    const int64_t a = (int64_t)t * P.tile, b = std::min<int64_t>(n, a + P.tile);
+    P.tile_nnz_max = std::max(P.tile_nnz_max, row_ptr[b] - row_ptr[a]);
+  }
   P.mtile = tile_rows_for((int)m);
   P.mntiles = (int)((m + P.mtile - 1) / P.mtile);
   P.diag_A = diag;
@@ -177,7 +181,8 @@ int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int
   L.val = opval;
   L.ldq = ((long long)P.n + 63) / 64 * 64;
   if (!(L.Q = (T*)ctx_alloc(c, pool, sizeof(T) * L.ldq * (m + 1)))) return USBS_ERR_OOM;
-  if (!(L.w = (T*)ctx_alloc(c, pool, sizeof(T) * L.ldq))) return USBS_ERR_OOM;
+  if (!(L.w = (T*)ctx_alloc(c, pool, sizeof(T) * 2 * L.ldq))) return USBS_ERR_OOM;
+  L.tile_nnz_max = P.tile_nnz_max;
   const size_t ps = sizeof(double) * (size_t)L.ntiles * L.pstride;
   if (!(L.part1 = (double*)ctx_alloc(c, pool, ps))) return USBS_ERR_OOM;
   if (!(L.part2 = (double*)ctx_alloc(c, pool, ps))) return USBS_ERR_OOM;
@@ -256,9 +261,10 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
     rc = lanczos_perop<T>(L, c->stream, host_ctrl);
     if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos per-op: ") + hipGetErrorString((hipError_t)rc));
   } else {
-    int nb = std::min(std::min(L.ntiles, c->persist_wgs > 0 ? c->persist_wgs : L.ntiles),
-                      std::max(1, c->coresident));
-    L.nblocks = nb;
+    size_t cache_bytes = 0;
+    rc = lanczos_plan_persistent<T>(L, c->device, c->persist_wgs, &cache_bytes);
+    if (rc) return set_err(c, USBS_ERR_HIP, "no persistent Lanczos launch fits the device");
+    const int nb = L.nblocks;
     LzInst<T>* d_inst = (LzInst<T>*)ctx_alloc(c, pool, sizeof(LzInst<T>));
     int* d_map = (int*)ctx_alloc(c, pool, sizeof(int2) * (nb + L.ntiles));
     if (!d_inst || !d_map) return USBS_ERR_OOM;
@@ -268,7 +274,7 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
     USBS_HIP(c, hipMemcpyAsync(d_inst, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
     USBS_HIP(c, hipMemcpyAsync(d_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, c->stream));
     rc = lanczos_persistent<T>(d_inst, 1, nb, m, max_restarts, c->stream, d_map, d_map + 2 * nb,
-                               L.ntiles, 1ull << 26);
+                               L.ntiles, 1ull << 26, cache_bytes);
     if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos persistent: ") + hipGetErrorString((hipError_t)rc));
     USBS_HIP(c, hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, c->stream));
     USBS_HIP(c, hipStreamSynchronize(c->stream));
@@ -313,7 +319,6 @@ int usbs_create(usbs_ctx** out, int device) {
     delete h;
     return USBS_ERR_HIP;
   }
-  h->c.coresident = lanczos_max_coresident_blocks(32, device);
   if (const char* e = getenv("USBS_PERSIST_WGS")) h->c.persist_wgs = atoi(e);
   *out = h;
   return USBS_OK;

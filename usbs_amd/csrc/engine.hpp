@@ -25,6 +25,7 @@ struct Problem {
   int n = 0, m = 0;
   long long nnz = 0, nA = 0;
   int tile = 0, ntiles = 0;     // row tiles of n
+  int tile_nnz_max = 0;         // most operator entries in one row tile
   int mtile = 0, mntiles = 0;   // tiles over the m constraints
   bool diag_A = false;          // A = {(i, i, i) with data 1}: max-cut's diag(X) = 1
   bool has_ineq = false;        // any b_ineq_mask != 0
@@ -54,7 +55,6 @@ struct Ctx {
   Problem P;
   std::vector<DevBuf> prob_bufs;  // freed on new problem / destroy
   std::vector<DevBuf> work_bufs;  // solve workspaces
-  int coresident = 0;             // max co-resident Lanczos workgroups on this device
   int persist_wgs = 0;            // workgroups per persistent Lanczos instance (0 = one per tile)
   bool timing = false;            // bracket persistent Lanczos launches with events
   usbs_stats stats{};

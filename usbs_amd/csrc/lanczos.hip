@@ -3,18 +3,28 @@
 //
 //  * per-op: one kernel per phase of a Lanczos step, the host reads the
 //    control words once per restart (lanczos_perop);
-//  * persistent: one cooperative launch runs every restart of one or many
-//    instances, with device-wide barriers between phases and the Ritz solve
-//    done by the instance's first workgroup (lanczos_persistent).
+//  * persistent: per restart, one cooperative launch runs every Lanczos step
+//    of one or many instances with device-wide barriers between phases, then
+//    the Ritz and rotation kernels run (lanczos_persistent).
 //
-// Per Lanczos step i (columns 0..i of Q are valid, the reference's Q_valid
-// mask at lanczos.py:102) the tile functions perform:
-//   A  w = Op q_i, part1 = Q^T w                 (SpMV fused with CGS pass 1 dot)
-//   B  h1 = sum part1; w -= Q h1; part2 = Q^T w  (CGS update 1 fused with pass 2 dot)
-//   C  h2 = sum part2; w -= Q h2; part3 = |w|^2  (CGS update 2 fused with the norm)
-//   D  beta = sqrt(sum part3); q_{i+1} = w/beta; alpha_i = h1_i + h2_i
-// with a counter-based random restart vector when beta < 1e-15
-// (lanczos.py:107-117).  Q is read three times per step.
+// Step i (columns 0..i of Q valid, the reference's Q_valid mask at
+// lanczos.py:102) is three phases over row tiles:
+//   A  [i > first step: beta = |w_{i-1}|, q_i = w_{i-1} / beta]
+//      y = Op q_i, part1 = Q^T y              (SpMV fused with CGS pass 1 dot)
+//   B  h1 = sum part1; y -= Q h1; part2 = Q^T y
+//   C  h2 = sum part2; y -= Q h2; part3 = |y|^2; w_i = y; alpha_i = h1_i + h2_i
+// Normalising the previous residual is fused into the next step's SpMV: the
+// operator gathers (T)(w_{i-1}[c] / beta), the same value the basis column
+// holds, so a step needs three device-wide reductions.  A breakdown
+// (beta < 1e-15, lanczos.py:107-117) is handled before phase A by
+// orthogonalising a counter-based random vector in place of w_{i-1}.  After
+// the last step the residual is normalised into Q[:, m].
+//
+// Reductions over rows: per tile, column-wise sums in a fixed thread order,
+// then tile partials summed in a fixed order (sum_tile_partials), so both
+// forms give bitwise the same numbers.  The persistent form keeps its tile's
+// rows of Q, the work vector and the CSR slice in LDS for the whole restart
+// when they fit.
 #include "lanczos.hpp"
 #include "dense.hpp"
 
@@ -26,103 +36,84 @@ int tile_rows_for(int n) {
   return t;
 }
 
-static inline int lds_doubles_for(int m, int maxc) {
-  const int red = USBS_RCH * USBS_NT + USBS_RCH * USBS_NPART + maxc + 32;
-  const int ritz = 4 * m * m + 4 * m + 64;
-  return red > ritz ? red : ritz;
+#define LZ_RED (USBS_NT + 64)  // doubles of reduction scratch in every tile kernel
+
+static inline size_t ritz_lds_bytes(int m) {
+  const int M = m + (m & 1);
+  return sizeof(double) * (size_t)(2 * M * (M + 1) + m * m + 2 * M + 16 + (8 + M + (M - 1) * M) / 2 + 16);
 }
+static inline size_t tile_lds_bytes(int maxc) { return sizeof(double) * (size_t)(LZ_RED + maxc + 8); }
+static inline size_t rot_lds_bytes(int m) { return sizeof(double) * (size_t)(m * m + 8); }
 
 // ----------------------------------------------------------------------------
-// tile phases
+// tile primitives.  Every routine is called by all USBS_NT threads of a block
+// and ends synchronised.  (rl, j) of a tile's Q block is qb[j * qld + rl].
 
-// mode 0: w = Op q_i (SpMV); mode 1: w = counter normals (breakdown restart);
-// mode 2: w = Q[:, i+1] (orthonormalise an existing column against 0..i).
-// Then part1[tile] = Q[:, 0..i]^T w.
-template <typename T, int MAXC>
-__device__ void tile_phaseA(const LzInst<T>& L, int tl, int i, int mode, double* lds) {
-  const int r0 = tl * L.tile;
-  const int r1 = min(L.n, r0 + L.tile);
-  const T* __restrict__ x = L.Q + (long long)i * L.ldq;
-  double acc[MAXC];
-#pragma unroll
-  for (int j = 0; j < MAXC; ++j) acc[j] = 0.0;
-  for (int r = r0 + (int)threadIdx.x; r < r1; r += USBS_NT) {
+// out[j] = sum_rl Q(rl, j) y(rl) for j < c: thread t sums rows g, g+G, ... of
+// column t / G (g = t % G, G = NT / c), then the G sums of a column are added
+// in g order.
+template <typename T>
+__device__ void col_dots(const T* qb, long long qld, const T* yb, int R, int c, double* red,
+                         double* out) {
+  const int t = threadIdx.x;
+  int G = USBS_NT / c;
+  if (G < 1) G = 1;
+  const int j = t / G, g = t - (t / G) * G;
+  if (j < c) {
+    const T* qj = qb + (long long)j * qld;
     double s = 0.0;
-    if (mode == 0) {
-      const int p0 = L.row_ptr[r], p1 = L.row_ptr[r + 1];
-      for (int p = p0; p < p1; ++p) s += (double)L.val[p] * (double)x[L.col[p]];
-    } else if (mode == 1) {
-      s = counter_normal((uint64_t)(i + 1), (uint64_t)r);
-    } else {
-      s = (double)L.Q[r + (long long)(i + 1) * L.ldq];
-    }
-    const T wv = (T)s;
-    L.w[r] = wv;
-    const double wd = (double)wv;
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j)
-      if (j <= i) acc[j] += (double)L.Q[r + (long long)j * L.ldq] * wd;
+    for (int rl = g; rl < R; rl += G) s += (double)qj[rl] * (double)yb[rl];
+    red[t] = s;
   }
-  double* red = lds;
-  double* red2 = lds + USBS_RCH * USBS_NT;
-  block_reduce_cols<MAXC>(acc, i + 1, red, red2, L.part1 + (long long)tl * L.pstride);
-}
-
-// h = sum of src partials; w -= Q h; then either dst = Q^T w (final = 0) or
-// dst[0] = |w|^2 (final = 1).  save (if non-null, tile 0 only) keeps h.
-template <typename T, int MAXC>
-__device__ void tile_update(const LzInst<T>& L, int tl, int i, const double* src, double* dst,
-                            double* save, int final_pass, double* lds) {
-  double* red = lds;
-  double* red2 = lds + USBS_RCH * USBS_NT;
-  double* h = red2 + USBS_RCH * USBS_NPART;
-  sum_tile_partials(src, L.ntiles, L.pstride, i + 1, h, red);
-  if (save != nullptr && tl == 0)
-    for (int j = threadIdx.x; j <= i; j += USBS_NT) save[j] = h[j];
-  const int r0 = tl * L.tile;
-  const int r1 = min(L.n, r0 + L.tile);
-  double acc[MAXC];
-#pragma unroll
-  for (int j = 0; j < MAXC; ++j) acc[j] = 0.0;
-  for (int r = r0 + (int)threadIdx.x; r < r1; r += USBS_NT) {
-    double q[MAXC];
-    double qh = 0.0;
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j) {
-      q[j] = (j <= i) ? (double)L.Q[r + (long long)j * L.ldq] : 0.0;
-      if (j <= i) qh += q[j] * h[j];
-    }
-    const T wv = (T)((double)L.w[r] - qh);
-    L.w[r] = wv;
-    const double wd = (double)wv;
-    if (final_pass) {
-      acc[0] += wd * wd;
-    } else {
-#pragma unroll
-      for (int j = 0; j < MAXC; ++j)
-        if (j <= i) acc[j] += q[j] * wd;
-    }
-  }
-  block_reduce_cols<MAXC>(acc, final_pass ? 1 : i + 1, red, red2,
-                          dst + (long long)tl * L.pstride);
-}
-
-// q_{i+1} = w / beta for the tile's rows (beta from the part3 sums).
-template <typename T>
-__device__ void tile_normalize(const LzInst<T>& L, int tl, int i, double beta) {
-  const int r0 = tl * L.tile;
-  const int r1 = min(L.n, r0 + L.tile);
-  T* __restrict__ q = L.Q + (long long)(i + 1) * L.ldq;
-  for (int r = r0 + (int)threadIdx.x; r < r1; r += USBS_NT) q[r] = (T)((double)L.w[r] / beta);
-}
-
-template <typename T>
-__device__ double lz_beta(const LzInst<T>& L, double* lds) {
-  double* s = lds + USBS_RCH * USBS_NT;
-  sum_tile_partials(L.part3, L.ntiles, L.pstride, 1, s, lds);
-  const double b = sqrt(s[0]);
   __syncthreads();
-  return b;
+  if (t < c) {
+    double s = 0.0;
+    for (int q = 0; q < G; ++q) s += red[t * G + q];
+    out[t] = s;
+  }
+  __syncthreads();
+}
+
+// out[0] = sum_rl y(rl)^2 (thread-strided partial sums, then in thread order).
+template <typename T>
+__device__ void self_dot(const T* yb, int R, double* red, double* out) {
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int rl = t; rl < R; rl += USBS_NT) s += (double)yb[rl] * (double)yb[rl];
+  red[t] = s;
+  __syncthreads();
+  if (t < 64) {
+    double v = 0.0;
+    for (int q = t * (USBS_NT / 64); q < (t + 1) * (USBS_NT / 64); ++q) v += red[q];
+    // 64 sums of 4, combined in lane order through LDS
+    red[USBS_NT + t] = v;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double v = 0.0;
+    for (int q = 0; q < 64; ++q) v += red[USBS_NT + q];
+    out[0] = v;
+  }
+  __syncthreads();
+}
+
+// y(rl) = (T)(y(rl) - sum_{j<c} Q(rl, j) h[j]), j ascending.
+template <typename T>
+__device__ void row_update(const T* qb, long long qld, T* yb, int R, int c, const double* h) {
+  for (int rl = threadIdx.x; rl < R; rl += USBS_NT) {
+    double s = 0.0;
+    for (int j = 0; j < c; ++j) s += (double)qb[(long long)j * qld + rl] * h[j];
+    yb[rl] = (T)((double)yb[rl] - s);
+  }
+  __syncthreads();
+}
+
+// beta = sqrt(sum of the part3 tile partials), identical in every block.
+template <typename T>
+__device__ double lz_beta(const LzInst<T>& L, double* red) {
+  double* s = red + USBS_NT;
+  sum_tile_partials(L.part3, L.ntiles, L.pstride, 1, s, red);
+  return sqrt(s[0]);
 }
 
 // Rotation Q[:, :k_next] = Q[:, :m] Yr; Q[:, k_next] = Q[:, m] (lanczos.py:217-221).
@@ -147,36 +138,40 @@ __device__ void tile_rotate(const LzInst<T>& L, int tl, int knext, const double*
   }
 }
 
+
 // ----------------------------------------------------------------------------
 // Ritz step (one block): arrowhead T, eigh, convergence count, saved set,
 // reorder, new alpha/beta and the rotation Yr (lanczos.py:128-168, 202-221).
 __device__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alpha, double* beta,
                         double* Yr, int* ctrl, double* lds) {
   const int t = threadIdx.x, nt = blockDim.x;
-  double* Tm = lds;
-  double* Y = Tm + m * m;
-  double* tmp = Y + m * m;
-  double* w = tmp + m * m;
-  double* coef = w + m;
-  double* red = coef + 2 * m;
-  int* sh = (int*)(red + 16);  // small int scratch
-  int* part = sh + 8;          // eigh's scratch: m*m + 2m + 4 ints
+  const int M = m + (m & 1), lda = M + 1;
+  double* Tw = lds;                 // M x lda working copy of T (destroyed)
+  double* Vw = Tw + M * lda;        // M x lda eigenvector accumulation
+  double* Y = Vw + M * lda;         // m x m sorted eigenvectors
+  double* w = Y + m * m;            // m eigenvalues
+  double* cs = w + M;               // M rotation coefficients
+  double* red = cs + M;             // 16
+  int* sh = (int*)(red + 16);       // small int scratch
+  int* part = sh + 8;               // m ints: saved-set order
+  int* sched = part + M;            // (M-1) * M ints: Jacobi pairing
   const int k = ctrl[LZ_K];
-  for (int idx = t; idx < m * m; idx += nt) {
-    const int i = idx / m, j = idx % m;
-    double v;
-    if (i == j) {
-      v = alpha[i];
-    } else {
-      const int a = i < j ? i : j, b = i < j ? j : i;
-      v = 0.0;
-      if (b == k && a < k) v = beta[a];
-      else if (b == a + 1 && a >= k) v = beta[a];
+  for (int idx = t; idx < M * lda; idx += nt) {
+    const int i = idx / lda, j = idx - (idx / lda) * lda;
+    double v = 0.0;
+    if (i < m && j < m) {
+      if (i == j) {
+        v = alpha[i];
+      } else {
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        if (b == k && a < k) v = beta[a];
+        else if (b == a + 1 && a >= k) v = beta[a];
+      }
     }
-    Tm[idx] = v;
+    Tw[idx] = v;
   }
   __syncthreads();
-  dense_eigh(Tm, m, w, Y, tmp, coef, part, red);
+  block_eigh(Tw, lda, m, w, Y, Vw, cs, sched, red);
   const double bm = beta[m - 1];
   if (t == 0) {
     int nc = m;
@@ -231,60 +226,181 @@ __device__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alp
   __syncthreads();
 }
 
+
 // ----------------------------------------------------------------------------
-// per-op kernels (one block per tile)
+// phases of step i for tile tl.  Cached (C) tiles read Q, y and the CSR slice
+// from LDS: qc (ldc x (m+1) column-major), yc (ldc), rp/ci/cv.
 
-template <typename T, int MAXC>
-__global__ void __launch_bounds__(USBS_NT) k_lz_phaseA(LzInst<T> L, int i, int mode) {
-  extern __shared__ double lds[];
-  tile_phaseA<T, MAXC>(L, blockIdx.x, i, mode, lds);
+template <typename T, bool C>
+struct TileCtx {
+  int r0, R;
+  T* qc;        // C: LDS Q block
+  int ldc;
+  T* yc;        // C: LDS work vector
+  const int* rp;  // C: local row pointers (R + 1)
+  const int* ci;  // C: column indices of the slice
+  const T* cv;    // C: values of the slice
+};
+
+// Phase A (mode 0: SpMV; mode 2: y = Q[:, i+1] for the column-by-column QR;
+// mode 3: only normalise w into column i).  first: q_i is already in Q.
+template <typename T, bool C>
+__device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, bool first,
+                       int mode, double* red, double beta_known = -1.0) {
+  const int t = threadIdx.x;
+  const T* gb = L.w + (long long)((i + 1) & 1) * L.ldq;  // w_{i-1}
+  T* wc = L.w + (long long)(i & 1) * L.ldq;               // this step's work vector (per-op)
+  double beta = 1.0;
+  if (!first) beta = beta_known >= 0.0 ? beta_known : lz_beta(L, red);
+  T* Qi = L.Q + (long long)i * L.ldq + X.r0;
+  if (!first) {
+    for (int rl = t; rl < X.R; rl += USBS_NT) {
+      const T q = (T)((double)gb[X.r0 + rl] / beta);
+      Qi[rl] = q;
+      if constexpr (C) X.qc[(long long)i * X.ldc + rl] = q;
+    }
+  }
+  if (mode == 3) { __syncthreads(); return; }
+  T* yb;
+  if constexpr (C) yb = X.yc; else yb = wc + X.r0;
+  const T* xq = L.Q + (long long)i * L.ldq;
+  for (int rl = t; rl < X.R; rl += USBS_NT) {
+    double s = 0.0;
+    if (mode == 0) {
+      if constexpr (C) {
+        const int p1 = X.rp[rl + 1];
+        for (int p = X.rp[rl]; p < p1; ++p) {
+          const int c = X.ci[p];
+          const T x = first ? xq[c] : (T)((double)gb[c] / beta);
+          s += (double)X.cv[p] * (double)x;
+        }
+      } else {
+        const int r = X.r0 + rl;
+        const int p1 = L.row_ptr[r + 1];
+        for (int p = L.row_ptr[r]; p < p1; ++p) {
+          const int c = L.col[p];
+          const T x = first ? xq[c] : (T)((double)gb[c] / beta);
+          s += (double)L.val[p] * (double)x;
+        }
+      }
+    } else {
+      s = (double)L.Q[(long long)(i + 1) * L.ldq + X.r0 + rl];
+    }
+    yb[rl] = (T)s;
+  }
+  __syncthreads();
+  const T* qb;
+  long long qld;
+  if constexpr (C) { qb = X.qc; qld = X.ldc; } else { qb = L.Q + X.r0; qld = L.ldq; }
+  col_dots(qb, qld, yb, X.R, i + 1, red, L.part1 + (long long)tl * L.pstride);
 }
 
-template <typename T, int MAXC>
-__global__ void __launch_bounds__(USBS_NT) k_lz_phaseB(LzInst<T> L, int i) {
-  extern __shared__ double lds[];
-  tile_update<T, MAXC>(L, blockIdx.x, i, L.part1, L.part2, L.h1, 0, lds);
-}
-
-template <typename T, int MAXC>
-__global__ void __launch_bounds__(USBS_NT) k_lz_phaseC(LzInst<T> L, int i) {
-  extern __shared__ double lds[];
-  tile_update<T, MAXC>(L, blockIdx.x, i, L.part2, L.part3, L.h2, 1, lds);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(USBS_NT) k_lz_phaseD(LzInst<T> L, int i) {
-  extern __shared__ double lds[];
-  const double beta = lz_beta(L, lds);
-  const bool brk = beta < 1e-15;
-  if (!brk) tile_normalize(L, blockIdx.x, i, beta);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    L.alpha[i] = L.h1[i] + L.h2[i];
-    L.beta[i] = beta;
-    L.ctrl[LZ_BREAKDOWN] = brk ? 1 : 0;
-    if (brk) L.ctrl[LZ_NBREAK] += 1;
-    L.ctrl[LZ_STEPS] += 1;
+// Phases B (final = 0) and C (final = 1).
+template <typename T, bool C>
+__device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, int final_pass,
+                        double* red) {
+  const int t = threadIdx.x;
+  double* h = red + LZ_RED;
+  const double* src = final_pass ? L.part2 : L.part1;
+  sum_tile_partials(src, L.ntiles, L.pstride, i + 1, h, red);
+  if (tl == 0)
+    for (int j = t; j <= i; j += USBS_NT) (final_pass ? L.h2 : L.h1)[j] = h[j];
+  T* yb;
+  const T* qb;
+  long long qld;
+  T* wc = L.w + (long long)(i & 1) * L.ldq + X.r0;
+  if constexpr (C) { yb = X.yc; qb = X.qc; qld = X.ldc; } else { yb = wc; qb = L.Q + X.r0; qld = L.ldq; }
+  row_update(qb, qld, yb, X.R, i + 1, h);
+  if (!final_pass) {
+    col_dots(qb, qld, yb, X.R, i + 1, red, L.part2 + (long long)tl * L.pstride);
+  } else {
+    if constexpr (C)
+      for (int rl = t; rl < X.R; rl += USBS_NT) wc[rl] = yb[rl];
+    self_dot(yb, X.R, red, L.part3 + (long long)tl * L.pstride);
+    if (tl == 0 && t == 0) {
+      L.alpha[i] = L.h1[i] + h[i];
+      L.ctrl[LZ_STEPS] += 1;
+    }
   }
 }
 
-// Breakdown restart vector, one block walking every tile in order (same
-// tile arithmetic as the multi-block phases, so results do not depend on
-// which form ran).  No-op unless the step flagged a breakdown.
-template <typename T, int MAXC>
-__global__ void __launch_bounds__(USBS_NT) k_lz_breakdown(LzInst<T> L, int i) {
-  extern __shared__ double lds[];
-  if (L.ctrl[LZ_BREAKDOWN] == 0) return;
-  for (int tl = 0; tl < L.ntiles; ++tl) tile_phaseA<T, MAXC>(L, tl, i, 1, lds);
-  __syncthreads();
-  for (int tl = 0; tl < L.ntiles; ++tl) tile_update<T, MAXC>(L, tl, i, L.part1, L.part2, nullptr, 0, lds);
-  __syncthreads();
-  for (int tl = 0; tl < L.ntiles; ++tl) tile_update<T, MAXC>(L, tl, i, L.part2, L.part3, nullptr, 1, lds);
-  __syncthreads();
-  const double b = lz_beta(L, lds);
-  for (int tl = 0; tl < L.ntiles; ++tl) tile_normalize(L, tl, i, b);
+// Breakdown restart vector for the residual w_{i-1} (lanczos.py:107-117):
+// counter normals keyed by step i, orthogonalised twice against columns
+// 0..i-1 and left in w_{i-1} with |.|^2 partials in part3.  sub = 0, 1, 2 are
+// the three reduction phases.
+template <typename T>
+__device__ void breakdown_sub(const LzInst<T>& L, int tl, int i, int sub, double* red) {
+  const int t = threadIdx.x;
+  const int r0 = tl * L.tile;
+  const int R = min(L.n, r0 + L.tile) - r0;
+  T* gb = L.w + (long long)((i + 1) & 1) * L.ldq + r0;
+  const T* qb = L.Q + r0;
+  if (sub == 0) {
+    for (int rl = t; rl < R; rl += USBS_NT) gb[rl] = (T)counter_normal((uint64_t)i, (uint64_t)(r0 + rl));
+    __syncthreads();
+    col_dots(qb, L.ldq, (const T*)gb, R, i, red, L.part1 + (long long)tl * L.pstride);
+  } else {
+    double* h = red + LZ_RED;
+    sum_tile_partials(sub == 1 ? L.part1 : L.part2, L.ntiles, L.pstride, i, h, red);
+    row_update(qb, L.ldq, gb, R, i, h);
+    if (sub == 1) col_dots(qb, L.ldq, (const T*)gb, R, i, red, L.part2 + (long long)tl * L.pstride);
+    else self_dot((const T*)gb, R, red, L.part3 + (long long)tl * L.pstride);
+  }
 }
 
-#define LZ_RITZ_NT 1024
+// beta_{i-1} from part3: recorded in beta[i-1]; returns whether it broke down.
+template <typename T>
+__device__ bool lz_record_beta(const LzInst<T>& L, int i, bool writer, double* red,
+                               double* beta_out = nullptr) {
+  const double b = lz_beta(L, red);
+  if (beta_out) *beta_out = b;
+  const bool brk = b < 1e-15;
+  if (writer && threadIdx.x == 0) {
+    L.beta[i - 1] = b;
+    L.ctrl[LZ_BREAKDOWN] = brk ? 1 : 0;
+    if (brk) L.ctrl[LZ_NBREAK] += 1;
+  }
+  __syncthreads();
+  return brk;
+}
+
+
+// ----------------------------------------------------------------------------
+// per-op kernels (one block per tile; Q and the work vector in global memory)
+
+template <typename T>
+__device__ TileCtx<T, false> plain_tile(const LzInst<T>& L, int tl) {
+  TileCtx<T, false> X{};
+  X.r0 = tl * L.tile;
+  X.R = min(L.n, X.r0 + L.tile) - X.r0;
+  return X;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(USBS_NT) k_lz_A(LzInst<T> L, int i, int first, int mode) {
+  extern __shared__ double lds[];
+  phaseA<T, false>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, first != 0, mode, lds);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(USBS_NT) k_lz_BC(LzInst<T> L, int i, int final_pass) {
+  extern __shared__ double lds[];
+  phaseBC<T, false>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, final_pass, lds);
+}
+
+// beta_{i-1} and, on breakdown, the restart vector: one block walking every
+// tile in order (same tile arithmetic as the multi-block phases).
+template <typename T>
+__global__ void __launch_bounds__(USBS_NT) k_lz_pre(LzInst<T> L, int i) {
+  extern __shared__ double lds[];
+  if (!lz_record_beta(L, i, true, lds)) return;
+  for (int sub = 0; sub < 3; ++sub) {
+    for (int tl = 0; tl < L.ntiles; ++tl) breakdown_sub(L, tl, i, sub, lds);
+    __syncthreads();
+  }
+}
+
+#define LZ_RITZ_NT 256
 template <typename T>
 __global__ void __launch_bounds__(LZ_RITZ_NT) k_lz_ritz(LzInst<T> L) {
   extern __shared__ double lds[];
@@ -326,21 +442,31 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate_multi(const LzInst<T>* __
   tile_rotate<T, MAXC>(L, mp.y, knext, lds);
 }
 
+// Steps i = i0..m-1 of one restart (or of the QR, mode 2) as per-op launches,
+// then the normalisation of Q[:, m].
+template <typename T>
+static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
+  if (i0 >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
+  const size_t lds = tile_lds_bytes(L.m + 1);
+  const dim3 g(L.ntiles), b(USBS_NT);
+  for (int i = i0; i < L.m; ++i) {
+    if (i > i0) k_lz_pre<T><<<1, b, lds, s>>>(L, i);
+    k_lz_A<T><<<g, b, lds, s>>>(L, i, i == i0 ? 1 : 0, mode);
+    k_lz_BC<T><<<g, b, lds, s>>>(L, i, 0);
+    k_lz_BC<T><<<g, b, lds, s>>>(L, i, 1);
+  }
+  k_lz_pre<T><<<1, b, lds, s>>>(L, L.m);
+  k_lz_A<T><<<g, b, lds, s>>>(L, L.m, 0, 3);
+}
+
 template <typename T, int MAXC>
 static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
-  const size_t lds = sizeof(double) * lds_doubles_for(L.m, MAXC);
   const dim3 g(L.ntiles), b(USBS_NT);
   int k = 0;
   for (int restart = 0; restart < L.max_restarts; ++restart) {
-    for (int i = k; i < L.m; ++i) {
-      k_lz_phaseA<T, MAXC><<<g, b, lds, s>>>(L, i, 0);
-      k_lz_phaseB<T, MAXC><<<g, b, lds, s>>>(L, i);
-      k_lz_phaseC<T, MAXC><<<g, b, lds, s>>>(L, i);
-      k_lz_phaseD<T><<<g, b, lds, s>>>(L, i);
-      k_lz_breakdown<T, MAXC><<<1, b, lds, s>>>(L, i);
-    }
-    k_lz_ritz<T><<<1, LZ_RITZ_NT, lds, s>>>(L);
-    k_lz_rotate<T, MAXC><<<g, b, lds, s>>>(L);
+    perop_steps(L, k, 0, s);
+    k_lz_ritz<T><<<1, LZ_RITZ_NT, ritz_lds_bytes(L.m), s>>>(L);
+    k_lz_rotate<T, MAXC><<<g, b, rot_lds_bytes(L.m), s>>>(L);
     hipError_t e = hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(int) * LZ_NCTRL,
                                   hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return (int)e;
@@ -355,25 +481,11 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
 // Column-by-column CGS2 orthonormalisation of Q[:, 1..m] (Q[:, 0] = 0):
 // step i orthonormalises column i+1 against columns 0..i with the Lanczos
 // step kernels in mode 2 (a breakdown draws a counter-based random column).
-template <typename T, int MAXC>
-static int orth_impl(const LzInst<T>& L, hipStream_t s) {
-  const size_t lds = sizeof(double) * lds_doubles_for(L.m, MAXC);
-  const dim3 g(L.ntiles), b(USBS_NT);
-  for (int i = 0; i < L.m; ++i) {
-    k_lz_phaseA<T, MAXC><<<g, b, lds, s>>>(L, i, 2);
-    k_lz_phaseB<T, MAXC><<<g, b, lds, s>>>(L, i);
-    k_lz_phaseC<T, MAXC><<<g, b, lds, s>>>(L, i);
-    k_lz_phaseD<T><<<g, b, lds, s>>>(L, i);
-    k_lz_breakdown<T, MAXC><<<1, b, lds, s>>>(L, i);
-  }
-  return (int)hipGetLastError();
-}
-
 template <typename T>
 int orth_perop(const LzInst<T>& L, hipStream_t s) {
-  if (L.m + 1 <= 17) return orth_impl<T, 17>(L, s);
-  if (L.m + 1 <= 33) return orth_impl<T, 33>(L, s);
-  return (int)hipErrorInvalidValue;
+  if (L.m + 1 > 49) return (int)hipErrorInvalidValue;
+  perop_steps(L, 0, 2, s);
+  return (int)hipGetLastError();
 }
 template int orth_perop<double>(const LzInst<double>&, hipStream_t);
 template int orth_perop<float>(const LzInst<float>&, hipStream_t);
@@ -389,8 +501,18 @@ int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
 // ----------------------------------------------------------------------------
 // persistent form: one cooperative launch runs every Lanczos step of restart
 // r for all instances; wgmap[blockIdx.x] = (instance, local workgroup index).
+// C: every workgroup owns one tile and keeps it in LDS (layout lz_cache_*).
 
-template <typename T, int MAXC>
+template <typename T>
+static inline size_t lz_cache_bytes(int tile, int m, int nnz_tile) {
+  size_t b = sizeof(double) * (size_t)(LZ_RED + m + 1 + 8);   // reduction scratch, h
+  b += sizeof(T) * (size_t)tile * (size_t)(m + 2);            // Q block + work vector
+  b += sizeof(T) * (size_t)nnz_tile;                          // values
+  b += sizeof(int) * (size_t)(tile + 1 + nnz_tile);           // row pointers, columns
+  return (b + 255) / 256 * 256;
+}
+
+template <typename T, bool C>
 __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __restrict__ insts,
                                                            const int2* __restrict__ wgmap, int r,
                                                            unsigned long long spin_limit) {
@@ -398,48 +520,75 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
   const int2 mp = wgmap[blockIdx.x];
   const LzInst<T> L = insts[mp.x];
   if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;  // uniform over the instance's workgroups
-  const int lw = mp.y, G = L.nblocks;
+  const int lw = mp.y, G = L.nblocks, t = threadIdx.x;
   int* err = L.ctrl + LZ_ERROR;
   unsigned* bar = L.bar + (r < L.nbar ? r : L.nbar - 1);
   unsigned target = 0;
   const int k = L.ctrl[LZ_K];
-  bool ok = true;
-  if (lw == 0 && threadIdx.x == 0 && r < L.nbar) {
+  if (k >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
+  if (lw == 0 && t == 0 && r < L.nbar) {
     int sumc = 0;
     for (int i = k; i < L.m; ++i) sumc += i + 1;
     L.lstat[2 * r] = L.m - k;
     L.lstat[2 * r + 1] = sumc;
   }
-  for (int i = k; i < L.m && ok; ++i) {
-    for (int tl = lw; tl < L.ntiles; tl += G) tile_phaseA<T, MAXC>(L, tl, i, 0, lds);
-    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G)
-      tile_update<T, MAXC>(L, tl, i, L.part1, L.part2, L.h1, 0, lds);
-    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G)
-      tile_update<T, MAXC>(L, tl, i, L.part2, L.part3, L.h2, 1, lds);
-    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    double beta = lz_beta(L, lds);
-    const bool brk = beta < 1e-15;
-    if (lw == 0 && threadIdx.x == 0) {
-      L.alpha[i] = L.h1[i] + L.h2[i];
-      L.beta[i] = beta;
-      L.ctrl[LZ_BREAKDOWN] = brk ? 1 : 0;
-      if (brk) L.ctrl[LZ_NBREAK] += 1;
-      L.ctrl[LZ_STEPS] += 1;
+  double* red = lds;
+  TileCtx<T, C> X{};
+  if constexpr (C) {
+    // one tile per workgroup: load its CSR slice and Q[:, 0..k] into LDS
+    X.r0 = lw * L.tile;
+    X.R = min(L.n, X.r0 + L.tile) - X.r0;
+    X.ldc = L.tile;
+    T* qc = (T*)(lds + LZ_RED + L.m + 1 + 8);
+    X.qc = qc;
+    X.yc = qc + (long long)L.tile * (L.m + 1);
+    T* cv = X.yc + L.tile;
+    const int pbase = L.row_ptr[X.r0];
+    const int nnzt = L.row_ptr[X.r0 + X.R] - pbase;
+    int* rp = (int*)(cv + L.tile_nnz_max);
+    int* ci = rp + L.tile + 1;
+    for (int rl = t; rl <= X.R; rl += USBS_NT) rp[rl] = L.row_ptr[X.r0 + rl] - pbase;
+    for (int p = t; p < nnzt; p += USBS_NT) { ci[p] = L.col[pbase + p]; cv[p] = L.val[pbase + p]; }
+    for (int j = 0; j <= k; ++j)
+      for (int rl = t; rl < X.R; rl += USBS_NT)
+        qc[(long long)j * X.ldc + rl] = L.Q[(long long)j * L.ldq + X.r0 + rl];
+    X.rp = rp; X.ci = ci; X.cv = cv;
+    __syncthreads();
+  }
+  auto tile_of = [&](int tl) {
+    if constexpr (C) return X;
+    else {
+      TileCtx<T, C> Y{};
+      Y.r0 = tl * L.tile;
+      Y.R = min(L.n, Y.r0 + L.tile) - Y.r0;
+      return Y;
     }
-    if (brk) {
-      for (int tl = lw; tl < L.ntiles; tl += G) tile_phaseA<T, MAXC>(L, tl, i, 1, lds);
-      if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-      for (int tl = lw; tl < L.ntiles; tl += G)
-        tile_update<T, MAXC>(L, tl, i, L.part1, L.part2, nullptr, 0, lds);
-      if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-      for (int tl = lw; tl < L.ntiles; tl += G)
-        tile_update<T, MAXC>(L, tl, i, L.part2, L.part3, nullptr, 1, lds);
-      if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-      beta = lz_beta(L, lds);
+  };
+  bool ok = true;
+  for (int i = k; i <= L.m && ok; ++i) {
+    double beta = -1.0;  // |w_{i-1}| when known here (no breakdown)
+    if (i > k) {
+      // beta_{i-1} and the breakdown restart vector
+      if (lz_record_beta(L, i, lw == 0, red, &beta)) {
+        beta = -1.0;
+        for (int sub = 0; sub < 3 && ok; ++sub) {
+          for (int tl = lw; tl < L.ntiles; tl += G) breakdown_sub(L, tl, i, sub, red);
+          ok = grid_barrier(bar, G, target, err, spin_limit);
+        }
+        if (!ok) break;
+      }
     }
-    for (int tl = lw; tl < L.ntiles; tl += G) tile_normalize(L, tl, i, beta);
+    if (i == L.m) {
+      for (int tl = lw; tl < L.ntiles; tl += G)
+        phaseA<T, C>(L, tile_of(tl), tl, i, false, 3, red, beta);
+      break;
+    }
+    for (int tl = lw; tl < L.ntiles; tl += G)
+      phaseA<T, C>(L, tile_of(tl), tl, i, i == k, 0, red, beta);
+    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C>(L, tile_of(tl), tl, i, 0, red);
+    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C>(L, tile_of(tl), tl, i, 1, red);
     ok = grid_barrier(bar, G, target, err, spin_limit);
   }
 }
@@ -448,17 +597,21 @@ template <typename T, int MAXC>
 static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                            int max_restarts, hipStream_t s, const int* dev_wgmap,
                            const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                           hipEvent_t* evs) {
-  const size_t lds = sizeof(double) * lds_doubles_for(m, MAXC);
+                           size_t cache_bytes, hipEvent_t* evs) {
+  const bool cache = cache_bytes > 0;
+  const size_t lds = cache ? cache_bytes : tile_lds_bytes(m + 1);
   for (int r = 0; r < max_restarts; ++r) {
     void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&r, (void*)&spin_limit};
     if (evs && hipEventRecord(evs[2 * r], s) != hipSuccess) return (int)hipErrorUnknown;
-    hipError_t e = hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, MAXC>, dim3(total_wgs),
-                                              dim3(USBS_NT), args, lds, s);
+    hipError_t e = cache ? hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, true>,
+                                                      dim3(total_wgs), dim3(USBS_NT), args, lds, s)
+                         : hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, false>,
+                                                      dim3(total_wgs), dim3(USBS_NT), args, lds, s);
     if (e != hipSuccess) return (int)e;
     if (evs && hipEventRecord(evs[2 * r + 1], s) != hipSuccess) return (int)hipErrorUnknown;
-    k_lz_ritz_multi<T><<<ninst, LZ_RITZ_NT, lds, s>>>(dev_insts, r);
-    k_lz_rotate_multi<T, MAXC><<<total_tiles, USBS_NT, lds, s>>>(dev_insts, (const int2*)dev_tilemap, r);
+    k_lz_ritz_multi<T><<<ninst, LZ_RITZ_NT, ritz_lds_bytes(m), s>>>(dev_insts, r);
+    k_lz_rotate_multi<T, MAXC><<<total_tiles, USBS_NT, rot_lds_bytes(m), s>>>(
+        dev_insts, (const int2*)dev_tilemap, r);
     e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
   }
@@ -466,52 +619,79 @@ static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs,
 }
 
 template <typename T>
+size_t lanczos_cache_bytes(const LzInst<T>& L) {
+  if (L.nblocks < L.ntiles) return 0;  // a workgroup would own several tiles
+  const size_t b = lz_cache_bytes<T>(L.tile, L.m, L.tile_nnz_max);
+  return b <= LZ_CACHE_MAX_BYTES ? b : 0;
+}
+template size_t lanczos_cache_bytes<double>(const LzInst<double>&);
+template size_t lanczos_cache_bytes<float>(const LzInst<float>&);
+
+template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
                        const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                       hipEvent_t* evs) {
+                       size_t cache_bytes, hipEvent_t* evs) {
   if (m + 1 <= 17)
     return persistent_impl<T, 17>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit, evs);
+                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs);
   if (m + 1 <= 33)
     return persistent_impl<T, 33>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit, evs);
+                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs);
   if (m + 1 <= 49)
     return persistent_impl<T, 49>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit, evs);
+                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs);
   return (int)hipErrorInvalidValue;
 }
 
-int lanczos_max_coresident_blocks(int m, int device) {
-  int maxc = (m + 1 <= 17) ? 17 : (m + 1 <= 33 ? 33 : 49);
-  const size_t lds = sizeof(double) * lds_doubles_for(m, maxc);
-  int per_cu = 0;
-  hipError_t e;
-  if (maxc == 17)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, 17>, USBS_NT, lds);
-  else if (maxc == 33)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, 33>, USBS_NT, lds);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, 49>, USBS_NT, lds);
-  int per_cu64 = 0;
-  if (maxc == 17)
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, 17>, USBS_NT, lds);
-  else if (maxc == 33)
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, 33>, USBS_NT, lds);
-  else
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, 49>, USBS_NT, lds);
-  if (e != hipSuccess) return 0;
-  if (per_cu64 < per_cu) per_cu = per_cu64;
+int lanczos_max_coresident_blocks(size_t lds_bytes, int device) {
+  int per_cu = 0, per_cu64 = 0, per_cu_c = 0, per_cu64_c = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, false>, USBS_NT,
+                                                   lds_bytes) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, false>,
+                                                   USBS_NT, lds_bytes) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_lz_steps_coop<float, true>, USBS_NT,
+                                                   lds_bytes) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64_c, k_lz_steps_coop<double, true>,
+                                                   USBS_NT, lds_bytes) != hipSuccess)
+    return 0;
+  int pc = per_cu;
+  if (per_cu64 < pc) pc = per_cu64;
+  if (per_cu_c < pc) pc = per_cu_c;
+  if (per_cu64_c < pc) pc = per_cu64_c;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
-  return per_cu * prop.multiProcessorCount;
+  return pc * prop.multiProcessorCount;
 }
+
+template <typename T>
+int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cache_bytes) {
+  int nb = L.ntiles;
+  if (want_wgs > 0 && want_wgs < nb) nb = want_wgs;
+  L.nblocks = nb;
+  size_t cb = lanczos_cache_bytes(L);
+  int cores = lanczos_max_coresident_blocks(cb ? cb : tile_lds_bytes(L.m + 1), device);
+  if (cb && cores < nb) {
+    cb = 0;
+    cores = lanczos_max_coresident_blocks(tile_lds_bytes(L.m + 1), device);
+  }
+  if (cores < 1) return (int)hipErrorInvalidConfiguration;
+  if (nb > cores) nb = cores;
+  L.nblocks = nb;
+  if (nb < L.ntiles) cb = 0;
+  *cache_bytes = cb;
+  return 0;
+}
+template int lanczos_plan_persistent<double>(LzInst<double>&, int, int, size_t*);
+template int lanczos_plan_persistent<float>(LzInst<float>&, int, int, size_t*);
 
 template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*);
 template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*);
 template int lanczos_persistent<double>(const LzInst<double>*, int, int, int, int, hipStream_t,
-                                        const int*, const int*, int, unsigned long long, hipEvent_t*);
+                                        const int*, const int*, int, unsigned long long, size_t,
+                                        hipEvent_t*);
 template int lanczos_persistent<float>(const LzInst<float>*, int, int, int, int, hipStream_t,
-                                       const int*, const int*, int, unsigned long long, hipEvent_t*);
+                                       const int*, const int*, int, unsigned long long, size_t,
+                                       hipEvent_t*);
 
 }  // namespace usbs

@@ -30,7 +30,7 @@ struct LzInst {
   const T* val;
   T* Q;          // n x (m+1), leading dimension ldq
   long long ldq;
-  T* w;          // n
+  T* w;          // 2 x ldq: residual / work vectors of alternate steps
   double* part1; // ntiles x pstride, one array per reduction pass
   double* part2;
   double* part3;
@@ -44,7 +44,11 @@ struct LzInst {
   int nbar;      // number of counters in bar (>= max_restarts)
   int* lstat;    // 2 x nbar: per persistent launch r, steps done and the sum
                  // over those steps of the valid column count i+1
+  int tile_nnz_max;  // largest number of operator entries in one row tile
 };
+
+// Largest LDS block of a persistent workgroup that caches its tile.
+#define LZ_CACHE_MAX_BYTES (64 * 1024)
 
 int tile_rows_for(int n);
 
@@ -55,13 +59,22 @@ int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl);
 // one cooperative launch of all Lanczos steps (dev_wgmap: int2 per workgroup
 // = instance, local workgroup), then the Ritz kernel (one block per
 // instance) and the rotation (dev_tilemap: int2 per tile = instance, tile).
-// evs (optional, 2 x max_restarts events) brackets each cooperative launch.
+// evs (optional, 2 x max_restarts events) brackets each cooperative launch;
+// cache_bytes > 0 selects the form that keeps each workgroup's tile in LDS
+// (from lanczos_plan_persistent).
 template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
                        const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                       hipEvent_t* evs = nullptr);
-int lanczos_max_coresident_blocks(int m, int device);
+                       size_t cache_bytes, hipEvent_t* evs = nullptr);
+// Workgroup count (L.nblocks) and LDS cache size for a persistent launch of
+// one instance: one workgroup per tile unless want_wgs > 0 asks for fewer,
+// capped by what can be co-resident on the device.
+template <typename T>
+int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cache_bytes);
+template <typename T>
+size_t lanczos_cache_bytes(const LzInst<T>& L);
+int lanczos_max_coresident_blocks(size_t lds_bytes, int device);
 template <typename T>
 int orth_perop(const LzInst<T>& L, hipStream_t s);
 

@@ -98,6 +98,7 @@ struct Solver {
   LzInst<T>* d_lz_kc = nullptr;
   int* d_map = nullptr;
   int nb = 1;
+  size_t cache_bytes = 0;            // LDS per persistent workgroup when tiles are cached
   int host_ctrl[LZ_NCTRL] = {0};
   int lz_restarts = 0;
   hipStream_t s2 = nullptr;          // side stream: the lower-bound IPM overlaps the Lanczos call
@@ -202,9 +203,9 @@ struct Solver {
       c->stats.value_bytes = (int)sizeof(T);
     }
     if (p.exec_mode == USBS_EXEC_PERSISTENT) {
-      nb = std::min(std::min(lz.ntiles, c->persist_wgs > 0 ? c->persist_wgs : lz.ntiles),
-                    std::max(1, c->coresident));
-      lz.nblocks = nb;
+      if (lanczos_plan_persistent<T>(lz, c->device, c->persist_wgs, &cache_bytes))
+        return set_err(c, USBS_ERR_HIP, "no persistent Lanczos launch fits the device");
+      nb = lz.nblocks;
       if (!(d_lz_k = alloc<LzInst<T>>(1)) || !(d_lz_kc = alloc<LzInst<T>>(1)) || !(d_map = alloc<int>(2 * (nb + lz.ntiles))))
         return USBS_ERR_OOM;
       std::vector<int2> map(nb + lz.ntiles);
@@ -248,7 +249,8 @@ struct Solver {
         if (rc) return rc;
       }
       CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
-                               d_map + 2 * nb, lz.ntiles, 1ull << 26, tev.empty() ? nullptr : tev.data()));
+                               d_map + 2 * nb, lz.ntiles, 1ull << 26, cache_bytes,
+                               tev.empty() ? nullptr : tev.data()));
       tev_pending = !tev.empty();
     }
     return USBS_OK;
