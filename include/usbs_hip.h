@@ -113,6 +113,35 @@ int usbs_solve(usbs_ctx* ctx, const usbs_params* params, const double* y0, const
                const double* q0, double* y_out, double* z_out, double* P_out, double* scal_out,
                usbs_record* log, int log_cap, int* log_len);
 
+/* Parameters of cgal() (solver/cgal.py:21-39), plus the execution choices. */
+typedef struct usbs_cgal_params {
+  int32_t max_iters;
+  int32_t lanczos_inner_iterations, lanczos_max_restarts;
+  int32_t sketch_dim;  /* r > 0: Nystrom sketch P (n x r) of the primal             */
+  int32_t rounding;    /* 1: max-cut rounding at every condition check              */
+  int32_t precision;   /* USBS_PREC_*                                               */
+  int32_t exec_mode;   /* USBS_EXEC_* for the Lanczos calls                          */
+  int32_t reserved;
+  double trace_ub, beta0;
+  double max_time, obj_gap_eps, infeas_gap_eps, max_infeas_eps;
+  double subprob_eps, cond_exp_base;
+  double SCALE_C, SCALE_X;
+} usbs_cgal_params;
+
+/*
+ * The whole CGAL loop on the device (solver/cgal.py:21-243): rank-one
+ * Frank-Wolfe steps from the smallest eigenvector of -C + A*(y + beta (z -
+ * proj_b)), sketch P = (1 - eta) P + eta trace_ub u (u^T Omega), dual step
+ * clipped to [0, beta0].  Same arguments and outputs as usbs_solve; records
+ * carry t, time, primal_obj, obj_gap, infeas_gap, max_infeas, callback_val
+ * (the fields of the reference's log line, cgal.py:77-88) and NaN elsewhere.
+ * The sketch is required (the reference's dense-X mode is not provided).
+ */
+int usbs_cgal_solve(usbs_ctx* ctx, const usbs_cgal_params* params, const double* y0,
+                    const double* z0, double tr_X0, double primal_obj0, const double* Omega,
+                    const double* P0, const double* q0, double* y_out, double* z_out,
+                    double* P_out, double* scal_out, usbs_record* log, int log_cap, int* log_len);
+
 /*
  * Nystrom fixed-rank reconstruction from the sketch (utils/common.py:99-116):
  * E (n x r row-major), Lambda (r), singular values in descending order.  A

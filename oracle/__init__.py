@@ -13,6 +13,7 @@ Module map (reference file:line each function follows is in its docstring):
 * ``loop``       -- utils/loop.py:21-115 (bounded while-loop semantics)
 * ``ipm``        -- solver/usbs.py:25-433 (prox-bundle IPM subproblems)
 * ``usbs``       -- solver/usbs.py:436-789 (the USBS outer loop)
+* ``cgal``       -- solver/cgal.py:21-243 (the CGAL loop)
 * ``common``     -- utils/common.py:11-121 (SDPState, scaling, Nystrom)
 * ``maxcut``     -- utils/maxcut_helpers.py:26-92, 292-302
 
@@ -37,6 +38,7 @@ PINS = {
     "ipm.solve_quad_subprob_ipm": "scipy SLSQP on the same convex QP over (S>=0, eta>=0, trS+eta<=1), k=2",
     "ipm.compute_lb_spec_est_ipm": "closed form: lower bound of a linear objective over the spectraplex-with-eta",
     "usbs.usbs": "SDP closed forms: bipartite |E|, K_n n^2/4, odd cycle (n/2)(1+cos(pi/n))",
+    "cgal.cgal": "SDP closed forms: bipartite |E| (exact), K_n n^2/4 (1e-3, first-order method)",
     "common.reconstruct_from_sketch": "exact recovery of a rank<=r PSD matrix (Tropp et al. 2017)",
     "maxcut.compute_max_cut": "brute force over all 2^n cuts on n<=14; integer edge-sum identity",
 }

@@ -69,3 +69,18 @@ def _solve(W, iters, rho, eps):
 def test_usbs_closed_forms(name, W, value):
     fin, log = _solve(W, 400, 1.0, 1e-7)
     assert abs(fin.primal_obj - value) <= 1e-4 * value, (name, fin.primal_obj, value, len(log))
+
+
+@pytest.mark.parametrize("name,W,value,tol", [
+    ("bipartite torus", problems.torus(6, 8, seed=0, signed=False), 96.0, 1e-9),
+    ("K_8", problems.erdos_renyi(8, 1.0, seed=0), 16.0, 1e-3),
+])
+def test_cgal_closed_forms(name, W, value, tol):
+    """The CGAL restatement reaches the max-cut SDP value of graphs whose
+    value is known in closed form (bipartite: |E|; K_n: n^2/4)."""
+    from oracle import cgal as ocg
+    n = W.shape[0]
+    st = omc.initialize_state(W, 4, problems.gaussian_matrix(n, 4, 0))
+    out, _ = ocg.cgal(st, n, n, float(n), 1.0, 2000, 1e9, 1e-6, 1e-6, 1e-6, min(n, 32), 50, 1e-10,
+                      1.0, None, None, None, problems.unit_start_vector(n, 1))
+    assert abs(out.primal_obj - value) <= tol * value, (name, out.primal_obj)

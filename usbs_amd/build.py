@@ -15,7 +15,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libusbs_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["engine.hip", "lanczos.hip", "ops.hip", "dense_kernels.hip", "solve.hip", "api_ops.hip"]
+SOURCES = ["engine.hip", "lanczos.hip", "ops.hip", "dense_kernels.hip", "solve.hip", "cgal.hip",
+           "api_ops.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

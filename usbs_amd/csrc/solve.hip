@@ -18,41 +18,9 @@
 
 #include "../../include/usbs_hip.h"
 #include "solve.hpp"
+#include "solve_util.hpp"
 
 namespace usbs {
-
-template <typename T>
-__global__ void k_load_colmajor(long long n, int r, const double* src, T* dst, long long ld) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * r) return;
-  const long long row = i / r, c = i % r;
-  dst[row + c * ld] = (T)src[i];
-}
-
-template <typename T>
-__global__ void k_store_rowmajor(long long n, int r, const T* src, long long ld, double* dst) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * r) return;
-  const long long row = i / r, c = i % r;
-  dst[i] = (double)src[row + c * ld];
-}
-
-template <typename T>
-__global__ void k_fill(T* p, long long count, T v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = v;
-}
-
-// Lanczos per-call reset: Q[:, 0] = q0, alpha = beta = 0, control and barrier 0.
-template <typename T>
-__global__ void k_lz_start(LzInst<T> L, const double* q0) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < L.n) L.Q[i] = (T)q0[i];
-  if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
-  if (i < LZ_NCTRL) L.ctrl[i] = 0;
-  for (long long x = i; x < L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
-  for (long long x = i; x < 2 * L.nbar; x += (long long)gridDim.x * blockDim.x) L.lstat[x] = 0;
-}
 
 // Penalised dual objective of the initial point (solver/usbs.py:704-706).
 __global__ void k_init_pen(double* sc, const double* by4, const double* alpha, double tu) {
@@ -393,25 +361,6 @@ struct Solver {
   }
 #undef CK
 };
-
-static std::vector<long long> geometric_schedule(long long maxiter, double base) {
-  // utils/loop.py:85-98
-  const double num = std::ceil(std::log((base - 1) * maxiter - 1) / std::log(base)) + 1;
-  std::vector<long long> steps;
-  long long total = 0;
-  for (long long i = 0; i < (long long)num; ++i) {
-    const long long st = (long long)std::pow(base, (double)i);
-    steps.push_back(st);
-  }
-  std::vector<long long> kept;
-  long long cum = 0;
-  for (long long st : steps) {
-    cum += st;
-    if (cum < maxiter) { kept.push_back(st); total += st; }
-  }
-  kept.push_back(maxiter - total);
-  return kept;
-}
 
 template <typename T>
 static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const double* z0, double trX0,
