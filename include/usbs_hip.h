@@ -76,6 +76,22 @@ int usbs_lanczos_smallest(usbs_ctx* ctx, double C_coef, const double* adjoint_le
                           double* eigvals, double* eigvecs, int* info);
 
 
+/*
+ * Batched form of usbs_lanczos_smallest: ninst independent instances, one per
+ * context (each with its own uploaded problem, sizes may differ; all on one
+ * device), advanced together in one persistent launch per restart (Ritz and
+ * rotation kernels cover every instance).  Each instance's results are
+ * bitwise the same as its own usbs_lanczos_smallest call with
+ * USBS_EXEC_PERSISTENT.  Arrays are per instance; info holds 4 ints per
+ * instance.  No counterpart in the reference, which solves one problem per
+ * call (solver/lanczos.py:239-284 is the per-instance arithmetic).
+ */
+int usbs_lanczos_smallest_batch(usbs_ctx* const* ctxs, int ninst, double C_coef,
+                                const double* const* adjoint_left_vecs, const double* const* q0s,
+                                int num_desired, int inner_iterations, int max_restarts,
+                                double tolerance, int precision, double* const* eigvals,
+                                double* const* eigvecs, int* info);
+
 /* Parameters of usbs() (solver/usbs.py:436-459), plus the execution choices. */
 typedef struct usbs_params {
   int32_t k_curr, k_past;

@@ -87,3 +87,24 @@ def test_krylov_breakdown_small_n(gpu_ctx):
     np.testing.assert_allclose(ev_g, dense[:4], rtol=1e-9, atol=1e-9)
     np.testing.assert_allclose(ev_o, dense[:4], rtol=1e-9, atol=1e-9)
     assert info["breakdowns"] >= 1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_equals_single(precision):
+    """Instances of different sizes advanced in one batched persistent launch
+    give bitwise the results of their own single-instance calls."""
+    from usbs_amd import _lib
+    from usbs_amd.lanczos import eigsh_smallest_batch
+    cases = [problems.torus(20, 30, seed=1), problems.erdos_renyi(300, 0.06, seed=3),
+             problems.torus(7, 9, seed=4)]
+    probs = []
+    for i, W in enumerate(cases):
+        negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 20 + i)
+        probs.append((negC.shape[0], negC, A_data, A_indices, y, q0))
+    print("case: batch of %d (n = %s) %s" % (len(probs), [p[0] for p in probs], precision), flush=True)
+    ctxs = [_lib.Context(0) for _ in probs]
+    batch = eigsh_smallest_batch(probs, 6, 24, 10, 1e-7, precision=precision, contexts=ctxs)
+    for (n, negC, A_data, A_indices, y, q0), (ev_b, V_b), ctx in zip(probs, batch, ctxs):
+        ev_s, V_s = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 6, 24, 10, 1e-7,
+                                   precision=precision, exec_mode="persistent", context=ctx)
+        assert np.array_equal(ev_b, ev_s) and np.array_equal(V_b, V_s)

@@ -298,6 +298,106 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
   return USBS_OK;
 }
 
+// Several instances (one per context, each with its own uploaded problem) in
+// one persistent launch per restart.  All contexts are on one device; the
+// launch runs on the first context's stream.  Workgroups: one per tile of
+// every instance (tiles cached in LDS) when they fit on the device together,
+// else a proportional share per instance.
+template <typename T>
+static int run_lanczos_batch(Ctx** cs, int ninst, double C_coef, const double* const* ys,
+                             const double* const* q0s, int nd, int m, int max_restarts, double tol,
+                             double* const* eigvals, double* const* eigvecs, int* info) {
+  Ctx* c0 = cs[0];
+  hipStream_t s = c0->stream;
+  std::vector<std::vector<DevBuf>> pools(ninst);
+  std::vector<DevBuf> pool0;
+  struct Guard {
+    std::vector<std::vector<DevBuf>>& p;
+    std::vector<DevBuf>& p0;
+    ~Guard() { for (auto& q : p) ctx_free_pool(q); ctx_free_pool(p0); }
+  } guard{pools, pool0};
+  std::vector<LzInst<T>> L(ninst);
+  int total_tiles = 0;
+  for (int q = 0; q < ninst; ++q) {
+    Ctx* c = cs[q];
+    const Problem& P = c->P;
+    double* d_y = upload(c, pools[q], ys[q], (size_t)P.m);
+    double* d_q0 = upload(c, pools[q], q0s[q], (size_t)P.n);
+    double* d_val64 = (double*)ctx_alloc(c, pools[q], sizeof(double) * P.nnz);
+    float* d_val32 = nullptr;
+    if (!d_y || !d_q0 || !d_val64) return USBS_ERR_OOM;
+    if (sizeof(T) == 4 && !(d_val32 = (float*)ctx_alloc(c, pools[q], sizeof(float) * P.nnz)))
+      return USBS_ERR_OOM;
+    if (launch_assemble(P, C_coef, d_y, 1.0, nullptr, 0.0, nullptr, 0.0, d_val64, d_val32, s))
+      return set_err(c0, USBS_ERR_HIP, "assemble launch failed");
+    int rc = make_lanczos<T>(c, pools[q], P, m, nd, max_restarts, tol,
+                             sizeof(T) == 4 ? (const T*)d_val32 : (const T*)d_val64, L[q]);
+    if (rc) return rc;
+    if (reset_lanczos(L[q], d_q0, s)) return set_err(c0, USBS_ERR_HIP, "reset launch failed");
+    total_tiles += L[q].ntiles;
+  }
+  // workgroups: try one per tile with the LDS cache, else shares without it
+  size_t cache = 0;
+  bool all_cache = true;
+  for (int q = 0; q < ninst; ++q) {
+    L[q].nblocks = L[q].ntiles;
+    const size_t b = lanczos_cache_bytes(L[q]);
+    if (b == 0) all_cache = false;
+    cache = std::max(cache, b);
+  }
+  if (!all_cache) cache = 0;
+  int cap = lanczos_max_coresident_blocks(cache ? cache : lanczos_tile_lds_bytes(m), c0->device);
+  if (cache && cap < total_tiles) {
+    cache = 0;
+    cap = lanczos_max_coresident_blocks(lanczos_tile_lds_bytes(m), c0->device);
+  }
+  if (cap < ninst) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
+  int total_wgs = 0;
+  for (int q = 0; q < ninst; ++q) {
+    int nb = L[q].ntiles;
+    if (total_tiles > cap) nb = std::max(1, (int)((long long)L[q].ntiles * cap / total_tiles));
+    L[q].nblocks = nb;
+    total_wgs += nb;
+  }
+  if (total_wgs > cap) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
+  std::vector<int2> wgmap, tilemap;
+  for (int q = 0; q < ninst; ++q) {
+    for (int i = 0; i < L[q].nblocks; ++i) wgmap.push_back(make_int2(q, i));
+    for (int i = 0; i < L[q].ntiles; ++i) tilemap.push_back(make_int2(q, i));
+  }
+  LzInst<T>* d_inst = (LzInst<T>*)ctx_alloc(c0, pool0, sizeof(LzInst<T>) * ninst);
+  int* d_wg = (int*)ctx_alloc(c0, pool0, sizeof(int2) * wgmap.size());
+  int* d_tm = (int*)ctx_alloc(c0, pool0, sizeof(int2) * tilemap.size());
+  if (!d_inst || !d_wg || !d_tm) return USBS_ERR_OOM;
+  USBS_HIP(c0, hipMemcpyAsync(d_inst, L.data(), sizeof(LzInst<T>) * ninst, hipMemcpyHostToDevice, s));
+  USBS_HIP(c0, hipMemcpyAsync(d_wg, wgmap.data(), sizeof(int2) * wgmap.size(), hipMemcpyHostToDevice, s));
+  USBS_HIP(c0, hipMemcpyAsync(d_tm, tilemap.data(), sizeof(int2) * tilemap.size(), hipMemcpyHostToDevice, s));
+  int rc = lanczos_persistent<T>(d_inst, ninst, total_wgs, m, max_restarts, s, d_wg, d_tm,
+                                 total_tiles, 1ull << 26, cache);
+  if (rc) return set_err(c0, USBS_ERR_HIP, std::string("lanczos batch: ") + hipGetErrorString((hipError_t)rc));
+  for (int q = 0; q < ninst; ++q) {
+    const Problem& P = cs[q]->P;
+    double* d_vecs = (double*)ctx_alloc(c0, pool0, sizeof(double) * (size_t)P.n * nd);
+    if (!d_vecs) return USBS_ERR_OOM;
+    const long long tot = (long long)P.n * nd;
+    k_copy_cols_out<T><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(L[q].Q, L[q].ldq, P.n, nd, d_vecs);
+    USBS_HIP(c0, hipGetLastError());
+    int hc[LZ_NCTRL];
+    USBS_HIP(c0, hipMemcpyAsync(eigvals[q], L[q].alpha, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+    USBS_HIP(c0, hipMemcpyAsync(eigvecs[q], d_vecs, sizeof(double) * tot, hipMemcpyDeviceToHost, s));
+    USBS_HIP(c0, hipMemcpyAsync(hc, L[q].ctrl, sizeof(hc), hipMemcpyDeviceToHost, s));
+    USBS_HIP(c0, hipStreamSynchronize(s));
+    if (hc[LZ_ERROR]) return set_err(c0, USBS_ERR_TIMEOUT, "persistent Lanczos barrier wait expired");
+    if (info) {
+      info[4 * q + 0] = hc[LZ_RESTART];
+      info[4 * q + 1] = hc[LZ_NCONV];
+      info[4 * q + 2] = hc[LZ_STEPS];
+      info[4 * q + 3] = hc[LZ_NBREAK];
+    }
+  }
+  return USBS_OK;
+}
+
 }  // namespace usbs
 
 using namespace usbs;
@@ -376,6 +476,40 @@ int usbs_lanczos_smallest(usbs_ctx* h, double C_coef, const double* y, const dou
                                           exec_mode, eigvals, eigvecs, info);
   return run_lanczos_standalone<float>(c, C_coef, y, q0, num_desired, m, max_restarts, tolerance,
                                        exec_mode, eigvals, eigvecs, info);
+}
+
+int usbs_lanczos_smallest_batch(usbs_ctx* const* ctxs, int ninst, double C_coef,
+                                const double* const* adjoint_left_vecs, const double* const* q0s,
+                                int num_desired, int inner_iterations, int max_restarts,
+                                double tolerance, int precision, double* const* eigvals,
+                                double* const* eigvecs, int* info) {
+  if (!ctxs || ninst < 1 || !ctxs[0] || !adjoint_left_vecs || !q0s || !eigvals || !eigvecs)
+    return USBS_ERR_INVALID;
+  Ctx* c0 = &ctxs[0]->c;
+  std::vector<Ctx*> cs(ninst);
+  for (int q = 0; q < ninst; ++q) {
+    if (!ctxs[q] || !adjoint_left_vecs[q] || !q0s[q] || !eigvals[q] || !eigvecs[q])
+      return set_err(c0, USBS_ERR_INVALID, "null argument in batch entry " + std::to_string(q));
+    cs[q] = &ctxs[q]->c;
+    if (!cs[q]->has_problem) return set_err(c0, USBS_ERR_STATE, "batch entry without a problem");
+    if (cs[q]->device != c0->device) return set_err(c0, USBS_ERR_INVALID, "batch spans devices");
+    const int n = cs[q]->P.n;
+    if (num_desired < 1 || inner_iterations < num_desired || inner_iterations > 48 ||
+        inner_iterations > n || max_restarts < 1)
+      return set_err(c0, USBS_ERR_INVALID, "need 1 <= num_desired <= inner_iterations <= min(n, 48)");
+    for (int i = 0; i < cs[q]->P.m; ++i)
+      if (!std::isfinite(adjoint_left_vecs[q][i])) return set_err(c0, USBS_ERR_NONFINITE, "non-finite adjoint_left_vec");
+  }
+  if (precision != USBS_PREC_FP64 && precision != USBS_PREC_MIXED)
+    return set_err(c0, USBS_ERR_INVALID, "unknown precision");
+  if (hipSetDevice(c0->device) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "hipSetDevice");
+  for (int q = 1; q < ninst; ++q)  // work queued on other contexts' streams finishes first
+    if (hipStreamSynchronize(cs[q]->stream) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "synchronise");
+  if (precision == USBS_PREC_FP64)
+    return run_lanczos_batch<double>(cs.data(), ninst, C_coef, adjoint_left_vecs, q0s, num_desired,
+                                     inner_iterations, max_restarts, tolerance, eigvals, eigvecs, info);
+  return run_lanczos_batch<float>(cs.data(), ninst, C_coef, adjoint_left_vecs, q0s, num_desired,
+                                  inner_iterations, max_restarts, tolerance, eigvals, eigvecs, info);
 }
 
 }  // extern "C"

@@ -644,6 +644,8 @@ int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int
   return (int)hipErrorInvalidValue;
 }
 
+size_t lanczos_tile_lds_bytes(int m) { return tile_lds_bytes(m + 1); }
+
 int lanczos_max_coresident_blocks(size_t lds_bytes, int device) {
   int per_cu = 0, per_cu64 = 0, per_cu_c = 0, per_cu64_c = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, false>, USBS_NT,

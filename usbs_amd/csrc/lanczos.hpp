@@ -75,6 +75,8 @@ int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cach
 template <typename T>
 size_t lanczos_cache_bytes(const LzInst<T>& L);
 int lanczos_max_coresident_blocks(size_t lds_bytes, int device);
+// LDS of a persistent workgroup that does not cache its tile (basis size m).
+size_t lanczos_tile_lds_bytes(int m);
 template <typename T>
 int orth_perop(const LzInst<T>& L, hipStream_t s);
 

@@ -5,6 +5,8 @@ the operator C + A*(adjoint_left_vec) by thick-restart Lanczos with full
 CGS2 reorthogonalisation.  The trailing keyword arguments select the backend
 (only "hip" exists here) and the execution form.
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -50,3 +52,50 @@ def eigsh_smallest(n, C, A_data, A_indices, adjoint_left_vec, q0, num_desired, i
         return vals_out, vecs_out, dict(restarts=int(info[0]), num_converged=int(info[1]),
                                         steps=int(info[2]), breakdowns=int(info[3]))
     return vals_out, vecs_out
+
+
+_PP = ctypes.POINTER(_lib._dp)
+_lib.EXTRA_SIGNATURES["usbs_lanczos_smallest_batch"] = [
+    ctypes.POINTER(_lib._vp), ctypes.c_int, ctypes.c_double, _PP, _PP, ctypes.c_int, ctypes.c_int,
+    ctypes.c_int, ctypes.c_double, ctypes.c_int, _PP, _PP, _lib._ip]
+
+
+def eigsh_smallest_batch(problems, num_desired, inner_iterations, max_restarts, tolerance,
+                         precision="fp64", contexts=None, device=0, return_info=False):
+    """Batched eigsh_smallest: ``problems`` is a list of (n, C, A_data,
+    A_indices, adjoint_left_vec, q0) tuples of possibly different sizes; all
+    instances advance together in one persistent launch per restart.  Each
+    result is bitwise the one eigsh_smallest(..., exec_mode="persistent")
+    returns for that instance alone.  Returns a list of (eigenvalues,
+    eigenvectors) pairs (and a list of info dicts with return_info)."""
+    k = len(problems)
+    ctxs = contexts if contexts is not None else [_lib.Context(device) for _ in range(k)]
+    zs, qs, outs_v, outs_e = [], [], [], []
+    for (n, C, A_data, A_indices, z, q0), ctx in zip(problems, ctxs):
+        rows, cols, vals, shape = coo_parts(C)
+        if shape != (n, n):
+            raise ValueError("C has shape %s, expected (%d, %d)" % (shape, n, n))
+        z = _lib.as_f64(z)
+        m = z.size
+        ctx.set_problem(n, m, rows, cols, vals, np.asarray(A_indices).reshape(-1, 3), A_data,
+                        np.zeros(m), np.zeros(m), np.ones(m))
+        q0 = _lib.as_f64(q0)
+        if q0.size != n:
+            raise ValueError("q0 must have n entries")
+        zs.append(z)
+        qs.append(q0)
+        outs_v.append(np.zeros(num_desired))
+        outs_e.append(np.zeros((n, num_desired)))
+    info = np.zeros(4 * k, dtype=np.int32)
+    handles = (_lib._vp * k)(*[c.h for c in ctxs])
+    arr = lambda xs: (_lib._dp * k)(*[_lib.dptr(x) for x in xs])  # noqa: E731
+    ctxs[0].check(_lib.lib().usbs_lanczos_smallest_batch(
+        handles, k, 1.0, arr(zs), arr(qs), int(num_desired), int(inner_iterations),
+        int(max_restarts), float(tolerance), _lib.PRECISION[precision], arr(outs_v), arr(outs_e),
+        _lib.iptr(info)))
+    res = list(zip(outs_v, outs_e))
+    if return_info:
+        infos = [dict(restarts=int(info[4 * i]), num_converged=int(info[4 * i + 1]),
+                      steps=int(info[4 * i + 2]), breakdowns=int(info[4 * i + 3])) for i in range(k)]
+        return res, infos
+    return res
