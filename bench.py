@@ -40,9 +40,19 @@ SOLVER = dict(trace_factor=2.0, rho=0.01, beta=0.25, k_curr=10, k_past=1, lanczo
               lanczos_restarts=10, subprob_eps=1e-7, subprob_max_iters=100, sketch_dim=10)
 
 
-def workload():
+WORKLOADS = {
+    "torus": "max-cut SDP, torus 100x200 (n=20000, 40000 edges, w in {-1,+1}, seed 0)",
+    "rgg": "max-cut SDP, random geometric graph n=2^20 in the unit square, radius 1.3504e-3 "
+           "(mean degree 6), unit weights, seed 0",
+}
+
+
+def workload(name="torus"):
     from usbs_amd import problems
-    W = problems.torus(100, 200, seed=0, signed=True)
+    if name == "rgg":
+        W = problems.random_geometric(1 << 20, 1.3504e-3, 0)
+    else:
+        W = problems.torus(100, 200, seed=0, signed=True)
     n = W.shape[0]
     Om = problems.gaussian_matrix(n, SOLVER["sketch_dim"], 1)
     q0 = problems.unit_start_vector(n, 2)
@@ -104,12 +114,12 @@ def roofline(stats):
             "steps_per_launch": round(steps / L, 3), "algorithmic_bytes_per_launch": int(per_launch)}
 
 
-def cpu_baseline(iters):
+def cpu_baseline(iters, name="torus"):
     """The FP64 CPU restatement (oracle) on the same workload, `iters`
     iterations including the initial Lanczos call, BLAS threads capped."""
     import threadpoolctl
     from oracle import maxcut as omc, usbs as ousbs
-    W, n, Om, q0 = workload()
+    W, n, Om, q0 = workload(name)
     cores = min(16, len(os.sched_getaffinity(0)))
     s = SOLVER
     with threadpoolctl.threadpool_limits(limits=cores):
@@ -123,8 +133,43 @@ def cpu_baseline(iters):
         el = time.perf_counter() - t0
     return {"value": round(iters / el, 4), "unit": "USBS iterations/s", "cores": cores,
             "kind": "port",
-            "sample": "FP64 NumPy/SciPy restatement, torus 100x200, %d iterations incl. the "
-                      "initial Lanczos call (%.1f s), BLAS threads %d" % (iters, el, cores)}
+            "sample": "FP64 NumPy/SciPy restatement, %s, %d iterations incl. the initial "
+                      "Lanczos call (%.1f s), BLAS threads %d" % (name, iters, el, cores)}
+
+
+def serving_bench(args):
+    """The serving workload: Poisson arrivals, sizes log-uniform in
+    [1e3, 1e5], ER mean degree 10 with +-1 weights, 30 % follow-ups flipping
+    1 % of an earlier graph's edge signs (warm-started)."""
+    from usbs_amd import serving
+    reqs = serving.request_stream(args.requests, args.rate, 1000, 100000, 10.0, 0.3, 0.01,
+                                  seed=0)
+    srv = serving.SolveServer(slots=args.slots, precision=args.precision, exec_mode="perop",
+                              max_iters=args.solve_iters)
+    # warm-up: one small solve per slot (module load, first kernel launches)
+    warm = serving.request_stream(args.slots, 1e6, 1000, 1001, seed=1)
+    srv.run(warm, realtime=False)
+    srv.held = [-1] * args.slots
+    results, summ = srv.run(reqs, realtime=True)
+    out = {"metric": METRIC, "value": round(summ["solves_per_s"], 4), "unit": "solves/s",
+           "n_gpus": 1, "steps": len(results), "warmup": args.slots,
+           "ms_per_step": round(1e3 * summ["makespan_s"] / max(1, len(results)), 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 storage / f64 accumulate" if args.precision == "mixed" else "f64",
+           "data": "synthetic (seeded Erdos-Renyi graphs, seeded arrivals)",
+           "config": {"workload": "serving stream: Poisson %.2f/s, n log-uniform 1e3..1e5, ER "
+                                  "degree 10, +-1 weights, 30%% follow-ups with 1%% sign flips"
+                                  % args.rate,
+                      "requests": len(results), "slots": args.slots,
+                      "iterations_per_solve": args.solve_iters, "k_curr": 4, "k_past": 1,
+                      "sketch_dim": 10},
+           "p50_latency_s": round(summ["p50_latency_s"], 4),
+           "p99_latency_s": round(summ["p99_latency_s"], 4),
+           "warm_starts": {"device": summ["warm_device"], "host": summ["warm_host"]},
+           "sizes": [r.n for r in results],
+           "latencies_s": [round(r.latency, 4) for r in results],
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -135,6 +180,12 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--exec-mode", default="persistent", choices=["persistent", "perop"])
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--workload", default="torus", choices=["torus", "rgg", "serving"],
+                    help="torus 100x200 (default), RGG 2^20, or the serving stream")
+    ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent solver slots")
+    ap.add_argument("--solve-iters", type=int, default=20, help="serving: USBS iterations per solve")
     ap.add_argument("--cpu-iters", type=int, default=15, help="oracle iterations (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 20 device-to-device copies of MIB MiB (the reference kernel "
@@ -157,6 +208,10 @@ def main():
               % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serving":
+        serving_bench(args)
+        return
+
     from usbs_amd import _lib, maxcut_helpers
     from usbs_amd.usbs import upload_problem
 
@@ -166,7 +221,7 @@ def main():
     except ImportError:
         sync = lambda: None  # noqa: E731
 
-    W, n, Om, q0 = workload()
+    W, n, Om, q0 = workload(args.workload)
     ctx = _lib.Context(0)
     state = maxcut_helpers.initialize_state(W, SOLVER["sketch_dim"], Omega=Om)
     upload_problem(ctx, state, n, n)
@@ -183,9 +238,8 @@ def main():
            "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": round(ms, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 storage / f64 accumulate" if args.precision == "mixed" else "f64",
-           "data": "synthetic (seeded torus graph, seeded Omega and q0)",
-           "config": {"workload": "max-cut SDP, torus 100x200 (n=20000, 40000 edges, "
-                                  "w in {-1,+1}, seed 0)",
+           "data": "synthetic (seeded graph, seeded Omega and q0)",
+           "config": {"workload": WORKLOADS[args.workload], "n": n,
                       "precision": args.precision, "exec_mode": args.exec_mode,
                       "step": "one USBS iteration", **SOLVER}}
     last = log[-1]
@@ -203,7 +257,8 @@ def main():
                              "primal_rel_diff_end": abs(log64[-1]["primal_obj"] - last["primal_obj"])
                              / max(1.0, abs(log64[-1]["primal_obj"]))}
     out["roofline"] = roofline(stats) if stats else None
-    out["cpu_baseline"] = cpu_baseline(args.cpu_iters) if args.cpu_iters > 0 else None
+    out["cpu_baseline"] = (cpu_baseline(args.cpu_iters, args.workload) if args.cpu_iters > 0
+                           else None)
     print(json.dumps(out), flush=True)
 
 

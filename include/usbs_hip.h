@@ -49,6 +49,9 @@ void usbs_destroy(usbs_ctx* ctx);
 const char* usbs_last_error(const usbs_ctx* ctx);
 /* Library version string. */
 const char* usbs_version(void);
+/* Free and total bytes of the context's device (for admission by footprint;
+ * no counterpart in the reference). */
+int usbs_device_memory(usbs_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 /*
  * Upload an SDP in the reference's SDPState layout (utils/common.py:11-26):
@@ -102,6 +105,9 @@ typedef struct usbs_params {
   int32_t rounding;    /* 1: max-cut rounding at every condition check (needs r > 0) */
   int32_t precision;   /* USBS_PREC_*                                               */
   int32_t exec_mode;   /* USBS_EXEC_* for the Lanczos calls                          */
+  int32_t warm;        /* 1: start from y, z and P the context kept from its last    */
+                       /*    usbs_solve (device-resident; y0, z0, P0 are ignored)     */
+  int32_t reserved;
   double trace_ub, trace_factor, rho, beta;
   double max_time, obj_gap_eps, infeas_gap_eps, max_infeas_eps;
   double subprob_eps, cond_exp_base;
@@ -118,7 +124,12 @@ typedef struct usbs_record {
 
 /*
  * The whole USBS loop on the device (solver/usbs.py:436-789): cold or warm
- * start from (y0, z0, tr_X0, primal_obj0, P0), Omega the sketch test matrix
+ * start from (y0, z0, tr_X0, primal_obj0, P0), or, with params->warm, from the
+ * y, z and P that this context kept on the device at the end of its previous
+ * solve (same n, m and sketch size; the problem's values may have changed, as
+ * in utils/maxcut_helpers.py:95-161 where y, z, P carry over and only the
+ * scalars tr_X0, primal_obj0 are recomputed by the caller).  Omega the sketch
+ * test matrix
  * (n x r row-major, NULL for diag-only), q0 the Lanczos start vector
  * (solver/usbs.py:691-692).  Outputs y_out (m), z_out (m), P_out (n x r or n),
  * scal_out = [tr_X, primal_obj, iterations, obj_gap, infeas_gap, max_infeas].

@@ -53,6 +53,7 @@ def _declare(lib):
                                           ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
     lib.usbs_set_timing.argtypes = [_vp, ctypes.c_int]
+    lib.usbs_device_memory.argtypes = [_vp, _i64p, _i64p]
     lib.usbs_get_stats.argtypes = [_vp, ctypes.POINTER(usbs_stats)]
     for name, args in EXTRA_SIGNATURES.items():
         if hasattr(lib, name):
@@ -125,6 +126,12 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def device_memory(self):
+        """(free, total) bytes of the context's device."""
+        free, total = ctypes.c_int64(0), ctypes.c_int64(0)
+        self.check(self._lib.usbs_device_memory(self.h, ctypes.byref(free), ctypes.byref(total)))
+        return free.value, total.value
 
     def set_timing(self, enable):
         """Bracket persistent Lanczos launches with HIP events (resets counters)."""

@@ -123,6 +123,12 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
 
   ctx_free_pool(c->prob_bufs);
   ctx_free_pool(c->work_bufs);
+  if (c->keep_n != (int)n || c->keep_m != (int)m) {
+    ctx_free_pool(c->keep_bufs);
+    c->keep_y = c->keep_z = c->keep_P = nullptr;
+    c->keep_n = c->keep_m = 0;
+    c->keep_r = -1;
+  }
   c->has_problem = false;
   Problem& P = c->P;
   P = Problem();
@@ -430,8 +436,19 @@ void usbs_destroy(usbs_ctx* h) {
   (void)hipStreamSynchronize(h->c.stream);
   ctx_free_pool(h->c.work_bufs);
   ctx_free_pool(h->c.prob_bufs);
+  ctx_free_pool(h->c.keep_bufs);
   (void)hipStreamDestroy(h->c.stream);
   delete h;
+}
+
+int usbs_device_memory(usbs_ctx* h, int64_t* free_bytes, int64_t* total_bytes) {
+  if (!h || !free_bytes || !total_bytes) return USBS_ERR_INVALID;
+  if (hipSetDevice(h->c.device) != hipSuccess) return set_err(&h->c, USBS_ERR_HIP, "hipSetDevice");
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return set_err(&h->c, USBS_ERR_HIP, "hipMemGetInfo");
+  *free_bytes = (int64_t)f;
+  *total_bytes = (int64_t)t;
+  return USBS_OK;
 }
 
 const char* usbs_last_error(const usbs_ctx* h) { return h ? h->c.err.c_str() : "null context"; }

@@ -57,6 +57,12 @@ struct Ctx {
   std::vector<DevBuf> work_bufs;  // solve workspaces
   int persist_wgs = 0;            // workgroups per persistent Lanczos instance (0 = one per tile)
   bool timing = false;            // bracket persistent Lanczos launches with events
+  // y, z, P (column-major n x r, ld = keep_ld) of the last usbs_solve, kept
+  // on the device for warm starts; survive usbs_set_problem while n, m agree
+  std::vector<DevBuf> keep_bufs;
+  double *keep_y = nullptr, *keep_z = nullptr, *keep_P = nullptr;
+  int keep_n = 0, keep_m = 0, keep_r = -1;
+  long long keep_ld = 0;
   usbs_stats stats{};
 };
 

@@ -380,20 +380,37 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
     if (dptr && hipMemcpy(dptr, h, sizeof(double) * cnt, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     return dptr;
   };
-  if (hipMemcpy(S.y, y0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(S.z, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(S.zbar, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(S.q0, q0h, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+  const int rk = r > 0 ? r : 1;  // columns of P (the diag-only P is one column)
+  const bool warm = p.warm != 0;
+  if (warm) {
+    if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld)
+      return set_err(c, USBS_ERR_STATE, "warm start needs a previous solve of the same size on this context");
+    if (hipMemcpyAsync(S.y, c->keep_y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(S.z, c->keep_z, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(S.zbar, c->keep_z, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "copy of the kept start point");
+  } else if (hipMemcpy(S.y, y0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(S.z, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(S.zbar, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess) {
     return set_err(c, USBS_ERR_HIP, "upload of the start point");
+  }
+  if (hipMemcpy(S.q0, q0h, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(c, USBS_ERR_HIP, "upload of q0");
   S.fill(S.ups, m, 0.0);
   S.fill_T(S.Qa, ld * (S.k + 1));
   S.fill_T(S.Qb, ld * (S.k + 1));
   S.fill_T(S.PO, ld * 2 * (r > 0 ? r : 1));
+  if (warm) {
+    const long long cnt = ld * rk;
+    k_cast<double, T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.P_());
+    k_cast<double, T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.Pbar);
+  }
   if (r > 0) {
     double* d_om = up(Omega, (size_t)n * r);
     if (!d_om) return USBS_ERR_OOM;
     k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_om, S.Om(), ld);
-    if (P0) {
+    if (warm) {
+    } else if (P0) {
       double* d_p = up(P0, (size_t)n * r);
       if (!d_p) return USBS_ERR_OOM;
       k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_p, S.P_(), ld);
@@ -404,7 +421,8 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
     if (launch_tsgram<T, T>(n, c->P.tile, c->P.ntiles, S.Om(), ld, r, S.Om(), ld, r, S.partial, S.OtO, s))
       return set_err(c, USBS_ERR_HIP, "Omega Gram");
   } else {
-    if (P0) {
+    if (warm) {
+    } else if (P0) {
       double* d_p = up(P0, (size_t)n);
       if (!d_p) return USBS_ERR_OOM;
       k_load_colmajor<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, 1, d_p, S.P_(), ld);
@@ -505,6 +523,31 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
       hipMemcpyAsync(hs.data(), S.sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return set_err(c, USBS_ERR_HIP, "reading results");
+  // keep y, z, P on the device for a warm start of the next solve
+  if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld) {
+    ctx_free_pool(c->keep_bufs);
+    c->keep_y = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
+    c->keep_z = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
+    c->keep_P = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * ld * rk);
+    if (!c->keep_y || !c->keep_z || !c->keep_P) {
+      ctx_free_pool(c->keep_bufs);
+      c->keep_n = c->keep_m = 0;
+      c->keep_r = -1;
+      return USBS_ERR_OOM;
+    }
+    c->keep_n = n;
+    c->keep_m = m;
+    c->keep_r = rk;
+    c->keep_ld = ld;
+  }
+  {
+    const long long cnt = ld * rk;
+    k_cast<T, double><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, S.P_(), c->keep_P);
+    if (hipMemcpyAsync(c->keep_y, S.y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(c->keep_z, S.z, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "keeping the final state");
+  }
   if (scal_out) {
     scal_out[0] = hs[SC_TRX];
     scal_out[1] = hs[SC_P];
@@ -541,7 +584,7 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   Ctx* c = &h->c;
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
   const usbs_params& p = *prm;
-  if (!y0 || !z0 || !q0) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
+  if (!q0 || (!p.warm && (!y0 || !z0))) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   const int k = p.k_curr + p.k_past;
   if (p.k_curr < 1 || p.k_past < 0 || k > 11)
     return set_err(c, USBS_ERR_INVALID, "need k_curr >= 1, k_past >= 0, k_curr + k_past <= 11");
@@ -561,8 +604,9 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
     return set_err(c, USBS_ERR_INVALID, "unknown precision");
   if (p.exec_mode != USBS_EXEC_PEROP && p.exec_mode != USBS_EXEC_PERSISTENT)
     return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
-  for (int i = 0; i < c->P.m; ++i)
-    if (!std::isfinite(y0[i]) || !std::isfinite(z0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite y0/z0");
+  if (!p.warm)
+    for (int i = 0; i < c->P.m; ++i)
+      if (!std::isfinite(y0[i]) || !std::isfinite(z0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite y0/z0");
   if (hipSetDevice(c->device) != hipSuccess) return set_err(c, USBS_ERR_HIP, "hipSetDevice");
   if (p.precision == USBS_PREC_FP64)
     return run_solve<double>(c, p, y0, z0, tr_X0, primal_obj0, Omega, P0, q0, y_out, z_out, P_out,

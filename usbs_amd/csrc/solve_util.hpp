@@ -29,6 +29,12 @@ __global__ void k_fill(T* p, long long count, T v) {
   if (i < count) p[i] = v;
 }
 
+template <typename TS, typename TD>
+__global__ void k_cast(long long count, const TS* __restrict__ src, TD* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = (TD)src[i];
+}
+
 // Lanczos per-call reset: Q[:, 0] = q0, alpha = beta = 0, control and barrier 0.
 template <typename T>
 __global__ void k_lz_start(LzInst<T> L, const double* q0) {

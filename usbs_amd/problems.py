@@ -115,3 +115,28 @@ def unit_start_vector(n, seed):
     """Normalised standard normal start vector q0 (cf. solver/usbs.py:691-692)."""
     q = _rng(seed).standard_normal(n)
     return q / np.linalg.norm(q)
+
+
+def flip_edge_signs(W, frac, seed):
+    """Copy of the symmetric weighted adjacency W with the signs of a random
+    ``frac`` share of its edges flipped (both (i, j) and (j, i))."""
+    import scipy.sparse
+    U = scipy.sparse.triu(W, k=1).tocoo()
+    rng = _rng(seed)
+    flip = rng.random(U.nnz) < frac
+    data = np.where(flip, -U.data, U.data)
+    T = scipy.sparse.coo_matrix((data, (U.row, U.col)), shape=W.shape)
+    return (T + T.T).tocsr()
+
+
+def random_geometric(n, radius, seed):
+    """Random geometric graph: n points uniform in the unit square (seeded),
+    an edge between every pair closer than ``radius``, unit weights.  Pairs
+    come from scipy's k-d tree, so n = 2^20 takes seconds."""
+    import scipy.sparse
+    import scipy.spatial
+    pts = _rng(seed).random((n, 2))
+    pairs = scipy.spatial.cKDTree(pts).query_pairs(radius, output_type="ndarray")
+    i, j = pairs[:, 0], pairs[:, 1]
+    W = scipy.sparse.coo_matrix((np.ones(i.size), (i, j)), shape=(n, n))
+    return (W + W.T).tocsr()

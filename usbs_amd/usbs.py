@@ -28,7 +28,7 @@ class usbs_params(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int32) for name in
                 ("k_curr", "k_past", "max_iters", "lanczos_inner_iterations",
                  "lanczos_max_restarts", "subprob_max_iters", "sketch_dim", "rounding",
-                 "precision", "exec_mode")] + \
+                 "precision", "exec_mode", "warm", "reserved")] + \
                [(name, ctypes.c_double) for name in
                 ("trace_ub", "trace_factor", "rho", "beta", "max_time", "obj_gap_eps",
                  "infeas_gap_eps", "max_infeas_eps", "subprob_eps", "cond_exp_base", "SCALE_C",
@@ -64,8 +64,14 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
          obj_gap_eps, infeas_gap_eps, max_infeas_eps, lanczos_inner_iterations,
          lanczos_max_restarts, subprob_eps, subprob_max_iters, cond_exp_base, callback_fn,
          callback_static_args, callback_nonstatic_args, backend="hip", precision="fp64",
-         exec_mode="perop", q0=None, log=None, device=0, context=None, upload=True):
-    """Run USBS on the GPU and return the final SDPState (scaled units)."""
+         exec_mode="perop", q0=None, log=None, device=0, context=None, upload=True,
+         warm_start=False):
+    """Run USBS on the GPU and return the final SDPState (scaled units).
+
+    warm_start=True starts from the y, z and sketch P that ``context`` kept on
+    the device from its previous solve (same sizes), instead of the state's
+    y, z, P; tr_X and primal_obj still come from ``sdp_state`` (the scalars a
+    warm-start transform recomputes, utils/maxcut_helpers.py:95-161)."""
     if backend != "hip":
         raise ValueError("backend=%r is not available in usbs_amd (only 'hip')" % (backend,))
     from .maxcut_helpers import compute_max_cut
@@ -84,7 +90,8 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
                       lanczos_inner_iterations=lanczos_inner_iterations,
                       lanczos_max_restarts=lanczos_max_restarts, subprob_max_iters=subprob_max_iters,
                       sketch_dim=r, rounding=rounding, precision=_lib.PRECISION[precision],
-                      exec_mode=_lib.EXEC_MODE[exec_mode], trace_ub=trace_ub,
+                      exec_mode=_lib.EXEC_MODE[exec_mode], warm=1 if warm_start else 0,
+                      reserved=0, trace_ub=trace_ub,
                       trace_factor=trace_factor, rho=rho, beta=beta,
                       max_time=_finite(max_time, 1e300), obj_gap_eps=_finite(obj_gap_eps, -1e300),
                       infeas_gap_eps=_finite(infeas_gap_eps, -1e300),
