@@ -108,3 +108,19 @@ def test_batched_equals_single(precision):
         ev_s, V_s = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 6, 24, 10, 1e-7,
                                    precision=precision, exec_mode="persistent", context=ctx)
         assert np.array_equal(ev_b, ev_s) and np.array_equal(V_b, V_s)
+
+
+def test_lanczos_large_tiles_match_oracle(gpu_ctx):
+    """n = 90 000 uses 512-row tiles, whose reductions run row-wise: Ritz
+    values match the oracle to 1e-9 and per-op equals persistent bitwise."""
+    W = problems.torus(300, 300, seed=5)
+    negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 13)
+    n = negC.shape[0]
+    print("case: torus 300x300 (n=%d) large tiles" % n, flush=True)
+    ev_o, _ = olz.eigsh_smallest(n, negC, A_data, A_indices, y, q0, 6, 24, 4, 1e-7)
+    a = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 6, 24, 4, 1e-7, exec_mode="perop",
+                       context=gpu_ctx)
+    b = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 6, 24, 4, 1e-7,
+                       exec_mode="persistent", context=gpu_ctx)
+    np.testing.assert_allclose(a[0], ev_o, rtol=1e-9, atol=1e-9 * np.abs(ev_o).max())
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

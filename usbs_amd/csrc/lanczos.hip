@@ -37,12 +37,25 @@ int tile_rows_for(int n) {
 }
 
 #define LZ_RED (USBS_NT + 64)  // doubles of reduction scratch in every tile kernel
+// Tiles of more than 256 rows reduce row-wise in registers and transpose
+// through LDS (block_reduce_cols): that scratch is larger.
+#define LZ_BIG (USBS_RCH * USBS_NT + USBS_RCH * USBS_NPART + 64)
 
 static inline size_t ritz_lds_bytes(int m) {
   const int M = m + (m & 1);
   return sizeof(double) * (size_t)(2 * M * (M + 1) + m * m + 2 * M + 16 + (8 + M + (M - 1) * M) / 2 + 16);
 }
-static inline size_t tile_lds_bytes(int maxc) { return sizeof(double) * (size_t)(LZ_RED + maxc + 8); }
+static inline size_t tile_lds_bytes(int maxc) { return sizeof(double) * (size_t)(LZ_BIG + maxc + 8); }
+
+// Row-wise reduction (tiles of more than 256 rows, where the Q block streams
+// from HBM): each thread owns whole rows, so every Q value is read once per
+// phase, coalesced, and the per-thread sums are transposed through LDS.
+template <typename T>
+__device__ __forceinline__ bool lz_rowwise(const LzInst<T>& L) { return L.tile > 256; }
+template <typename T>
+__device__ __forceinline__ double* lz_hbuf(const LzInst<T>& L, double* red) {
+  return red + (lz_rowwise(L) ? LZ_BIG : LZ_RED);
+}
 static inline size_t rot_lds_bytes(int m) { return sizeof(double) * (size_t)(m * m + 8); }
 
 // ----------------------------------------------------------------------------
@@ -244,7 +257,7 @@ struct TileCtx {
 
 // Phase A (mode 0: SpMV; mode 2: y = Q[:, i+1] for the column-by-column QR;
 // mode 3: only normalise w into column i).  first: q_i is already in Q.
-template <typename T, bool C>
+template <typename T, bool C, int MAXC>
 __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, bool first,
                        int mode, double* red, double beta_known = -1.0) {
   const int t = threadIdx.x;
@@ -263,6 +276,13 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
   if (mode == 3) { __syncthreads(); return; }
   T* yb;
   if constexpr (C) yb = X.yc; else yb = wc + X.r0;
+  const T* qb;
+  long long qld;
+  if constexpr (C) { qb = X.qc; qld = X.ldc; } else { qb = L.Q + X.r0; qld = L.ldq; }
+  const bool rowwise = lz_rowwise(L);
+  double acc[MAXC];
+#pragma unroll
+  for (int j = 0; j < MAXC; ++j) acc[j] = 0.0;
   const T* xq = L.Q + (long long)i * L.ldq;
   for (int rl = t; rl < X.R; rl += USBS_NT) {
     double s = 0.0;
@@ -286,21 +306,34 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
     } else {
       s = (double)L.Q[(long long)(i + 1) * L.ldq + X.r0 + rl];
     }
-    yb[rl] = (T)s;
+    const T yv = (T)s;
+    yb[rl] = yv;
+    if (rowwise) {
+      // loads of columns past i read column i (in bounds) and are masked, so
+      // every load of the row is issued without a branch
+      const double yd = (double)yv;
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) {
+        const double qv = (double)qb[(long long)(j <= i ? j : i) * qld + rl];
+        acc[j] += (j <= i) ? qv * yd : 0.0;
+      }
+    }
+  }
+  if (rowwise) {
+    block_reduce_cols<MAXC>(acc, i + 1, red, red + USBS_RCH * USBS_NT,
+                            L.part1 + (long long)tl * L.pstride);
+    return;
   }
   __syncthreads();
-  const T* qb;
-  long long qld;
-  if constexpr (C) { qb = X.qc; qld = X.ldc; } else { qb = L.Q + X.r0; qld = L.ldq; }
   col_dots(qb, qld, yb, X.R, i + 1, red, L.part1 + (long long)tl * L.pstride);
 }
 
 // Phases B (final = 0) and C (final = 1).
-template <typename T, bool C>
+template <typename T, bool C, int MAXC>
 __device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, int final_pass,
                         double* red) {
   const int t = threadIdx.x;
-  double* h = red + LZ_RED;
+  double* h = lz_hbuf(L, red);
   const double* src = final_pass ? L.part2 : L.part1;
   sum_tile_partials(src, L.ntiles, L.pstride, i + 1, h, red);
   if (tl == 0)
@@ -310,17 +343,46 @@ __device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int 
   long long qld;
   T* wc = L.w + (long long)(i & 1) * L.ldq + X.r0;
   if constexpr (C) { yb = X.yc; qb = X.qc; qld = X.ldc; } else { yb = wc; qb = L.Q + X.r0; qld = L.ldq; }
-  row_update(qb, qld, yb, X.R, i + 1, h);
-  if (!final_pass) {
-    col_dots(qb, qld, yb, X.R, i + 1, red, L.part2 + (long long)tl * L.pstride);
-  } else {
-    if constexpr (C)
-      for (int rl = t; rl < X.R; rl += USBS_NT) wc[rl] = yb[rl];
-    self_dot(yb, X.R, red, L.part3 + (long long)tl * L.pstride);
-    if (tl == 0 && t == 0) {
-      L.alpha[i] = L.h1[i] + h[i];
-      L.ctrl[LZ_STEPS] += 1;
+  double* part_out = (final_pass ? L.part3 : L.part2) + (long long)tl * L.pstride;
+  if (lz_rowwise(L)) {
+    // one pass over the tile's Q rows: update, then the next dots
+    double acc[MAXC];
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) acc[j] = 0.0;
+    for (int rl = t; rl < X.R; rl += USBS_NT) {
+      double q[MAXC];
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) {
+        const double qv = (double)qb[(long long)(j <= i ? j : i) * qld + rl];
+        q[j] = (j <= i) ? qv : 0.0;
+        s += q[j] * ((j <= i) ? h[j] : 0.0);
+      }
+      const T yv = (T)((double)yb[rl] - s);
+      yb[rl] = yv;
+      const double yd = (double)yv;
+      if (final_pass) {
+        acc[0] += yd * yd;
+      } else {
+#pragma unroll
+        for (int j = 0; j < MAXC; ++j)
+          if (j <= i) acc[j] += q[j] * yd;
+      }
     }
+    block_reduce_cols<MAXC>(acc, final_pass ? 1 : i + 1, red, red + USBS_RCH * USBS_NT, part_out);
+  } else {
+    row_update(qb, qld, yb, X.R, i + 1, h);
+    if (!final_pass) {
+      col_dots(qb, qld, yb, X.R, i + 1, red, part_out);
+    } else {
+      if constexpr (C)
+        for (int rl = t; rl < X.R; rl += USBS_NT) wc[rl] = yb[rl];
+      self_dot(yb, X.R, red, part_out);
+    }
+  }
+  if (final_pass && tl == 0 && t == 0) {
+    L.alpha[i] = L.h1[i] + h[i];
+    L.ctrl[LZ_STEPS] += 1;
   }
 }
 
@@ -340,7 +402,7 @@ __device__ void breakdown_sub(const LzInst<T>& L, int tl, int i, int sub, double
     __syncthreads();
     col_dots(qb, L.ldq, (const T*)gb, R, i, red, L.part1 + (long long)tl * L.pstride);
   } else {
-    double* h = red + LZ_RED;
+    double* h = lz_hbuf(L, red);
     sum_tile_partials(sub == 1 ? L.part1 : L.part2, L.ntiles, L.pstride, i, h, red);
     row_update(qb, L.ldq, gb, R, i, h);
     if (sub == 1) col_dots(qb, L.ldq, (const T*)gb, R, i, red, L.part2 + (long long)tl * L.pstride);
@@ -376,16 +438,16 @@ __device__ TileCtx<T, false> plain_tile(const LzInst<T>& L, int tl) {
   return X;
 }
 
-template <typename T>
+template <typename T, int MAXC>
 __global__ void __launch_bounds__(USBS_NT) k_lz_A(LzInst<T> L, int i, int first, int mode) {
   extern __shared__ double lds[];
-  phaseA<T, false>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, first != 0, mode, lds);
+  phaseA<T, false, MAXC>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, first != 0, mode, lds);
 }
 
-template <typename T>
+template <typename T, int MAXC>
 __global__ void __launch_bounds__(USBS_NT) k_lz_BC(LzInst<T> L, int i, int final_pass) {
   extern __shared__ double lds[];
-  phaseBC<T, false>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, final_pass, lds);
+  phaseBC<T, false, MAXC>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, final_pass, lds);
 }
 
 // beta_{i-1} and, on breakdown, the restart vector: one block walking every
@@ -444,19 +506,26 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate_multi(const LzInst<T>* __
 
 // Steps i = i0..m-1 of one restart (or of the QR, mode 2) as per-op launches,
 // then the normalisation of Q[:, m].
-template <typename T>
-static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
+template <typename T, int MAXC>
+static void perop_steps_m(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
   if (i0 >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
   const size_t lds = tile_lds_bytes(L.m + 1);
   const dim3 g(L.ntiles), b(USBS_NT);
   for (int i = i0; i < L.m; ++i) {
     if (i > i0) k_lz_pre<T><<<1, b, lds, s>>>(L, i);
-    k_lz_A<T><<<g, b, lds, s>>>(L, i, i == i0 ? 1 : 0, mode);
-    k_lz_BC<T><<<g, b, lds, s>>>(L, i, 0);
-    k_lz_BC<T><<<g, b, lds, s>>>(L, i, 1);
+    k_lz_A<T, MAXC><<<g, b, lds, s>>>(L, i, i == i0 ? 1 : 0, mode);
+    k_lz_BC<T, MAXC><<<g, b, lds, s>>>(L, i, 0);
+    k_lz_BC<T, MAXC><<<g, b, lds, s>>>(L, i, 1);
   }
   k_lz_pre<T><<<1, b, lds, s>>>(L, L.m);
-  k_lz_A<T><<<g, b, lds, s>>>(L, L.m, 0, 3);
+  k_lz_A<T, MAXC><<<g, b, lds, s>>>(L, L.m, 0, 3);
+}
+
+template <typename T>
+static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
+  if (L.m + 1 <= 17) perop_steps_m<T, 17>(L, i0, mode, s);
+  else if (L.m + 1 <= 33) perop_steps_m<T, 33>(L, i0, mode, s);
+  else perop_steps_m<T, 49>(L, i0, mode, s);
 }
 
 template <typename T, int MAXC>
@@ -512,7 +581,7 @@ static inline size_t lz_cache_bytes(int tile, int m, int nnz_tile) {
   return (b + 255) / 256 * 256;
 }
 
-template <typename T, bool C>
+template <typename T, bool C, int MAXC>
 __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __restrict__ insts,
                                                            const int2* __restrict__ wgmap, int r,
                                                            unsigned long long spin_limit) {
@@ -580,15 +649,15 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
     }
     if (i == L.m) {
       for (int tl = lw; tl < L.ntiles; tl += G)
-        phaseA<T, C>(L, tile_of(tl), tl, i, false, 3, red, beta);
+        phaseA<T, C, MAXC>(L, tile_of(tl), tl, i, false, 3, red, beta);
       break;
     }
     for (int tl = lw; tl < L.ntiles; tl += G)
-      phaseA<T, C>(L, tile_of(tl), tl, i, i == k, 0, red, beta);
+      phaseA<T, C, MAXC>(L, tile_of(tl), tl, i, i == k, 0, red, beta);
     if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C>(L, tile_of(tl), tl, i, 0, red);
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC>(L, tile_of(tl), tl, i, 0, red);
     if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C>(L, tile_of(tl), tl, i, 1, red);
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC>(L, tile_of(tl), tl, i, 1, red);
     ok = grid_barrier(bar, G, target, err, spin_limit);
   }
 }
@@ -603,9 +672,9 @@ static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs,
   for (int r = 0; r < max_restarts; ++r) {
     void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&r, (void*)&spin_limit};
     if (evs && hipEventRecord(evs[2 * r], s) != hipSuccess) return (int)hipErrorUnknown;
-    hipError_t e = cache ? hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, true>,
+    hipError_t e = cache ? hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, true, MAXC>,
                                                       dim3(total_wgs), dim3(USBS_NT), args, lds, s)
-                         : hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, false>,
+                         : hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, false, MAXC>,
                                                       dim3(total_wgs), dim3(USBS_NT), args, lds, s);
     if (e != hipSuccess) return (int)e;
     if (evs && hipEventRecord(evs[2 * r + 1], s) != hipSuccess) return (int)hipErrorUnknown;
@@ -621,6 +690,7 @@ static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs,
 template <typename T>
 size_t lanczos_cache_bytes(const LzInst<T>& L) {
   if (L.nblocks < L.ntiles) return 0;  // a workgroup would own several tiles
+  if (L.tile > 256) return 0;          // row-wise tiles stream Q from memory
   const size_t b = lz_cache_bytes<T>(L.tile, L.m, L.tile_nnz_max);
   return b <= LZ_CACHE_MAX_BYTES ? b : 0;
 }
@@ -648,13 +718,13 @@ size_t lanczos_tile_lds_bytes(int m) { return tile_lds_bytes(m + 1); }
 
 int lanczos_max_coresident_blocks(size_t lds_bytes, int device) {
   int per_cu = 0, per_cu64 = 0, per_cu_c = 0, per_cu64_c = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, false>, USBS_NT,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, false, 49>, USBS_NT,
                                                    lds_bytes) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, false>,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, false, 49>,
                                                    USBS_NT, lds_bytes) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_lz_steps_coop<float, true>, USBS_NT,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_lz_steps_coop<float, true, 49>, USBS_NT,
                                                    lds_bytes) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64_c, k_lz_steps_coop<double, true>,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64_c, k_lz_steps_coop<double, true, 49>,
                                                    USBS_NT, lds_bytes) != hipSuccess)
     return 0;
   int pc = per_cu;
