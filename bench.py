@@ -38,12 +38,21 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lz_steps_coop_torus100x200_traffi
 
 SOLVER = dict(trace_factor=2.0, rho=0.01, beta=0.25, k_curr=10, k_past=1, lanczos_inner=32,
               lanczos_restarts=10, subprob_eps=1e-7, subprob_max_iters=100, sketch_dim=10)
+# The lifted QAP with the QAP script's settings (k_curr=2, k_past=0,
+# sketch_dim=l, rho=0.1, 100 restarts, 15 subproblem iterations) and
+# trace_factor 2 (trace_ub = 2 after SCALE_X = 1/(l+1)).
+QAP_L = 48
+QAP_SOLVER = dict(trace_factor=2.0, rho=0.1, beta=0.25, k_curr=2, k_past=0, lanczos_inner=32,
+                  lanczos_restarts=100, subprob_eps=1e-7, subprob_max_iters=15, sketch_dim=QAP_L)
 
 
 WORKLOADS = {
     "torus": "max-cut SDP, torus 100x200 (n=20000, 40000 edges, w in {-1,+1}, seed 0)",
     "rgg": "max-cut SDP, random geometric graph n=2^20 in the unit square, radius 1.3504e-3 "
            "(mean degree 6), unit weights, seed 0",
+    "qap": "lifted QAP SDP, l=48 (X of order 2305): flow F symmetric integers 0-100 with 70% "
+           "zeros, distance D from 48 random points on the 100x100 grid (rounded Euclidean, "
+           "clipped at 100), seed 0",
 }
 
 
@@ -59,18 +68,46 @@ def workload(name="torus"):
     return W, n, Om, q0
 
 
-def run_gpu(ctx, state, n, q0, iters, precision, exec_mode, timing=False):
-    from usbs_amd import maxcut_helpers
-    from usbs_amd.usbs import usbs
+def make_problem(name, oracle=False):
+    """The named workload as (state, n, m, q0, settings, trace_ub, callback):
+    max-cut states come from the graph, the QAP state from its builder
+    (library host code; ``oracle`` selects the oracle's max-cut builder and
+    rounding for the CPU baseline)."""
+    from usbs_amd import problems
+    if name == "qap":
+        from usbs_amd import qap_helpers
+        l = QAP_L
+        D, F = problems.qap_instance(l, 0)
+        N = l * l + 1
+        st = qap_helpers.initialize_state(qap_helpers.build_objective_matrix(D, F), l,
+                                          Omega=problems.gaussian_matrix(N, l, 1))
+        if oracle:
+            st = st._replace(C=st.C.to_scipy())
+        s = QAP_SOLVER
+        return (st, N, st.b.size, problems.unit_start_vector(N, 2), s,
+                s["trace_factor"] * (l + 1) * st.SCALE_X, None)
+    W, n, Om, q0 = workload(name)
     s = SOLVER
+    if oracle:
+        from oracle import maxcut as omc
+        return (omc.initialize_state(W, s["sketch_dim"], Om), n, n, q0, s, s["trace_factor"] * n,
+                omc.compute_max_cut)
+    from usbs_amd import maxcut_helpers
+    st = maxcut_helpers.initialize_state(W, s["sketch_dim"], Omega=Om)
+    return st, n, n, q0, s, s["trace_factor"] * n, maxcut_helpers.compute_max_cut
+
+
+def run_gpu(ctx, prob, iters, precision, exec_mode, timing=False):
+    from usbs_amd.usbs import usbs
+    state, n, m, q0, s, tu, cb = prob
     log = []
     if timing:
         ctx.set_timing(True)
     t0 = time.perf_counter()
-    usbs(state, n, n, s["trace_factor"] * n, s["trace_factor"], s["rho"], s["beta"], s["k_curr"],
+    usbs(state, n, m, tu, s["trace_factor"], s["rho"], s["beta"], s["k_curr"],
          s["k_past"], iters, float("inf"), -float("inf"), -float("inf"), -float("inf"),
          s["lanczos_inner"], s["lanczos_restarts"], s["subprob_eps"], s["subprob_max_iters"], 1.0,
-         maxcut_helpers.compute_max_cut, None, state.C, precision=precision, exec_mode=exec_mode,
+         cb, None, state.C, precision=precision, exec_mode=exec_mode,
          q0=q0, log=log, context=ctx, upload=False)
     el = time.perf_counter() - t0
     stats = ctx.stats() if timing else None
@@ -118,18 +155,16 @@ def cpu_baseline(iters, name="torus"):
     """The FP64 CPU restatement (oracle) on the same workload, `iters`
     iterations including the initial Lanczos call, BLAS threads capped."""
     import threadpoolctl
-    from oracle import maxcut as omc, usbs as ousbs
-    W, n, Om, q0 = workload(name)
+    from oracle import usbs as ousbs
     cores = min(16, len(os.sched_getaffinity(0)))
-    s = SOLVER
     with threadpoolctl.threadpool_limits(limits=cores):
-        st = omc.initialize_state(W, s["sketch_dim"], Om)
+        st, n, m, q0, s, tu, cb = make_problem(name, oracle=True)
         log = []
         t0 = time.perf_counter()
-        ousbs.usbs(st, n, n, s["trace_factor"] * n, s["trace_factor"], s["rho"], s["beta"],
+        ousbs.usbs(st, n, m, tu, s["trace_factor"], s["rho"], s["beta"],
                    s["k_curr"], s["k_past"], iters, 1e300, -1e300, -1e300, -1e300,
                    s["lanczos_inner"], s["lanczos_restarts"], s["subprob_eps"],
-                   s["subprob_max_iters"], 1.0, omc.compute_max_cut, None, st.C, q0, log=log)
+                   s["subprob_max_iters"], 1.0, cb, None, st.C, q0, log=log)
         el = time.perf_counter() - t0
     return {"value": round(iters / el, 4), "unit": "USBS iterations/s", "cores": cores,
             "kind": "port",
@@ -178,10 +213,11 @@ def main():
     ap.add_argument("--steps", type=int, default=300, help="timed USBS iterations")
     ap.add_argument("--warmup", type=int, default=20, help="untimed USBS iterations")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--exec-mode", default="persistent", choices=["persistent", "perop"])
+    ap.add_argument("--exec-mode", default="auto", choices=["auto", "persistent", "perop"],
+                    help="auto: per-op Lanczos for heavy-row operators (QAP), else persistent")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison")
-    ap.add_argument("--workload", default="torus", choices=["torus", "rgg", "serving"],
-                    help="torus 100x200 (default), RGG 2^20, or the serving stream")
+    ap.add_argument("--workload", default="torus", choices=["torus", "rgg", "qap", "serving"],
+                    help="torus 100x200 (default), RGG 2^20, QAP l=48, or the serving stream")
     ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent solver slots")
@@ -212,7 +248,7 @@ def main():
         serving_bench(args)
         return
 
-    from usbs_amd import _lib, maxcut_helpers
+    from usbs_amd import _lib
     from usbs_amd.usbs import upload_problem
 
     try:
@@ -221,16 +257,16 @@ def main():
     except ImportError:
         sync = lambda: None  # noqa: E731
 
-    W, n, Om, q0 = workload(args.workload)
+    prob = make_problem(args.workload)
+    state, n, m = prob[0], prob[1], prob[2]
     ctx = _lib.Context(0)
-    state = maxcut_helpers.initialize_state(W, SOLVER["sketch_dim"], Omega=Om)
-    upload_problem(ctx, state, n, n)
+    upload_problem(ctx, state, n, m)
 
     if args.warmup > 0:
-        run_gpu(ctx, state, n, q0, args.warmup, args.precision, args.exec_mode)
+        run_gpu(ctx, prob, args.warmup, args.precision, args.exec_mode)
     sync()
-    el, log, stats = run_gpu(ctx, state, n, q0, args.steps, args.precision, args.exec_mode,
-                             timing=(args.exec_mode == "persistent"))
+    el, log, stats = run_gpu(ctx, prob, args.steps, args.precision, args.exec_mode,
+                             timing=(args.exec_mode != "perop"))
     sync()
     K = int(log[-1]["t"])
     ms = 1e3 * el / max(K, 1)
@@ -239,16 +275,18 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 storage / f64 accumulate" if args.precision == "mixed" else "f64",
            "data": "synthetic (seeded graph, seeded Omega and q0)",
-           "config": {"workload": WORKLOADS[args.workload], "n": n,
+           "config": {"workload": WORKLOADS[args.workload], "n": n, "m": m,
                       "precision": args.precision, "exec_mode": args.exec_mode,
-                      "step": "one USBS iteration", **SOLVER}}
+                      "step": "one USBS iteration", **prob[4]}}
     last = log[-1]
     out["gap"] = {"obj_gap_end": last["obj_gap"], "infeas_gap_end": last["infeas_gap"],
-                  "max_infeas_end": last["max_infeas"], "cut_value_end": last["callback_val"],
+                  "max_infeas_end": last["max_infeas"],
+                  "cut_value_end": (None if last["callback_val"] != last["callback_val"]
+                                    else last["callback_val"]),
                   "time_to_gap": gap_marks(log), "target_1e-5_reached":
                       gap_marks(log)["1e-05"] is not None}
     if not args.no_fp64_leg:
-        el64, log64, _ = run_gpu(ctx, state, n, q0, args.steps, "fp64", "perop")
+        el64, log64, _ = run_gpu(ctx, prob, args.steps, "fp64", "perop")
         K64 = int(log64[-1]["t"])
         ms64 = 1e3 * el64 / max(K64, 1)
         out["plain_fp64"] = {"exec_mode": "perop", "ms_per_step": round(ms64, 4),

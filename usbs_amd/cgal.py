@@ -5,7 +5,7 @@ Mirrors solver/cgal.py:21-243.  The whole loop (Lanczos eigen-oracle on
 updates, gaps and the max-cut rounding callback) runs on the device through
 ``usbs_cgal_solve``.  Extra keyword arguments select the execution, as for
 usbs(): backend ("hip"), precision ("fp64" | "mixed"), exec_mode ("perop" |
-"persistent"), q0 (the Lanczos start vector used at every iteration; the
+"persistent" | "auto"), q0 (the Lanczos start vector used at every iteration; the
 reference draws it from jax.random with PRNGKey(0), cgal.py:108-109), log.
 """
 import ctypes

@@ -218,7 +218,7 @@ int usbs_objective_infeas(usbs_ctx* h, int32_t k, int32_t kc, const double* F, d
   Scratch S(c);
   hipStream_t s = c->stream;
   const long long ld = ((long long)n + 63) / 64 * 64;
-  const int ntl = Pb.ntiles > Pb.mntiles ? Pb.ntiles : Pb.mntiles;
+  const int ntl = std::max(std::max(Pb.ntiles, Pb.mntiles), objective_ntiles(Pb));
   double* dF = upload_cm<double>(S, F, n, k, ld, s);
   double* d_eta = S.up(&eta, 1);
   double* dzb = S.up(zbar, m);

@@ -75,8 +75,8 @@ k_cgal_pass1(int m, int mtile, const double* __restrict__ y, const double* __res
 // per-tile partials of v.zu (= u^T A*(v) u).
 template <typename T>
 __global__ void __launch_bounds__(USBS_NT)
-k_cgal_Au(int m, int mtile, const int* __restrict__ con_ptr, const int* __restrict__ con_trip,
-          const int* __restrict__ ti, const int* __restrict__ tj, const double* __restrict__ td,
+k_cgal_Au(int m, int mtile, const int* __restrict__ con_ptr, const int* __restrict__ ct_i,
+          const int* __restrict__ ct_j, const double* __restrict__ ct_d,
           const T* __restrict__ u, const double* __restrict__ v, double* __restrict__ zu,
           double* __restrict__ partial) {
   __shared__ double red[8];
@@ -85,10 +85,8 @@ k_cgal_Au(int m, int mtile, const int* __restrict__ con_ptr, const int* __restri
   double s = 0.0;
   for (int c = c0 + (int)threadIdx.x; c < c1; c += USBS_NT) {
     double a = 0.0;
-    for (int p = con_ptr[c]; p < con_ptr[c + 1]; ++p) {
-      const int t = con_trip[p];
-      a += td[t] * (double)u[ti[t]] * (double)u[tj[t]];
-    }
+    for (int p = con_ptr[c]; p < con_ptr[c + 1]; ++p)
+      a += ct_d[p] * (double)u[ct_i[p]] * (double)u[ct_j[p]];
     zu[c] = a;
     s += v[c] * a;
   }
@@ -193,7 +191,8 @@ struct CgalSolver {
       return USBS_ERR_OOM;
     if (!(PO = alloc<T>(ld * 2 * r)) || !(op64 = alloc<double>(P->nnz))) return USBS_ERR_OOM;
     if (sizeof(T) == 4 && !(op32 = alloc<float>(P->nnz))) return USBS_ERR_OOM;
-    if (!(partial = alloc<double>((size_t)ntl * std::max(2 * r * r, 64))) ||
+    if (!(partial = alloc<double>(std::max((size_t)P->ntiles * std::max(2 * r * r, 64),
+                                           (size_t)objective_ntiles(*P) * 16))) ||
         !(dpart = alloc<double>((size_t)ntl * r)) || !(ipart = alloc<long long>((size_t)ntl * r)) ||
         !(obj2 = alloc<double>(2)) || !(uu = alloc<double>(1)) || !(FO = alloc<double>(r)) ||
         !(PtPO = alloc<double>(2 * r * r)) || !(OtO = alloc<double>(r * r)) ||
@@ -228,8 +227,11 @@ struct CgalSolver {
     if (p.exec_mode == USBS_EXEC_PEROP) {
       CK(lanczos_perop<T>(lz, s, host_ctrl));
     } else {
+      int* ctrl_ptr = lz.ctrl;
+      const LzStop stop{&ctrl_ptr, 2, nullptr};
       CK(lanczos_persistent<T>(d_lz, 1, nb, mL, lz.max_restarts, s, d_map, d_map + 2 * nb,
-                               lz.ntiles, 1ull << 26, cache_bytes));
+                               lz.ntiles, 1ull << 26, cache_bytes, nullptr,
+                               lz.tile > 256 ? 1 : 0, &stop));
     }
     return USBS_OK;
   }
@@ -276,8 +278,8 @@ struct CgalSolver {
     CK(launch_objective<T>(*P, 1, 1, u, ld, partial, obj2, s));
     CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, u, ld, 1, u, ld, 1, partial, uu, s)));
     if (r > 0) CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, u, ld, 1, Om(), ld, r, partial, FO, s)));
-    k_cgal_Au<T><<<P->mntiles, USBS_NT, 0, s>>>(m, P->mtile, P->con_ptr, P->con_trip, P->ti, P->tj,
-                                                 P->tdata, u, v, zu, mpart);
+    k_cgal_Au<T><<<P->mntiles, USBS_NT, 0, s>>>(m, P->mtile, P->con_ptr, P->ct_i, P->ct_j,
+                                                 P->ct_d, u, v, zu, mpart);
     CK((int)hipGetLastError());
     double h[4];  // lambda, u^T C u, |u|^2
     if (hipMemcpyAsync(h, lz.alpha, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -445,7 +447,8 @@ extern "C" int usbs_cgal_solve(usbs_ctx* h, const usbs_cgal_params* prm, const d
   if (!h || !prm) return USBS_ERR_INVALID;
   Ctx* c = &h->c;
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
-  const usbs_cgal_params& p = *prm;
+  usbs_cgal_params p = *prm;
+  if (p.exec_mode == USBS_EXEC_AUTO) p.exec_mode = resolve_exec_mode(c->P);
   if (!y0 || !z0 || !q0) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   if (p.lanczos_inner_iterations < 1 || p.lanczos_inner_iterations > 32 ||
       p.lanczos_inner_iterations > c->P.n)

@@ -96,7 +96,9 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
     if (std::fabs(v4) > 1e9 || v4 != std::nearbyint(v4)) cut_int = false;
     else cw4[s] = (int)std::nearbyint(v4);
   }
-  std::vector<int> tslot(nA), slot_ptr(nnz + 1, 0), slot_trip(nA), con_ptr(m + 1, 0), con_trip(nA);
+  std::vector<int> tslot(nA), slot_ptr(nnz + 1, 0), slot_trip(nA), con_ptr(m + 1, 0);
+  std::vector<int> ct_i(nA), ct_j(nA);
+  std::vector<double> ct_d(nA);
   std::vector<int> tcon(nA), ti(nA), tj(nA);
   for (int64_t t = 0; t < nA; ++t) {
     tcon[t] = Ai[3 * t];
@@ -112,7 +114,10 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
     std::vector<int> fill_s(slot_ptr.begin(), slot_ptr.end() - 1), fill_c(con_ptr.begin(), con_ptr.end() - 1);
     for (int64_t t = 0; t < nA; ++t) {
       slot_trip[fill_s[tslot[t]]++] = (int)t;
-      con_trip[fill_c[tcon[t]]++] = (int)t;
+      const int e = fill_c[tcon[t]]++;
+      ct_i[e] = ti[t];
+      ct_j[e] = tj[t];
+      ct_d[e] = Ad[t];
     }
   }
   bool diag = (nA == n && m == n);
@@ -138,11 +143,14 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
   P.nA = nA;
   P.tile = tile_rows_for((int)n);
   P.ntiles = (int)((n + P.tile - 1) / P.tile);
-  for (int t = 0; t < P.ntiles; ++t) {
-    const int64_t a = (int64_t)t * P.tile, b = std::min<int64_t>(n, a + P.tile);
+  P.lz_heavy = (n <= 65536 && (int64_t)row_ptr[n] >= 32 * n) ? 1 : 0;
+  P.lz_tile = P.lz_heavy ? lanczos_heavy_tile_rows((int)n) : P.tile;
+  P.lz_ntiles = (int)((n + P.lz_tile - 1) / P.lz_tile);
+  for (int t = 0; t < P.lz_ntiles; ++t) {
+    const int64_t a = (int64_t)t * P.lz_tile, b = std::min<int64_t>(n, a + P.lz_tile);
     P.tile_nnz_max = std::max(P.tile_nnz_max, row_ptr[b] - row_ptr[a]);
   }
-  P.mtile = tile_rows_for((int)m);
+  P.mtile = 256;
   P.mntiles = (int)((m + P.mtile - 1) / P.mtile);
   P.diag_A = diag;
   P.has_ineq = ineq;
@@ -160,7 +168,9 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
   if (!(P.tj = upload(c, pool, tj.data(), tj.size()))) return USBS_ERR_OOM;
   if (!(P.tdata = upload(c, pool, tdata.data(), tdata.size()))) return USBS_ERR_OOM;
   if (!(P.con_ptr = upload(c, pool, con_ptr.data(), con_ptr.size()))) return USBS_ERR_OOM;
-  if (!(P.con_trip = upload(c, pool, con_trip.data(), con_trip.size()))) return USBS_ERR_OOM;
+  if (!(P.ct_i = upload(c, pool, ct_i.data(), ct_i.size()))) return USBS_ERR_OOM;
+  if (!(P.ct_j = upload(c, pool, ct_j.data(), ct_j.size()))) return USBS_ERR_OOM;
+  if (!(P.ct_d = upload(c, pool, ct_d.data(), ct_d.size()))) return USBS_ERR_OOM;
   if (!(P.b = upload(c, pool, b, (size_t)m))) return USBS_ERR_OOM;
   if (!(P.mask = upload(c, pool, mask, (size_t)m))) return USBS_ERR_OOM;
   if (!(P.scale_a = upload(c, pool, sa, (size_t)m))) return USBS_ERR_OOM;
@@ -174,8 +184,9 @@ int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int
                  int max_restarts, double tol, const T* opval, LzInst<T>& L) {
   L = LzInst<T>();
   L.n = P.n;
-  L.tile = P.tile;
-  L.ntiles = P.ntiles;
+  L.tile = P.lz_tile;
+  L.ntiles = P.lz_ntiles;
+  L.heavy = P.lz_heavy;
   L.m = m;
   L.nd = nd;
   L.max_restarts = max_restarts;
@@ -279,8 +290,11 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
     for (int i = 0; i < L.ntiles; ++i) map[nb + i] = make_int2(0, i);
     USBS_HIP(c, hipMemcpyAsync(d_inst, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
     USBS_HIP(c, hipMemcpyAsync(d_map, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice, c->stream));
+    int* ctrl_ptr = L.ctrl;
+    const LzStop stop{&ctrl_ptr, 2, nullptr};
     rc = lanczos_persistent<T>(d_inst, 1, nb, m, max_restarts, c->stream, d_map, d_map + 2 * nb,
-                               L.ntiles, 1ull << 26, cache_bytes);
+                               L.ntiles, 1ull << 26, cache_bytes, nullptr, L.tile > 256 ? 1 : 0,
+                               &stop);
     if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos persistent: ") + hipGetErrorString((hipError_t)rc));
     USBS_HIP(c, hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, c->stream));
     USBS_HIP(c, hipStreamSynchronize(c->stream));
@@ -378,8 +392,15 @@ static int run_lanczos_batch(Ctx** cs, int ninst, double C_coef, const double* c
   USBS_HIP(c0, hipMemcpyAsync(d_inst, L.data(), sizeof(LzInst<T>) * ninst, hipMemcpyHostToDevice, s));
   USBS_HIP(c0, hipMemcpyAsync(d_wg, wgmap.data(), sizeof(int2) * wgmap.size(), hipMemcpyHostToDevice, s));
   USBS_HIP(c0, hipMemcpyAsync(d_tm, tilemap.data(), sizeof(int2) * tilemap.size(), hipMemcpyHostToDevice, s));
+  int rowwise = 0;
+  std::vector<int*> ctrls(ninst);
+  for (int q = 0; q < ninst; ++q) {
+    rowwise |= (L[q].tile > 256) ? 1 : 0;
+    ctrls[q] = L[q].ctrl;
+  }
+  const LzStop stop{ctrls.data(), 2, nullptr};
   int rc = lanczos_persistent<T>(d_inst, ninst, total_wgs, m, max_restarts, s, d_wg, d_tm,
-                                 total_tiles, 1ull << 26, cache);
+                                 total_tiles, 1ull << 26, cache, nullptr, rowwise, &stop);
   if (rc) return set_err(c0, USBS_ERR_HIP, std::string("lanczos batch: ") + hipGetErrorString((hipError_t)rc));
   for (int q = 0; q < ninst; ++q) {
     const Problem& P = cs[q]->P;
@@ -481,6 +502,7 @@ int usbs_lanczos_smallest(usbs_ctx* h, double C_coef, const double* y, const dou
   if (max_restarts < 1) return set_err(c, USBS_ERR_INVALID, "max_restarts must be >= 1");
   if (precision != USBS_PREC_FP64 && precision != USBS_PREC_MIXED)
     return set_err(c, USBS_ERR_INVALID, "unknown precision");
+  if (exec_mode == USBS_EXEC_AUTO) exec_mode = resolve_exec_mode(c->P);
   if (exec_mode != USBS_EXEC_PEROP && exec_mode != USBS_EXEC_PERSISTENT)
     return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
   for (int i = 0; i < c->P.m; ++i)

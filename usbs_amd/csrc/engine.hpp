@@ -25,8 +25,13 @@ struct Problem {
   int n = 0, m = 0;
   long long nnz = 0, nA = 0;
   int tile = 0, ntiles = 0;     // row tiles of n
-  int tile_nnz_max = 0;         // most operator entries in one row tile
-  int mtile = 0, mntiles = 0;   // tiles over the m constraints
+  // Lanczos row tiles: the same as tile, except for operators whose rows
+  // average 32 or more entries (heavy rows, a wavefront per row in the SpMV),
+  // which get tiles of 16+ rows so that enough workgroups share the work.
+  int lz_tile = 0, lz_ntiles = 0;
+  int lz_heavy = 0;
+  int tile_nnz_max = 0;         // most operator entries in one Lanczos row tile
+  int mtile = 0, mntiles = 0;   // tiles over the m constraints (256: a thread per constraint)
   bool diag_A = false;          // A = {(i, i, i) with data 1}: max-cut's diag(X) = 1
   bool has_ineq = false;        // any b_ineq_mask != 0
   bool cut_integer = false;     // every off-diagonal 4*C_ij is an integer
@@ -41,11 +46,20 @@ struct Problem {
   int* tj = nullptr;            // nA
   double* tdata = nullptr;      // nA
   int* con_ptr = nullptr;       // m+1: triples per constraint
-  int* con_trip = nullptr;      // nA
+  int* ct_i = nullptr;          // nA: the triples' (i, j, data) sorted by
+  int* ct_j = nullptr;          //     constraint (input order within one), so
+  double* ct_d = nullptr;       //     per-constraint loops read them directly
   double* b = nullptr;          // m
   double* mask = nullptr;       // m
   double* scale_a = nullptr;    // m
 };
+
+// USBS_EXEC_AUTO: heavy-row operators run the per-op Lanczos (their short
+// row tiles make device-wide barriers cost more than kernel launches).
+inline int resolve_exec_mode(const Problem& P) { return P.lz_heavy ? 0 : 1; }
+
+// Row tiles of the objective's partials: the Lanczos tiles for heavy rows.
+inline int objective_ntiles(const Problem& P) { return P.lz_heavy ? P.lz_ntiles : P.ntiles; }
 
 struct Ctx {
   int device = 0;

@@ -25,6 +25,8 @@
 // forms give bitwise the same numbers.  The persistent form keeps its tile's
 // rows of Q, the work vector and the CSR slice in LDS for the whole restart
 // when they fit.
+#include <vector>
+
 #include "lanczos.hpp"
 #include "dense.hpp"
 
@@ -35,6 +37,20 @@ int tile_rows_for(int n) {
   while ((n + t - 1) / t > 256) t *= 2;
   return t;
 }
+
+int lanczos_heavy_tile_rows(int n) {
+  int t = 16;
+  while ((n + t - 1) / t > 256) t *= 2;
+  return t;
+}
+
+// Heavy-row SpMV: rows of a wavefront taken together, entries per lane and row per round.
+#ifndef LZ_HEAVY_RG
+#define LZ_HEAVY_RG 2
+#endif
+#ifndef LZ_HEAVY_UN
+#define LZ_HEAVY_UN 8
+#endif
 
 #define LZ_RED (USBS_NT + 64)  // doubles of reduction scratch in every tile kernel
 // Tiles of more than 256 rows reduce row-wise in registers and transpose
@@ -257,7 +273,9 @@ struct TileCtx {
 
 // Phase A (mode 0: SpMV; mode 2: y = Q[:, i+1] for the column-by-column QR;
 // mode 3: only normalise w into column i).  first: q_i is already in Q.
-template <typename T, bool C, int MAXC>
+// RW: the kernel may meet tiles of more than 256 rows (row-wise reductions);
+// kernels instantiated with RW = false carry no row-wise code.
+template <typename T, bool C, int MAXC, bool RW>
 __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, bool first,
                        int mode, double* red, double beta_known = -1.0) {
   const int t = threadIdx.x;
@@ -279,11 +297,72 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
   const T* qb;
   long long qld;
   if constexpr (C) { qb = X.qc; qld = X.ldc; } else { qb = L.Q + X.r0; qld = L.ldq; }
-  const bool rowwise = lz_rowwise(L);
+  const bool rowwise = RW && lz_rowwise(L);
   double acc[MAXC];
 #pragma unroll
   for (int j = 0; j < MAXC; ++j) acc[j] = 0.0;
   const T* xq = L.Q + (long long)i * L.ldq;
+  if (mode == 0 && L.heavy) {
+    // a wavefront per row, four rows of a wavefront at a time: lane l takes
+    // entries l, l+64, ... of each row, four per row per round, so sixteen
+    // independent loads are in flight per lane; each row's lane sums add in
+    // entry order, then a butterfly over the wavefront, and lane 0 writes.
+    // Heavy tiles have at most 256 rows (column-wise dots).
+    constexpr int RG = LZ_HEAVY_RG, UN = LZ_HEAVY_UN;
+    const int lane = t & 63, wv = t >> 6;
+    const int* ci;
+    const T* cv;
+    const int* rp;
+    if constexpr (C) { ci = X.ci; cv = X.cv; rp = X.rp; }
+    else { ci = L.col; cv = L.val; rp = L.row_ptr + X.r0; }
+    for (int q0 = wv; q0 < X.R; q0 += RG * (USBS_NT / 64)) {
+      int pb[RG], pe[RG];
+      double s[RG];
+      int rounds = 0;
+#pragma unroll
+      for (int g = 0; g < RG; ++g) {
+        const int rl = q0 + g * (USBS_NT / 64);
+        pb[g] = rl < X.R ? rp[rl] : 0;
+        pe[g] = rl < X.R ? rp[rl + 1] : 0;
+        s[g] = 0.0;
+        const int nr = (pe[g] - pb[g] + 64 * UN - 1) / (64 * UN);
+        rounds = nr > rounds ? nr : rounds;
+      }
+      for (int it = 0; it < rounds; ++it) {
+        int c[RG][UN];
+        T v[RG][UN];
+#pragma unroll
+        for (int g = 0; g < RG; ++g)
+#pragma unroll
+          for (int u = 0; u < UN; ++u) {
+            // past the row's end: load entry 0 (in bounds) and mask it
+            const int e = pb[g] + (it * UN + u) * 64 + lane;
+            const bool ok = e < pe[g];
+            const int cl = ci[ok ? e : 0];
+            const T vl = cv[ok ? e : 0];
+            c[g][u] = ok ? cl : 0;
+            v[g][u] = ok ? vl : (T)0;
+          }
+#pragma unroll
+        for (int g = 0; g < RG; ++g)
+#pragma unroll
+          for (int u = 0; u < UN; ++u) {
+            const T x = first ? xq[c[g][u]] : (T)((double)gb[c[g][u]] / beta);
+            s[g] += (double)v[g][u] * (double)x;
+          }
+      }
+#pragma unroll
+      for (int g = 0; g < RG; ++g) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s[g] += __shfl_xor(s[g], off, 64);
+        const int rl = q0 + g * (USBS_NT / 64);
+        if (lane == 0 && rl < X.R) yb[rl] = (T)s[g];
+      }
+    }
+    __syncthreads();
+    col_dots(qb, qld, yb, X.R, i + 1, red, L.part1 + (long long)tl * L.pstride);
+    return;
+  }
   for (int rl = t; rl < X.R; rl += USBS_NT) {
     double s = 0.0;
     if (mode == 0) {
@@ -329,7 +408,7 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
 }
 
 // Phases B (final = 0) and C (final = 1).
-template <typename T, bool C, int MAXC>
+template <typename T, bool C, int MAXC, bool RW>
 __device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, int final_pass,
                         double* red) {
   const int t = threadIdx.x;
@@ -344,7 +423,7 @@ __device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int 
   T* wc = L.w + (long long)(i & 1) * L.ldq + X.r0;
   if constexpr (C) { yb = X.yc; qb = X.qc; qld = X.ldc; } else { yb = wc; qb = L.Q + X.r0; qld = L.ldq; }
   double* part_out = (final_pass ? L.part3 : L.part2) + (long long)tl * L.pstride;
-  if (lz_rowwise(L)) {
+  if (RW && lz_rowwise(L)) {
     // one pass over the tile's Q rows: update, then the next dots
     double acc[MAXC];
 #pragma unroll
@@ -438,16 +517,16 @@ __device__ TileCtx<T, false> plain_tile(const LzInst<T>& L, int tl) {
   return X;
 }
 
-template <typename T, int MAXC>
+template <typename T, int MAXC, bool RW>
 __global__ void __launch_bounds__(USBS_NT) k_lz_A(LzInst<T> L, int i, int first, int mode) {
   extern __shared__ double lds[];
-  phaseA<T, false, MAXC>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, first != 0, mode, lds);
+  phaseA<T, false, MAXC, RW>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, first != 0, mode, lds);
 }
 
-template <typename T, int MAXC>
+template <typename T, int MAXC, bool RW>
 __global__ void __launch_bounds__(USBS_NT) k_lz_BC(LzInst<T> L, int i, int final_pass) {
   extern __shared__ double lds[];
-  phaseBC<T, false, MAXC>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, final_pass, lds);
+  phaseBC<T, false, MAXC, RW>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, final_pass, lds);
 }
 
 // beta_{i-1} and, on breakdown, the restart vector: one block walking every
@@ -506,26 +585,32 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate_multi(const LzInst<T>* __
 
 // Steps i = i0..m-1 of one restart (or of the QR, mode 2) as per-op launches,
 // then the normalisation of Q[:, m].
-template <typename T, int MAXC>
+template <typename T, int MAXC, bool RW>
 static void perop_steps_m(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
   if (i0 >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
   const size_t lds = tile_lds_bytes(L.m + 1);
   const dim3 g(L.ntiles), b(USBS_NT);
   for (int i = i0; i < L.m; ++i) {
     if (i > i0) k_lz_pre<T><<<1, b, lds, s>>>(L, i);
-    k_lz_A<T, MAXC><<<g, b, lds, s>>>(L, i, i == i0 ? 1 : 0, mode);
-    k_lz_BC<T, MAXC><<<g, b, lds, s>>>(L, i, 0);
-    k_lz_BC<T, MAXC><<<g, b, lds, s>>>(L, i, 1);
+    k_lz_A<T, MAXC, RW><<<g, b, lds, s>>>(L, i, i == i0 ? 1 : 0, mode);
+    k_lz_BC<T, MAXC, RW><<<g, b, lds, s>>>(L, i, 0);
+    k_lz_BC<T, MAXC, RW><<<g, b, lds, s>>>(L, i, 1);
   }
   k_lz_pre<T><<<1, b, lds, s>>>(L, L.m);
-  k_lz_A<T, MAXC><<<g, b, lds, s>>>(L, L.m, 0, 3);
+  k_lz_A<T, MAXC, RW><<<g, b, lds, s>>>(L, L.m, 0, 3);
+}
+
+template <typename T, bool RW>
+static void perop_steps_rw(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
+  if (L.m + 1 <= 17) perop_steps_m<T, 17, RW>(L, i0, mode, s);
+  else if (L.m + 1 <= 33) perop_steps_m<T, 33, RW>(L, i0, mode, s);
+  else perop_steps_m<T, 49, RW>(L, i0, mode, s);
 }
 
 template <typename T>
 static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
-  if (L.m + 1 <= 17) perop_steps_m<T, 17>(L, i0, mode, s);
-  else if (L.m + 1 <= 33) perop_steps_m<T, 33>(L, i0, mode, s);
-  else perop_steps_m<T, 49>(L, i0, mode, s);
+  if (L.tile > 256) perop_steps_rw<T, true>(L, i0, mode, s);
+  else perop_steps_rw<T, false>(L, i0, mode, s);
 }
 
 template <typename T, int MAXC>
@@ -581,7 +666,7 @@ static inline size_t lz_cache_bytes(int tile, int m, int nnz_tile) {
   return (b + 255) / 256 * 256;
 }
 
-template <typename T, bool C, int MAXC>
+template <typename T, bool C, int MAXC, bool RW>
 __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __restrict__ insts,
                                                            const int2* __restrict__ wgmap, int r,
                                                            unsigned long long spin_limit) {
@@ -649,15 +734,15 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
     }
     if (i == L.m) {
       for (int tl = lw; tl < L.ntiles; tl += G)
-        phaseA<T, C, MAXC>(L, tile_of(tl), tl, i, false, 3, red, beta);
+        phaseA<T, C, MAXC, RW>(L, tile_of(tl), tl, i, false, 3, red, beta);
       break;
     }
     for (int tl = lw; tl < L.ntiles; tl += G)
-      phaseA<T, C, MAXC>(L, tile_of(tl), tl, i, i == k, 0, red, beta);
+      phaseA<T, C, MAXC, RW>(L, tile_of(tl), tl, i, i == k, 0, red, beta);
     if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC>(L, tile_of(tl), tl, i, 0, red);
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC, RW>(L, tile_of(tl), tl, i, 0, red);
     if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC>(L, tile_of(tl), tl, i, 1, red);
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC, RW>(L, tile_of(tl), tl, i, 1, red);
     ok = grid_barrier(bar, G, target, err, spin_limit);
   }
 }
@@ -666,16 +751,20 @@ template <typename T, int MAXC>
 static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                            int max_restarts, hipStream_t s, const int* dev_wgmap,
                            const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                           size_t cache_bytes, hipEvent_t* evs) {
+                           size_t cache_bytes, hipEvent_t* evs, int rowwise,
+                           const LzStop* stop) {
   const bool cache = cache_bytes > 0;
   const size_t lds = cache ? cache_bytes : tile_lds_bytes(m + 1);
+  const void* kern = cache ? (const void*)k_lz_steps_coop<T, true, MAXC, false>
+                     : rowwise ? (const void*)k_lz_steps_coop<T, false, MAXC, true>
+                               : (const void*)k_lz_steps_coop<T, false, MAXC, false>;
+  int next_check = (stop && stop->ctrls && stop->first_check > 0) ? stop->first_check : max_restarts;
+  std::vector<int> hc;
+  int launched = 0;
   for (int r = 0; r < max_restarts; ++r) {
     void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&r, (void*)&spin_limit};
     if (evs && hipEventRecord(evs[2 * r], s) != hipSuccess) return (int)hipErrorUnknown;
-    hipError_t e = cache ? hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, true, MAXC>,
-                                                      dim3(total_wgs), dim3(USBS_NT), args, lds, s)
-                         : hipLaunchCooperativeKernel((const void*)k_lz_steps_coop<T, false, MAXC>,
-                                                      dim3(total_wgs), dim3(USBS_NT), args, lds, s);
+    hipError_t e = hipLaunchCooperativeKernel(kern, dim3(total_wgs), dim3(USBS_NT), args, lds, s);
     if (e != hipSuccess) return (int)e;
     if (evs && hipEventRecord(evs[2 * r + 1], s) != hipSuccess) return (int)hipErrorUnknown;
     k_lz_ritz_multi<T><<<ninst, LZ_RITZ_NT, ritz_lds_bytes(m), s>>>(dev_insts, r);
@@ -683,7 +772,26 @@ static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs,
         dev_insts, (const int2*)dev_tilemap, r);
     e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
+    launched = r + 1;
+    if (launched == next_check && launched < max_restarts) {
+      // later launches of finished instances return at once but still cost
+      // a launch each: stop when every instance is done (or failed)
+      hc.assign((size_t)LZ_NCTRL * ninst, 0);
+      for (int q = 0; q < ninst; ++q) {
+        e = hipMemcpyAsync(hc.data() + (size_t)q * LZ_NCTRL, stop->ctrls[q], sizeof(int) * LZ_NCTRL,
+                           hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return (int)e;
+      }
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return (int)e;
+      bool all = true;
+      for (int q = 0; q < ninst; ++q)
+        all = all && (hc[(size_t)q * LZ_NCTRL + LZ_DONE] || hc[(size_t)q * LZ_NCTRL + LZ_ERROR]);
+      if (all) break;
+      next_check *= 2;
+    }
   }
+  if (stop && stop->launched) *stop->launched = launched;
   return 0;
 }
 
@@ -701,36 +809,38 @@ template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
                        const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                       size_t cache_bytes, hipEvent_t* evs) {
+                       size_t cache_bytes, hipEvent_t* evs, int rowwise, const LzStop* stop) {
   if (m + 1 <= 17)
     return persistent_impl<T, 17>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs);
+                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs,
+                                  rowwise, stop);
   if (m + 1 <= 33)
     return persistent_impl<T, 33>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs);
+                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs,
+                                  rowwise, stop);
   if (m + 1 <= 49)
     return persistent_impl<T, 49>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
-                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs);
+                                  dev_tilemap, total_tiles, spin_limit, cache_bytes, evs,
+                                  rowwise, stop);
   return (int)hipErrorInvalidValue;
 }
 
 size_t lanczos_tile_lds_bytes(int m) { return tile_lds_bytes(m + 1); }
 
 int lanczos_max_coresident_blocks(size_t lds_bytes, int device) {
-  int per_cu = 0, per_cu64 = 0, per_cu_c = 0, per_cu64_c = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lz_steps_coop<float, false, 49>, USBS_NT,
-                                                   lds_bytes) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64, k_lz_steps_coop<double, false, 49>,
-                                                   USBS_NT, lds_bytes) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, k_lz_steps_coop<float, true, 49>, USBS_NT,
-                                                   lds_bytes) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu64_c, k_lz_steps_coop<double, true, 49>,
-                                                   USBS_NT, lds_bytes) != hipSuccess)
-    return 0;
-  int pc = per_cu;
-  if (per_cu64 < pc) pc = per_cu64;
-  if (per_cu_c < pc) pc = per_cu_c;
-  if (per_cu64_c < pc) pc = per_cu64_c;
+  const void* kerns[] = {(const void*)k_lz_steps_coop<float, false, 49, false>,
+                         (const void*)k_lz_steps_coop<float, false, 49, true>,
+                         (const void*)k_lz_steps_coop<float, true, 49, false>,
+                         (const void*)k_lz_steps_coop<double, false, 49, false>,
+                         (const void*)k_lz_steps_coop<double, false, 49, true>,
+                         (const void*)k_lz_steps_coop<double, true, 49, false>};
+  int pc = 1 << 30;
+  for (const void* kfn : kerns) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, USBS_NT, lds_bytes) != hipSuccess)
+      return 0;
+    if (per_cu < pc) pc = per_cu;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
   return pc * prop.multiProcessorCount;
@@ -761,9 +871,9 @@ template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*);
 template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*);
 template int lanczos_persistent<double>(const LzInst<double>*, int, int, int, int, hipStream_t,
                                         const int*, const int*, int, unsigned long long, size_t,
-                                        hipEvent_t*);
+                                        hipEvent_t*, int, const LzStop*);
 template int lanczos_persistent<float>(const LzInst<float>*, int, int, int, int, hipStream_t,
                                        const int*, const int*, int, unsigned long long, size_t,
-                                       hipEvent_t*);
+                                       hipEvent_t*, int, const LzStop*);
 
 }  // namespace usbs

@@ -45,12 +45,28 @@ struct LzInst {
   int* lstat;    // 2 x nbar: per persistent launch r, steps done and the sum
                  // over those steps of the valid column count i+1
   int tile_nnz_max;  // largest number of operator entries in one row tile
+  int heavy;         // 1: a wavefront per operator row in the SpMV (rows of
+                     // 32+ entries on average), else a thread per row
 };
 
 // Largest LDS block of a persistent workgroup that caches its tile.
 #define LZ_CACHE_MAX_BYTES (64 * 1024)
 
 int tile_rows_for(int n);
+// Row tiles of the Lanczos steps for heavy-row operators (n <= 65536):
+// 16 rows, doubled until there are at most 256 tiles.
+int lanczos_heavy_tile_rows(int n);
+
+// Early stop of the persistent form: after first_check restarts, and at each
+// doubling of that count, the host reads the control words of every instance
+// (ctrls[q]: device pointers, one per instance) and launches no further
+// restart once all are done or failed.  launched (optional) receives the
+// number of restarts launched.
+struct LzStop {
+  int* const* ctrls;
+  int first_check;
+  int* launched;
+};
 
 // Host launchers (lanczos.hip).  All return hipError_t as int.
 template <typename T>
@@ -66,7 +82,8 @@ template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
                        const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                       size_t cache_bytes, hipEvent_t* evs = nullptr);
+                       size_t cache_bytes, hipEvent_t* evs = nullptr, int rowwise = 1,
+                       const LzStop* stop = nullptr);
 // Workgroup count (L.nblocks) and LDS cache size for a persistent launch of
 // one instance: one workgroup per tile unless want_wgs > 0 asks for fewer,
 // capped by what can be co-resident on the device.

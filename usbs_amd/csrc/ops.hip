@@ -93,18 +93,56 @@ k_spmm(int n, const int* __restrict__ row_ptr, const int* __restrict__ col,
     if (j < p) Y[r + j * ldy] = acc[j];
 }
 
+// The same for heavy rows (32+ entries on average): a wavefront per row,
+// lane l summing entries l, l+64, ..., the p sums combined by a butterfly
+// over the wavefront; lane 0 writes.
+template <typename TV, typename TX>
+__global__ void __launch_bounds__(USBS_NT)
+k_spmm_wave(int n, const int* __restrict__ row_ptr, const int* __restrict__ col,
+            const TV* __restrict__ val, const TX* __restrict__ X, long long ldx, int p,
+            double* __restrict__ Y, long long ldy) {
+  const int r = blockIdx.x * (USBS_NT / 64) + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= n) return;  // uniform over the wavefront
+  double acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+  const int p1 = row_ptr[r + 1];
+  for (int e = row_ptr[r] + lane; e < p1; e += 64) {
+    const double v = (double)val[e];
+    const long long c = col[e];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (j < p) acc[j] += v * (double)X[c + j * ldx];
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (j < p) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+      if (lane == 0) Y[r + j * ldy] = acc[j];
+    }
+  }
+}
+
 template <typename TV, typename TX>
 int launch_spmm(int n, const int* row_ptr, const int* col, const TV* val, const TX* X,
-                long long ldx, int p, double* Y, long long ldy, hipStream_t s) {
+                long long ldx, int p, double* Y, long long ldy, int heavy, hipStream_t s) {
   if (p > 16) return (int)hipErrorInvalidValue;
-  k_spmm<TV, TX><<<(n + USBS_NT - 1) / USBS_NT, USBS_NT, 0, s>>>(n, row_ptr, col, val, X, ldx, p,
-                                                                 Y, ldy);
+  if (heavy) {
+    constexpr int rows = USBS_NT / 64;
+    k_spmm_wave<TV, TX><<<(n + rows - 1) / rows, USBS_NT, 0, s>>>(n, row_ptr, col, val, X, ldx, p,
+                                                                  Y, ldy);
+  } else {
+    k_spmm<TV, TX><<<(n + USBS_NT - 1) / USBS_NT, USBS_NT, 0, s>>>(n, row_ptr, col, val, X, ldx,
+                                                                   p, Y, ldy);
+  }
   return (int)hipGetLastError();
 }
 template int launch_spmm<double, double>(int, const int*, const int*, const double*, const double*,
-                                         long long, int, double*, long long, hipStream_t);
+                                         long long, int, double*, long long, int, hipStream_t);
 template int launch_spmm<double, float>(int, const int*, const int*, const double*, const float*,
-                                        long long, int, double*, long long, hipStream_t);
+                                        long long, int, double*, long long, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Tall-skinny Gram: partial[tile][a*q + b] = sum_{rows of tile} A[r,a] B[r,b]
@@ -243,8 +281,8 @@ template int launch_tsmm<double, float>(int, const double*, long long, int, cons
 template <typename TV>
 __global__ void __launch_bounds__(USBS_NT)
 k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
-        const int* __restrict__ con_ptr, const int* __restrict__ con_trip,
-        const int* __restrict__ ti, const int* __restrict__ tj, const double* __restrict__ tdata,
+        const int* __restrict__ con_ptr, const int* __restrict__ ct_i,
+        const int* __restrict__ ct_j, const double* __restrict__ ct_d,
         double* __restrict__ partial) {
   extern __shared__ double lds[];
   const int d = k * (k + 1) / 2;
@@ -298,14 +336,14 @@ k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
         const int bb = pb[a], cc = pc[a];
         const int c = c0 + rr;
         for (int e = con_ptr[c]; e < con_ptr[c + 1]; ++e) {
-          const int t = con_trip[e];
-          const long long i = ti[t], j = tj[t];
+          const long long i = ct_i[e], j = ct_j[e];
+          const double td = ct_d[e];
           const double vib = (double)V[i + bb * ldv], vjc = (double)V[j + cc * ldv];
           if (bb == cc) {
-            v += tdata[t] * (vib * vjc);
+            v += td * (vib * vjc);
           } else {
             const double vic = (double)V[i + cc * ldv], vjb = (double)V[j + bb * ldv];
-            v += tdata[t] * (vib * vjc) * u + tdata[t] * (vic * vjb) * u;
+            v += td * (vib * vjc) * u + td * (vic * vjb) * u;
           }
         }
       }
@@ -358,6 +396,12 @@ __global__ void __launch_bounds__(256) k_reduce_sym(const double* __restrict__ p
   if ((threadIdx.x & 63) == 0) out[e] = s;
 }
 
+int qbase_tile_rows(int m, int d) {
+  int t = 256;
+  while ((long long)((m + t - 1) / t) * d * d > (1 << 20) && (m + t - 1) / t > 256) t *= 2;
+  return t;
+}
+
 template <typename TV>
 int launch_qbase(const Problem& P, int k, const TV* V, long long ldv, double* partial, double* out,
                  hipStream_t s) {
@@ -366,11 +410,12 @@ int launch_qbase(const Problem& P, int k, const TV* V, long long ldv, double* pa
   const int nbk = dp / QB_BS;
   if (nbk * (nbk + 1) / 2 > 2 * USBS_NT) return (int)hipErrorInvalidValue;
   const size_t lds = sizeof(double) * GRAM_CH * dp + sizeof(int) * 2 * dp;
-  k_qbase<TV><<<P.mntiles, USBS_NT, lds, s>>>(P.m, P.mtile, k, V, ldv, P.con_ptr, P.con_trip,
-                                              P.ti, P.tj, P.tdata, partial);
+  const int qt = qbase_tile_rows(P.m, d), qnt = (P.m + qt - 1) / qt;
+  k_qbase<TV><<<qnt, USBS_NT, lds, s>>>(P.m, qt, k, V, ldv, P.con_ptr, P.ct_i, P.ct_j, P.ct_d,
+                                        partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
-  k_reduce_sym<<<(d * d + 3) / 4, 256, 0, s>>>(partial, P.mntiles, d, out);
+  k_reduce_sym<<<(d * d + 3) / 4, 256, 0, s>>>(partial, qnt, d, out);
   return (int)hipGetLastError();
 }
 template int launch_qbase<double>(const Problem&, int, const double*, long long, double*, double*,
@@ -388,9 +433,9 @@ template int launch_qbase<float>(const Problem&, int, const float*, long long, d
 template <typename TF>
 __global__ void __launch_bounds__(USBS_NT)
 k_constraint_update(int m, int mtile, int k, int kc, const TF* __restrict__ F, long long ldf,
-                    const int* __restrict__ con_ptr, const int* __restrict__ con_trip,
-                    const int* __restrict__ ti, const int* __restrict__ tj,
-                    const double* __restrict__ tdata, const double* __restrict__ eta_ptr,
+                    const int* __restrict__ con_ptr, const int* __restrict__ ct_i,
+                    const int* __restrict__ ct_j, const double* __restrict__ ct_d,
+                    const double* __restrict__ eta_ptr,
                     const double* __restrict__ zbar,
                     const double* __restrict__ y, const double* __restrict__ b,
                     const double* __restrict__ ups, const double* __restrict__ mask,
@@ -405,16 +450,15 @@ k_constraint_update(int m, int mtile, int k, int kc, const TF* __restrict__ F, l
   for (int c = c0 + (int)threadIdx.x; c < c1; c += USBS_NT) {
     double sall = 0.0, scur = 0.0;
     for (int e = con_ptr[c]; e < con_ptr[c + 1]; ++e) {
-      const int t = con_trip[e];
-      const long long i = ti[t], j = tj[t];
+      const long long i = ct_i[e], j = ct_j[e];
       double pa = 0.0, pc = 0.0;
       for (int x = 0; x < k; ++x) {
         const double pr = (double)F[i + x * ldf] * (double)F[j + x * ldf];
         pa += pr;
         if (x < kc) pc += pr;
       }
-      sall += tdata[t] * pa;
-      scur += tdata[t] * pc;
+      sall += ct_d[e] * pa;
+      scur += ct_d[e] * pc;
     }
     const double zb = zbar[c];
     const double zn = eta * zb + sall;
@@ -464,7 +508,7 @@ int launch_constraint_update(const Problem& P, int k, int kc, const TF* F, long 
                              double rho, double* z, double* zbn, double* ycand, double* partial,
                              double* out4, hipStream_t s) {
   k_constraint_update<TF><<<P.mntiles, USBS_NT, 0, s>>>(
-      P.m, P.mtile, k, kc, F, ldf, P.con_ptr, P.con_trip, P.ti, P.tj, P.tdata, eta, zbar, y, P.b,
+      P.m, P.mtile, k, kc, F, ldf, P.con_ptr, P.ct_i, P.ct_j, P.ct_d, eta, zbar, y, P.b,
       ups, P.mask, P.scale_a, rho, z, zbn, ycand, partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
@@ -577,6 +621,44 @@ k_objective(int n, int tile, int k, const int* __restrict__ row_ptr, const int* 
   block_reduce_cols<16>(acc, k, lds, lds + USBS_RCH * USBS_NT, partial + (long long)tl * 16);
 }
 
+// Heavy rows: a wavefront per row of a Lanczos row tile (lanes over the row's
+// entries, butterfly), lane 0 accumulating f_x[r] (C f_x)[r].
+template <typename T>
+__global__ void __launch_bounds__(USBS_NT)
+k_objective_wave(int n, int tile, int k, const int* __restrict__ row_ptr,
+                 const int* __restrict__ col, const double* __restrict__ cval,
+                 const T* __restrict__ F, long long ldf, double* __restrict__ partial) {
+  extern __shared__ double lds[];
+  const int tl = blockIdx.x;
+  const int r0 = tl * tile, r1 = min(n, r0 + tile);
+  const int lane = threadIdx.x & 63;
+  double acc[16];
+#pragma unroll
+  for (int x = 0; x < 16; ++x) acc[x] = 0.0;
+  for (int r = r0 + (int)(threadIdx.x >> 6); r < r1; r += USBS_NT / 64) {
+    double cf[16];
+#pragma unroll
+    for (int x = 0; x < 16; ++x) cf[x] = 0.0;
+    const int p1 = row_ptr[r + 1];
+    for (int e = row_ptr[r] + lane; e < p1; e += 64) {
+      const double v = cval[e];
+      const long long c = col[e];
+#pragma unroll
+      for (int x = 0; x < 16; ++x)
+        if (x < k) cf[x] += v * (double)F[c + x * ldf];
+    }
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+      if (x < k) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) cf[x] += __shfl_xor(cf[x], off, 64);
+        if (lane == 0) acc[x] += (double)F[r + x * ldf] * cf[x];
+      }
+    }
+  }
+  block_reduce_cols<16>(acc, k, lds, lds + USBS_RCH * USBS_NT, partial + (long long)tl * 16);
+}
+
 // out[0] = sum over all k columns, out[1] = over the first kc columns.
 __global__ void __launch_bounds__(1024) k_reduce_objective(const double* __restrict__ partial,
                                                            int ntiles, int k, int kc,
@@ -604,11 +686,15 @@ int launch_objective(const Problem& P, int k, int kc, const T* F, long long ldf,
                      double* out2, hipStream_t s) {
   if (k > 16) return (int)hipErrorInvalidValue;
   const size_t lds = sizeof(double) * (USBS_RCH * USBS_NT + USBS_RCH * USBS_NPART);
-  k_objective<T><<<P.ntiles, USBS_NT, lds, s>>>(P.n, P.tile, k, P.row_ptr, P.col, P.cval, F, ldf,
-                                                partial);
+  if (P.lz_heavy)
+    k_objective_wave<T><<<P.lz_ntiles, USBS_NT, lds, s>>>(P.n, P.lz_tile, k, P.row_ptr, P.col,
+                                                          P.cval, F, ldf, partial);
+  else
+    k_objective<T><<<P.ntiles, USBS_NT, lds, s>>>(P.n, P.tile, k, P.row_ptr, P.col, P.cval, F,
+                                                  ldf, partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
-  k_reduce_objective<<<1, 1024, 0, s>>>(partial, P.ntiles, k, kc, out2);
+  k_reduce_objective<<<1, 1024, 0, s>>>(partial, objective_ntiles(P), k, kc, out2);
   return (int)hipGetLastError();
 }
 template int launch_objective<double>(const Problem&, int, int, const double*, long long, double*,
@@ -670,8 +756,8 @@ int launch_select(int m, const double* flag, double* y, const double* ycand, hip
 template <typename TF>
 __global__ void __launch_bounds__(USBS_NT)
 k_upsilon(int m, int mtile, int k, const TF* __restrict__ F, long long ldf,
-          const int* __restrict__ con_ptr, const int* __restrict__ con_trip,
-          const int* __restrict__ ti, const int* __restrict__ tj, const double* __restrict__ tdata,
+          const int* __restrict__ con_ptr, const int* __restrict__ ct_i,
+          const int* __restrict__ ct_j, const double* __restrict__ ct_d,
           const double* __restrict__ eta_ptr, const double* __restrict__ zbar,
           const double* __restrict__ y, const double* __restrict__ b,
           const double* __restrict__ mask, double rho, double* __restrict__ ups,
@@ -684,11 +770,10 @@ k_upsilon(int m, int mtile, int k, const TF* __restrict__ F, long long ldf,
   for (int c = c0 + (int)threadIdx.x; c < c1; c += USBS_NT) {
     double sall = 0.0;
     for (int e = con_ptr[c]; e < con_ptr[c + 1]; ++e) {
-      const int t = con_trip[e];
-      const long long i = ti[t], j = tj[t];
+      const long long i = ct_i[e], j = ct_j[e];
       double pa = 0.0;
       for (int x = 0; x < k; ++x) pa += (double)F[i + x * ldf] * (double)F[j + x * ldf];
-      sall += tdata[t] * pa;
+      sall += ct_d[e] * pa;
     }
     const double zn = eta * zbar[c] + sall;
     double u = b[c] - zn - rho * y[c];
@@ -718,8 +803,8 @@ template <typename TF>
 int launch_upsilon(const Problem& P, int k, const TF* F, long long ldf, const double* eta,
                    const double* zbar, const double* y, double rho, double* ups, double* partial,
                    double* out, hipStream_t s) {
-  k_upsilon<TF><<<P.mntiles, USBS_NT, 0, s>>>(P.m, P.mtile, k, F, ldf, P.con_ptr, P.con_trip, P.ti,
-                                              P.tj, P.tdata, eta, zbar, y, P.b, P.mask, rho, ups,
+  k_upsilon<TF><<<P.mntiles, USBS_NT, 0, s>>>(P.m, P.mtile, k, F, ldf, P.con_ptr, P.ct_i,
+                                              P.ct_j, P.ct_d, eta, zbar, y, P.b, P.mask, rho, ups,
                                               partial);
   int rc = (int)hipGetLastError();
   if (rc) return rc;

@@ -69,6 +69,8 @@ struct Solver {
   size_t cache_bytes = 0;            // LDS per persistent workgroup when tiles are cached
   int host_ctrl[LZ_NCTRL] = {0};
   int lz_restarts = 0;
+  int lz_hint = 2;        // restarts the last persistent call needed: its first early-stop check
+  int tev_launched = 0;   // restarts launched by the last timed persistent call
   hipStream_t s2 = nullptr;          // side stream: the lower-bound IPM overlaps the Lanczos call
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<hipEvent_t> tev;       // timing: 2 events per persistent launch
@@ -88,7 +90,8 @@ struct Solver {
   int collect_timing() {
     if (!tev_pending) return USBS_OK;
     tev_pending = false;
-    const int R = lz.max_restarts;
+    const int R = tev_launched;
+    if (R <= 0) return USBS_OK;
     if (hipEventSynchronize(tev[2 * R - 1]) != hipSuccess ||
         hipMemcpy(lstat_h.data(), lz.lstat, sizeof(int) * 2 * R, hipMemcpyDeviceToHost) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "reading Lanczos timing");
@@ -139,7 +142,11 @@ struct Solver {
     if (!(PO = alloc<T>(ld * 2 * rr)) || !(Pbar = alloc<T>(ld * rr))) return USBS_ERR_OOM;
     if (!(op64 = alloc<double>(P->nnz))) return USBS_ERR_OOM;
     if (sizeof(T) == 4 && !(op32 = alloc<float>(P->nnz))) return USBS_ERR_OOM;
-    const size_t pw = (size_t)ntl * (size_t)std::max(d * d, std::max(2 * rr * rr, std::max(k * k, 16 * 4)));
+    const int qt = qbase_tile_rows(m, d);
+    const size_t pw = std::max(
+        std::max((size_t)P->ntiles * (size_t)std::max(std::max(k * k, 2 * rr * rr), std::max(k * rr, 16)),
+                 (size_t)P->mntiles * 4),
+        std::max((size_t)((m + qt - 1) / qt) * d * d, (size_t)objective_ntiles(*P) * 16));
     if (!(partial = alloc<double>(pw)) || !(dpart = alloc<double>((size_t)ntl * rr)) ||
         !(ipart = alloc<long long>((size_t)ntl * rr)))
       return USBS_ERR_OOM;
@@ -216,9 +223,13 @@ struct Solver {
         const int rc = collect_timing();
         if (rc) return rc;
       }
+      int* ctrl_ptr = lz.ctrl;
+      int launched = 0;
+      const LzStop stop{&ctrl_ptr, lz_hint, &launched};
       CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
                                d_map + 2 * nb, lz.ntiles, 1ull << 26, cache_bytes,
-                               tev.empty() ? nullptr : tev.data()));
+                               tev.empty() ? nullptr : tev.data(), lz.tile > 256 ? 1 : 0, &stop));
+      tev_launched = launched;
       tev_pending = !tev.empty();
     }
     return USBS_OK;
@@ -231,7 +242,7 @@ struct Solver {
 
   // out (k x k) = V^T M V for the operator currently in op64.
   int project(double* out, hipStream_t st) {
-    CK((launch_spmm<double, T>(n, P->row_ptr, P->col, op64, V(), ld, k, W, ld, st)));
+    CK((launch_spmm<double, T>(n, P->row_ptr, P->col, op64, V(), ld, k, W, ld, P->lz_heavy, st)));
     CK((launch_tsgram<T, double>(n, P->tile, P->ntiles, V(), ld, k, W, ld, k, partial, out, st)));
     return USBS_OK;
   }
@@ -339,6 +350,7 @@ struct Solver {
         return set_err(c, USBS_ERR_HIP, "reading Lanczos control");
       if (ctrl[LZ_ERROR]) return set_err(c, USBS_ERR_TIMEOUT, "persistent Lanczos barrier wait expired");
       lz_restarts = ctrl[LZ_RESTART];
+      lz_hint = lz_restarts > 1 ? lz_restarts : 1;
     }
     const double scx = p.SCALE_C * p.SCALE_X;
     rec.t = h[SC_T];
@@ -583,7 +595,8 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   if (!h || !prm) return USBS_ERR_INVALID;
   Ctx* c = &h->c;
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
-  const usbs_params& p = *prm;
+  usbs_params p = *prm;
+  if (p.exec_mode == USBS_EXEC_AUTO) p.exec_mode = resolve_exec_mode(c->P);
   if (!q0 || (!p.warm && (!y0 || !z0))) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   const int k = p.k_curr + p.k_past;
   if (p.k_curr < 1 || p.k_past < 0 || k > 11)

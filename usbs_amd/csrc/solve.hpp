@@ -50,9 +50,10 @@ enum Sc {
 int launch_assemble(const Problem& P, double coefC, const double* z1, double a1, const double* z2,
                     double a2, const double* z3, double a3, double* out64, float* out32,
                     hipStream_t s);
+// heavy != 0: a wavefront per row (operators whose rows average 32+ entries)
 template <typename TV, typename TX>
 int launch_spmm(int n, const int* row_ptr, const int* col, const TV* val, const TX* X,
-                long long ldx, int p, double* Y, long long ldy, hipStream_t s);
+                long long ldx, int p, double* Y, long long ldy, int heavy, hipStream_t s);
 int launch_reduce(const double* partial, int ntiles, int count, double scale, double* out,
                   hipStream_t s);
 template <typename TA, typename TB>
@@ -61,6 +62,10 @@ int launch_tsgram(int n, int tile, int ntiles, const TA* A, long long lda, int p
 template <typename TA, typename TC>
 int launch_tsmm(int n, const TA* A, long long lda, int p, const double* B, int q, TC* C,
                 long long ldc, double beta, const TC* C0, long long ldc0, hipStream_t s);
+// Constraint tiles of the Q_base Gram (d = k(k+1)/2): 256 constraints,
+// doubled while the d x d partials of more than 256 tiles would exceed 2^20
+// doubles.
+int qbase_tile_rows(int m, int d);
 template <typename TV>
 int launch_qbase(const Problem& P, int k, const TV* V, long long ldv, double* partial, double* out,
                  hipStream_t s);

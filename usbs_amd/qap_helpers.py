@@ -1,0 +1,157 @@
+"""The lifted QAP SDP, mirroring utils/qap_helpers.py:157-381.
+
+A QAP with flow matrix W and distance matrix D (both l x l) is lifted to an
+SDP over X of order N = l^2 + 1: row/column 0 is the homogenising border,
+entry 1 + l*a + i stands for "facility a sits at location i".  The problem
+build is host-side setup, as in the reference (which builds it with JAX and
+scales it with scipy); the solve runs on the device through the same
+usbs()/cgal() entry points as max-cut.
+
+Layout facts the device operator relies on (and that the reference fixes):
+
+* ``C[1 + l*a + i, 1 + l*b + j] = W[a, b] * D[i, j]``, entries in the order
+  W's nonzeros (row-major) x D's nonzeros (row-major)
+  (utils/qap_helpers.py:157-172), negated by initialize_state.
+* constraint rows, in this order (utils/qap_helpers.py:175-318):
+  X00 = 1 | diag(Y) = vec(B) (N-1 rows) | B1 = 1 (l) | 1'B = 1' (l) |
+  tr_1(Y) = I (l^2) | tr_2(Y) = I (l^2) | -Y <= 0 on C's upper triangle.
+"""
+import numpy as np
+import scipy.sparse
+import scipy.sparse.linalg
+
+from . import problems
+from .common import SDPState, scale_sdp_state
+from .sparse import BCOO
+
+
+def build_objective_matrix(D, W):
+    """C = W (x) D padded by one leading row and column
+    (utils/qap_helpers.py:157-172).  D, W: dense l x l arrays; zeros are
+    not stored."""
+    D = np.asarray(D, dtype=np.float64)
+    W = np.asarray(W, dtype=np.float64)
+    l = D.shape[0]
+    wi, wj = np.nonzero(W)
+    di, dj = np.nonzero(D)
+    rows = 1 + (l * wi)[:, None] + di[None, :]
+    cols = 1 + (l * wj)[:, None] + dj[None, :]
+    data = W[wi, wj][:, None] * D[di, dj][None, :]
+    idx = np.stack([rows.ravel(), cols.ravel()], axis=1)
+    return BCOO((data.ravel(), idx), shape=(l * l + 1, l * l + 1))
+
+
+def _pairs_with_border(cons, v):
+    """Triples (c, v, 0), (c, 0, v) interleaved per entry, data 1 each."""
+    t = np.zeros((2 * v.size, 3), dtype=np.int64)
+    t[0::2, 0] = cons
+    t[1::2, 0] = cons
+    t[0::2, 1] = v
+    t[1::2, 2] = v
+    return t
+
+
+def get_all_problem_data(C):
+    """(A_data, A_indices, b, b_ineq_mask) of the lifted QAP for a cost
+    matrix C of order l^2 + 1 (utils/qap_helpers.py:175-318)."""
+    N = C.shape[0]
+    l = int(round(np.sqrt(N - 1)))
+    if l * l + 1 != N:
+        raise ValueError("C must have order l^2 + 1")
+    trip, data, b, mask = [], [], [], []
+
+    def add(t, d, bb, mm):
+        trip.append(np.asarray(t, dtype=np.int64).reshape(-1, 3))
+        data.append(np.asarray(d, dtype=np.float64))
+        b.append(np.asarray(bb, dtype=np.float64))
+        mask.append(np.asarray(mm, dtype=np.float64))
+
+    base = 0
+    # X(0, 0) = 1
+    add([[0, 0, 0]], [1.0], [1.0], [0.0])
+    base += 1
+    # diag(Y) = vec(B): -X[j,0] - X[0,j] + 2 X[j,j] = 0 for j = 1..N-1
+    j = np.arange(1, N)
+    t = np.zeros((3 * (N - 1), 3), dtype=np.int64)
+    t[:, 0] = base + np.repeat(np.arange(N - 1), 3)
+    t[0::3, 1] = j
+    t[1::3, 2] = j
+    t[2::3, 1] = j
+    t[2::3, 2] = j
+    add(t, np.tile([-1.0, -1.0, 2.0], N - 1), np.zeros(N - 1), np.zeros(N - 1))
+    base += N - 1
+    # B 1 = 1 (row a sums entries 1 + l*a + i over i) and 1'B = 1' (column i)
+    grid = np.arange(1, N).reshape(l, l)
+    for order in (grid, grid.T):
+        v = order.ravel()
+        add(_pairs_with_border(base + np.repeat(np.arange(l), l), v), np.ones(2 * (N - 1)),
+            2.0 * np.ones(l), np.zeros(l))
+        base += l
+    # tr_1(Y) = I: row (j1, j2) sums Y[(d, j1), (d, j2)] over blocks d;
+    # tr_2(Y) = I: row (j1, j2) sums Y[(j1, d), (j2, d)] over inner index d.
+    j1, j2, d = np.meshgrid(np.arange(l), np.arange(l), np.arange(l), indexing="ij")
+    for ca, cb in ((1 + j1 + l * d, 1 + j2 + l * d), (1 + l * j1 + d, 1 + l * j2 + d)):
+        cons = base + (j1 * l + j2).ravel()
+        t1 = np.stack([cons, ca.ravel(), cb.ravel()], axis=1)
+        t2 = np.stack([cons, cb.ravel(), ca.ravel()], axis=1)
+        add(np.concatenate([t1, t2]), np.ones(2 * t1.shape[0]),
+            2.0 * (np.arange(l)[:, None] == np.arange(l)[None, :]).ravel(), np.zeros(l * l))
+        base += l * l
+    # -Y <= 0 on the entries of C's upper triangle (diagonal included)
+    ci = np.asarray(C.indices)
+    up = ci[ci[:, 0] <= ci[:, 1]]
+    cons = base + np.arange(up.shape[0])
+    t1 = np.stack([cons, up[:, 0], up[:, 1]], axis=1)
+    t2 = np.stack([cons, up[:, 1], up[:, 0]], axis=1)
+    add(np.concatenate([t1, t2]), np.full(2 * up.shape[0], -0.5), np.zeros(up.shape[0]),
+        np.ones(up.shape[0]))
+    return (np.concatenate(data), np.concatenate(trip), np.concatenate(b), np.concatenate(mask))
+
+
+def constraint_norm(A_data, A_indices, m, N):
+    """Spectral norm of the m x N^2 constraint matrix (duplicate triples
+    summed), as sqrt of the largest eigenvalue of A A^T by Lanczos
+    (scipy's eigsh, as utils/qap_helpers.py:333-336)."""
+    A = scipy.sparse.coo_matrix((A_data, (A_indices[:, 0], A_indices[:, 1] * N + A_indices[:, 2])),
+                                shape=(m, N * N)).tocsr()
+    G = (A @ A.T).tocsr()
+    if m <= 3:
+        return float(np.sqrt(np.linalg.eigvalsh(G.toarray())[-1]))
+    lam = scipy.sparse.linalg.eigsh(G, k=1, which="LM", return_eigenvectors=False,
+                                    maxiter=np.iinfo(np.int32).max)[0]
+    return float(np.sqrt(lam))
+
+
+def initialize_state(C, sketch_dim, omega_seed=0, Omega=None):
+    """Cold start of the lifted QAP (utils/qap_helpers.py:321-381): negate C,
+    SCALE_X = 1/(l+1), SCALE_C = 1/||C||_F, SCALE_A = row norms of A divided
+    by ||A||_2, then scale.  Omega is a seeded standard normal N x sketch_dim
+    matrix (the reference draws it from jax.random with PRNGKey(0))."""
+    C = -C
+    A_data, A_indices, b, b_ineq_mask = get_all_problem_data(C)
+    N = C.shape[0]
+    m = b.shape[0]
+    l = int(round(np.sqrt(N - 1)))
+    SCALE_X = 1.0 / float(l + 1)
+    SCALE_C = 1.0 / float(np.linalg.norm(C.data))
+    SCALE_A = 1.0 / np.sqrt(np.bincount(A_indices[:, 0], weights=A_data ** 2, minlength=m))
+    SCALE_A = SCALE_A / constraint_norm(SCALE_A[A_indices[:, 0]] * A_data, A_indices, m, N)
+    if sketch_dim == -1:
+        raise NotImplementedError("dense-X storage (sketch_dim=-1) is not provided by the HIP backend")
+    elif sketch_dim > 0:
+        if Omega is None:
+            Omega = problems.gaussian_matrix(N, sketch_dim, omega_seed)
+        P = np.zeros_like(Omega)
+    else:
+        raise ValueError("Invalid value for sketch_dim")
+    s = SDPState(C=C, A_indices=A_indices, A_data=A_data, b=b, b_ineq_mask=b_ineq_mask, X=None,
+                 P=P, Omega=Omega, y=np.zeros(m), z=np.zeros(m), tr_X=0.0, primal_obj=0.0,
+                 SCALE_C=SCALE_C, SCALE_X=SCALE_X, SCALE_A=SCALE_A)
+    return scale_sdp_state(s)
+
+
+def qap_objective(D, W, perm):
+    """sum_{a,b} W[a,b] D[perm[a], perm[b]]: the QAP cost of placing
+    facility a at location perm[a]."""
+    perm = np.asarray(perm)
+    return float(np.sum(np.asarray(W) * np.asarray(D)[np.ix_(perm, perm)]))
