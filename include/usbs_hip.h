@@ -35,6 +35,7 @@ extern "C" {
 #define USBS_PREC_FP64 0  /* plain FP64 everywhere                                  */
 #define USBS_PREC_MIXED 1 /* FP32 storage for Lanczos / sketch, FP64 accumulation,   */
                           /* FP64 Ritz solve, IPM, duals and convergence test       */
+#define USBS_PREC_BF16 2  /* usbs_qap_kron_matvec only: BF16 operands, FP32 accum.  */
 
 #define USBS_EXEC_PEROP 0      /* one kernel per phase, host reads control per restart */
 #define USBS_EXEC_PERSISTENT 1 /* one cooperative launch per Lanczos call            */
@@ -96,6 +97,22 @@ int usbs_lanczos_smallest_batch(usbs_ctx* const* ctxs, int ninst, double C_coef,
                                 int num_desired, int inner_iterations, int max_restarts,
                                 double tolerance, int precision, double* const* eigvals,
                                 double* const* eigvecs, int* info);
+
+/*
+ * The lifted QAP's cost operator on a batch of vectors: U_b = W Y_b D^T for
+ * b < nvec, i.e. C v for C = W (x) D padded by an empty leading row and
+ * column (utils/qap_helpers.py:157-172, the cost whose matvec the reference
+ * runs inside the operator at solver/lanczos.py:86), with Y_b[c][j] =
+ * v_b[1 + l c + j] and (C v_b)[1 + l a + i] = U_b[a][i], (C v_b)[0] = 0.
+ * W, D: l x l row-major; Y, U: nvec x l x l.  precision USBS_PREC_BF16: BF16
+ * operands on the matrix cores with FP32 accumulation (l <= 64);
+ * USBS_PREC_FP64: FP64 FMA (l <= 48).  The product runs reps times on
+ * device-resident inputs; kernel_ms (may be NULL) receives the mean time of
+ * one launch.
+ */
+int usbs_qap_kron_matvec(usbs_ctx* ctx, int32_t l, const double* W, const double* D,
+                         int64_t nvec, const double* Y, int32_t precision, int32_t reps,
+                         double* U, double* kernel_ms);
 
 /* Parameters of usbs() (solver/usbs.py:436-459), plus the execution choices. */
 typedef struct usbs_params {

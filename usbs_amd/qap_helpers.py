@@ -16,11 +16,13 @@ Layout facts the device operator relies on (and that the reference fixes):
   X00 = 1 | diag(Y) = vec(B) (N-1 rows) | B1 = 1 (l) | 1'B = 1' (l) |
   tr_1(Y) = I (l^2) | tr_2(Y) = I (l^2) | -Y <= 0 on C's upper triangle.
 """
+import ctypes
+
 import numpy as np
 import scipy.sparse
 import scipy.sparse.linalg
 
-from . import problems
+from . import _lib, problems
 from .common import SDPState, scale_sdp_state
 from .sparse import BCOO
 
@@ -155,3 +157,33 @@ def qap_objective(D, W, perm):
     facility a at location perm[a]."""
     perm = np.asarray(perm)
     return float(np.sum(np.asarray(W) * np.asarray(D)[np.ix_(perm, perm)]))
+
+
+_lib.EXTRA_SIGNATURES["usbs_qap_kron_matvec"] = [
+    ctypes.c_void_p, ctypes.c_int32, _lib._dp, _lib._dp, ctypes.c_int64, _lib._dp, ctypes.c_int32,
+    ctypes.c_int32, _lib._dp, _lib._dp]
+
+KRON_PRECISION = {"fp64": 0, "bf16": 2}
+
+
+def kron_matvec(D, W, Y, precision="bf16", reps=1, context=None, device=0, return_time=False):
+    """C v for a batch of vectors, C = W (x) D padded (build_objective_matrix),
+    on the device: Y is (nvec, l, l) (or (nvec, l*l)) with Y[b, c, j] =
+    v_b[1 + l c + j]; returns U (nvec, l, l) with (C v_b)[1 + l a + i] =
+    U[b, a, i] ((C v_b)[0] = 0).  precision "bf16": BF16 operands on the
+    matrix cores, FP32 accumulation; "fp64": FP64 FMA.  With return_time
+    also the mean kernel time (ms) of ``reps`` launches on device-resident
+    inputs."""
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    l = D.shape[0]
+    if D.shape != (l, l) or W.shape != (l, l):
+        raise ValueError("D and W must be l x l")
+    Y = np.ascontiguousarray(Y, dtype=np.float64).reshape(-1, l, l)
+    U = np.zeros_like(Y)
+    ms = ctypes.c_double(0.0)
+    ctx = context if context is not None else _lib.Context(device)
+    ctx.check(_lib.lib().usbs_qap_kron_matvec(ctx.h, l, _lib.dptr(W), _lib.dptr(D), Y.shape[0],
+                                              _lib.dptr(Y), KRON_PRECISION[precision], int(reps),
+                                              _lib.dptr(U), ctypes.byref(ms)))
+    return (U, ms.value) if return_time else U
