@@ -35,6 +35,17 @@ import numpy as np  # noqa: E402
 METRIC = "time-to-1e-5 rel gap: mixed/batched vs plain FP64 on GPU; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lz_steps_coop_torus100x200_traffic.txt")
+KRON_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_kron_bf16_qap48_8192vec_traffic.txt")
+
+
+def traffic_from(path):
+    """traffic_bytes_per_launch recorded in a counter summary, or None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
 
 SOLVER = dict(trace_factor=2.0, rho=0.01, beta=0.25, k_curr=10, k_past=1, lanczos_inner=32,
               lanczos_restarts=10, subprob_eps=1e-7, subprob_max_iters=100, sketch_dim=10)
@@ -139,11 +150,8 @@ def roofline(stats):
     per_launch = total / L
     avg_s = stats["lz_kernel_ms"] / L / 1e3
     achieved = per_launch / avg_s / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch"):
-                traffic = float(line.split("=")[1])
+    # counters were taken on the torus workload in FP32 storage; others report None
+    traffic = traffic_from(TRAFFIC_FILE) if (n, nnz, s) == (20000, 100000, 4) else None
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "kernel": "k_lz_steps_coop (all Lanczos steps of one restart)",
@@ -203,7 +211,8 @@ def kron_bench(args):
            "mfma": {"achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5)},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "traffic": traffic_from(KRON_TRAFFIC_FILE) if nvec == 8192 else None,
                         "kernel": "k_kron_bf16 (W Y D^T per vector)",
                         "avg_launch_us": round(ms16 * 1e3, 3),
                         "algorithmic_bytes_per_launch": int(bytes16)}}

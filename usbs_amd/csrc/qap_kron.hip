@@ -311,7 +311,9 @@ extern "C" int usbs_qap_kron_matvec(usbs_ctx* h, int32_t l, const double* W, con
     if (hipGetLastError() != hipSuccess) return set_err(c, USBS_ERR_HIP, "conversion launch failed");
     kU = dY;
   }
-  if (hipMemcpy(U, kU, sizeof(double) * ny, hipMemcpyDeviceToHost) != hipSuccess)
+  // the context's stream does not synchronise with the null stream: copy on it
+  if (hipMemcpyAsync(U, kU, sizeof(double) * ny, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
     return set_err(c, USBS_ERR_HIP, "download failed");
   return USBS_OK;
 }
