@@ -39,8 +39,8 @@ extern "C" {
 
 #define USBS_EXEC_PEROP 0      /* one kernel per phase, host reads control per restart */
 #define USBS_EXEC_PERSISTENT 1 /* one cooperative launch per Lanczos call            */
-#define USBS_EXEC_AUTO 2       /* per-op for operators whose rows average 32+ entries,
-                                  persistent otherwise                               */
+#define USBS_EXEC_AUTO 2       /* persistent where every row tile fits in LDS for a
+                                  restart (light rows, <= 256-row tiles), else per-op */
 
 typedef struct usbs_ctx usbs_ctx;
 

@@ -54,9 +54,15 @@ struct Problem {
   double* scale_a = nullptr;    // m
 };
 
-// USBS_EXEC_AUTO: heavy-row operators run the per-op Lanczos (their short
-// row tiles make device-wide barriers cost more than kernel launches).
-inline int resolve_exec_mode(const Problem& P) { return P.lz_heavy ? 0 : 1; }
+int lanczos_prefers_persistent(int tile, int heavy, int m, int tile_nnz_max, int value_bytes);
+// USBS_EXEC_AUTO: the persistent form where it keeps each tile in LDS for the
+// whole restart (light rows, small tiles), per-op launches otherwise (heavy
+// rows, and tiles streamed from HBM, where the device-wide barriers cost more
+// than launches).  m: Lanczos basis size; precision: USBS_PREC_*.
+inline int resolve_exec_mode(const Problem& P, int m, int precision) {
+  return lanczos_prefers_persistent(P.lz_tile, P.lz_heavy, m, P.tile_nnz_max,
+                                    precision == 1 ? 4 : 8);
+}
 
 // Row tiles of the objective's partials: the Lanczos tiles for heavy rows.
 inline int objective_ntiles(const Problem& P) { return P.lz_heavy ? P.lz_ntiles : P.ntiles; }

@@ -805,6 +805,13 @@ size_t lanczos_cache_bytes(const LzInst<T>& L) {
 template size_t lanczos_cache_bytes<double>(const LzInst<double>&);
 template size_t lanczos_cache_bytes<float>(const LzInst<float>&);
 
+int lanczos_prefers_persistent(int tile, int heavy, int m, int tile_nnz_max, int value_bytes) {
+  if (heavy || tile > 256) return 0;
+  const size_t b = value_bytes == 4 ? lz_cache_bytes<float>(tile, m, tile_nnz_max)
+                                    : lz_cache_bytes<double>(tile, m, tile_nnz_max);
+  return b <= LZ_CACHE_MAX_BYTES ? 1 : 0;
+}
+
 template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,

@@ -91,6 +91,10 @@ template <typename T>
 int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cache_bytes);
 template <typename T>
 size_t lanczos_cache_bytes(const LzInst<T>& L);
+// 1 when the persistent form would keep every tile in LDS for a restart (the
+// case where it beats per-op launches): light rows, tiles of at most 256
+// rows, cache within LZ_CACHE_MAX_BYTES.
+int lanczos_prefers_persistent(int tile, int heavy, int m, int tile_nnz_max, int value_bytes);
 int lanczos_max_coresident_blocks(size_t lds_bytes, int device);
 // LDS of a persistent workgroup that does not cache its tile (basis size m).
 size_t lanczos_tile_lds_bytes(int m);

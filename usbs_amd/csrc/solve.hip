@@ -596,7 +596,8 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   Ctx* c = &h->c;
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
   usbs_params p = *prm;
-  if (p.exec_mode == USBS_EXEC_AUTO) p.exec_mode = resolve_exec_mode(c->P);
+  if (p.exec_mode == USBS_EXEC_AUTO)
+    p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision);
   if (!q0 || (!p.warm && (!y0 || !z0))) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   const int k = p.k_curr + p.k_past;
   if (p.k_curr < 1 || p.k_past < 0 || k > 11)

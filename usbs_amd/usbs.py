@@ -10,8 +10,8 @@ after the reference's select the execution:
   precision "fp64" (plain FP64) or "mixed" (FP32 Lanczos / sketch storage,
             FP64 accumulation, Ritz solve, IPM, duals and convergence test)
   exec_mode "perop" (a kernel per Lanczos phase), "persistent" (one
-            cooperative launch per Lanczos restart) or "auto" (per-op for
-            operators whose rows average 32+ entries, else persistent)
+            cooperative launch per Lanczos restart) or "auto" (persistent
+            where every row tile stays in LDS for a restart, else per-op)
   q0        Lanczos start vector (default: seeded normal, normalised; the
             reference draws it from jax.random with PRNGKey(0))
   log       a list that receives one dict per condition check
