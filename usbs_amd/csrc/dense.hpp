@@ -380,4 +380,324 @@ __device__ __forceinline__ void dense_svec_inv(const double* v, int k, double* M
   }
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric eigendecomposition for m <= 32 by Householder tridiagonalisation
+// (the reflectors accumulated into Q as they are formed), multisection on
+// Sturm counts for every eigenvalue, eigenvectors of the tridiagonal from
+// twisted factorisations, Gram-Schmidt (two passes) inside clusters
+// (consecutive eigenvalues closer than 1e-3 |T|), and Y = Q Z.  Eigenvalues
+// come out in ascending order by construction; each is bracketed to
+// max(2 eps |lambda|, eps |T|).  The code is written for latency: small
+// reductions are done redundantly by every thread from LDS broadcasts
+// rather than with cross-lane shuffles, and the Sturm and twisted-
+// factorisation chains run on a register copy of the tridiagonal with a
+// Newton-refined reciprocal.
+//   A  : m x lda in LDS (lda >= m), symmetric; destroyed.
+//   Qw : m x lda LDS scratch (the accumulated orthogonal factor).
+//   w  : m eigenvalues ascending;  Y: m x m row-major, column j = eigenvector
+//        of w[j].
+//   scr: tridiag_scratch_doubles(m, blockDim.x) doubles of LDS.
+// Called by all threads of the block (blockDim 256).  Returns 1 (in every
+// thread) when a cluster is degenerate -- a vector loses more than half its
+// norm to the Gram-Schmidt projection, as for an exactly repeated eigenvalue
+// -- and the caller must use another eigensolver; 0 otherwise.
+#define TD_MAXM 32
+__host__ __device__ constexpr int tridiag_scratch_doubles(int m, int nt) {
+  return 9 * m + 64 + (nt + 1) / 2 + 3 * m * m;
+}
+
+__device__ __forceinline__ double td_guard(double q, double pivmin) {
+  return fabs(q) < pivmin ? -pivmin : q;
+}
+
+// 1 / q: hardware reciprocal refined by one Newton step.
+__device__ __forceinline__ double td_rcp(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  const double e = fma(-q, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Number of eigenvalues of the tridiagonal (register copy dr, e2r; entries
+// at and past m ignored) below x.  Straight-line code: steps past m keep q.
+__device__ __forceinline__ int td_sturm(const double (&dr)[TD_MAXM], const double (&e2r)[TD_MAXM],
+                                        int m, double x, double pivmin) {
+  double q = td_guard(dr[0] - x, pivmin);
+  int c = q < 0.0 ? 1 : 0;
+#pragma unroll
+  for (int i = 1; i < TD_MAXM; ++i) {
+    const double qn = td_guard((dr[i] - x) - e2r[i - 1] * td_rcp(q), pivmin);
+    const bool in = i < m;
+    c += (in && qn < 0.0) ? 1 : 0;
+    q = in ? qn : q;
+  }
+  return c;
+}
+
+__device__ int tridiag_eigh(double* A, int lda, int m, double* w, double* Y, double* Qw,
+                            double* scr) {
+  constexpr int MT = TD_MAXM;
+  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63;
+  double* d = scr;           // diagonal
+  double* e = d + m;         // e[i] couples i and i + 1
+  double* e2 = e + m;
+  double* hv = e2 + m;       // current reflector, by absolute row (hv[j+1] = 1)
+  double* p = hv + m;        // A22 v by absolute row, then Gram-Schmidt coefficients
+  double* lo = p + m;
+  double* hi = lo + m;
+  double* misc = hi + m;     // 2m + 64 doubles: [0] tnorm, [1] pivmin, [2] tau, [8..8+m) Q v,
+                             // [8+m..8+2m) p*v
+  int* cnt = (int*)(misc + 2 * m + 64);                  // nt ints
+  double* dp = misc + 2 * m + 64 + (nt + 1) / 2;         // m x m: D+ of eigenvector j (row j)
+  double* dm = dp + m * m;                               // m x m: D-
+  double* Z = dm + m * m;                                // m x m: eigenvectors of the tridiagonal
+  double* qv = misc + 8;
+  double* pvp = qv + m;     // p_r v_r of the current step
+  for (int idx = t; idx < m * m; idx += nt) {
+    const int r = idx / m, c = idx - (idx / m) * m;
+    Qw[r * lda + c] = r == c ? 1.0 : 0.0;
+  }
+  // 1. Householder: step j maps A[j+1.., j] to (|x|, 0, ...), Q <- Q H_j
+  for (int j = 0; j + 2 < m; ++j) {
+    const int n1 = m - j - 1;
+    __syncthreads();
+    if (t < 64) {
+      const double xi = lane < n1 ? A[(j + 1 + lane) * lda + j] : 0.0;
+      const double sg = wave_sum((lane >= 1 && lane < n1) ? xi * xi : 0.0);
+      const double x0 = __shfl(xi, 0, 64);
+      double v0 = 1.0, mu = x0, tj = 0.0;
+      if (sg != 0.0) {
+        mu = sqrt(x0 * x0 + sg);
+        v0 = x0 <= 0.0 ? x0 - mu : -sg / (x0 + mu);
+        tj = 2.0 * v0 * v0 / (sg + v0 * v0);
+      }
+      if (lane < n1) hv[j + 1 + lane] = lane == 0 ? 1.0 : (sg != 0.0 ? xi / v0 : 0.0);
+      if (lane == 0) {
+        misc[2] = tj;
+        e[j] = mu;
+        d[j] = A[j * lda + j];
+      }
+    }
+    __syncthreads();
+    const double tj = misc[2];
+    if (tj == 0.0) continue;  // uniform
+    {
+      // p = tau A22 v: 8 threads per row
+      const int g = t >> 3, sub = t & 7;
+      const bool isp = g < n1;
+      double s = 0.0;
+      if (isp)
+        for (int c = sub; c < n1; c += 8) s = fma(A[(j + 1 + g) * lda + j + 1 + c], hv[j + 1 + c], s);
+      s += __shfl_xor(s, 4, 8);
+      s += __shfl_xor(s, 2, 8);
+      s += __shfl_xor(s, 1, 8);
+      if (isp && sub == 0) {
+        p[j + 1 + g] = tj * s;
+        pvp[g] = tj * s * hv[j + 1 + g];
+      }
+    }
+    {
+      // (Q v)_i: 8 threads per row of Q
+      const int g = t >> 3, sub = t & 7;
+      double s = 0.0;
+      if (g < m)
+        for (int c = sub; c < n1; c += 8) s = fma(Qw[g * lda + j + 1 + c], hv[j + 1 + c], s);
+      s += __shfl_xor(s, 4, 8);
+      s += __shfl_xor(s, 2, 8);
+      s += __shfl_xor(s, 1, 8);
+      if (g < m && sub == 0) qv[g] = s;
+    }
+    __syncthreads();
+    double K0 = 0.0, K1 = 0.0, K2 = 0.0, K3 = 0.0;
+#pragma unroll
+    for (int c = 0; c < MT; c += 4) {
+      K0 += c < n1 ? pvp[c < n1 ? c : 0] : 0.0;
+      K1 += c + 1 < n1 ? pvp[c + 1 < n1 ? c + 1 : 0] : 0.0;
+      K2 += c + 2 < n1 ? pvp[c + 2 < n1 ? c + 2 : 0] : 0.0;
+      K3 += c + 3 < n1 ? pvp[c + 3 < n1 ? c + 3 : 0] : 0.0;
+    }
+    const double K = 0.5 * tj * ((K0 + K1) + (K2 + K3));
+    for (int idx = t; idx < n1 * n1; idx += nt) {
+      const int r = j + 1 + idx / n1, c = j + 1 + idx - (idx / n1) * n1;
+      const double wr = p[r] - K * hv[r], wc = p[c] - K * hv[c];
+      A[r * lda + c] -= hv[r] * wc + wr * hv[c];
+    }
+    for (int idx = t; idx < m * n1; idx += nt) {
+      const int i = idx / n1, c = j + 1 + idx - (idx / n1) * n1;
+      Qw[i * lda + c] -= tj * qv[i] * hv[c];
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (m >= 2) {
+      d[m - 2] = A[(m - 2) * lda + m - 2];
+      e[m - 2] = A[(m - 1) * lda + m - 2];
+    }
+    d[m - 1] = A[(m - 1) * lda + m - 1];
+    e[m - 1] = 0.0;
+    double gl = 0.0, gu = 0.0, emax = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < m ? fabs(e[i]) : 0.0);
+      gl = (i == 0 || d[i] - r < gl) ? d[i] - r : gl;
+      gu = (i == 0 || d[i] + r > gu) ? d[i] + r : gu;
+      e2[i] = e[i] * e[i];
+      emax = e2[i] > emax ? e2[i] : emax;
+    }
+    const double tnorm = fmax(fabs(gl), fabs(gu));
+    const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+    const double fudge = 2.1 * 2.220446049250313e-16 * tnorm * m + 4.0 * pivmin;
+    misc[0] = tnorm;
+    misc[1] = pivmin;
+    for (int i = 0; i < m; ++i) {
+      lo[i] = gl - fudge;
+      hi[i] = gu + fudge;
+    }
+  }
+  __syncthreads();
+  const double tnorm = misc[0], pivmin = misc[1];
+  const double eps = 2.220446049250313e-16;
+  double dr[MT], e2r[MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) {
+    dr[i] = i < m ? d[i] : 0.0;
+    e2r[i] = i < m ? e2[i] : 0.0;
+  }
+  // 2. every eigenvalue by multisection: P points per eigenvalue per round
+  {
+    int P = nt / m;
+    if (P > 32) P = 32;
+    const int jj = t / P, pp = t - (t / P) * P;
+    for (int it = 0; it < 80; ++it) {
+      if (jj < m) {
+        const double l0 = lo[jj], h0 = hi[jj];
+        const double h = (h0 - l0) / (P + 1);
+        cnt[t] = td_sturm(dr, e2r, m, l0 + (pp + 1) * h, pivmin);
+      }
+      __syncthreads();
+      int open = 0;
+      if (t < m) {
+        const double l0 = lo[t], h0 = hi[t];
+        const double h = (h0 - l0) / (P + 1);
+        int ps = P;
+        for (int q = 0; q < P; ++q)
+          if (cnt[t * P + q] > t) { ps = q; break; }
+        const double nl = ps == 0 ? l0 : l0 + ps * h;
+        const double nh = ps == P ? h0 : l0 + (ps + 1) * h;
+        lo[t] = nl;
+        hi[t] = nh;
+        const double tol = fmax(2.0 * eps * fmax(fabs(nl), fabs(nh)), eps * tnorm);
+        open = (nh - nl > tol && nh - nl < h0 - l0) ? 1 : 0;
+      }
+      if (!__syncthreads_or(open)) break;
+    }
+    if (t < m) w[t] = 0.5 * (lo[t] + hi[t]);
+  }
+  __syncthreads();
+  // 3. eigenvectors of the tridiagonal: D+ (threads < m) and D- (64..64+m)
+  const double pv = fmax(pivmin, eps * (tnorm > 0.0 ? tnorm : 1.0));
+  if (t < m) {
+    const double lam = w[t];
+    double* q = dp + t * m;
+    double qq = td_guard(dr[0] - lam, pv);
+    q[0] = qq;
+#pragma unroll
+    for (int i = 1; i < MT; ++i) {
+      const double qn = td_guard((dr[i] - lam) - e2r[i - 1] * td_rcp(qq), pv);
+      if (i < m) q[i] = qn;
+      qq = i < m ? qn : qq;
+    }
+  } else if (t >= 64 && t < 64 + m) {
+    // backward chain on the reversed index k = m - 1 - i, read from LDS
+    const int j = t - 64;
+    const double lam = w[j];
+    double* q = dm + j * m;
+    double qq = td_guard(d[m - 1] - lam, pv);
+    q[m - 1] = qq;
+    for (int i = m - 2; i >= 0; --i) {
+      qq = td_guard((d[i] - lam) - e2[i] * td_rcp(qq), pv);
+      q[i] = qq;
+    }
+  }
+  __syncthreads();
+  if (t < m) {
+    const int j = t;
+    const double lam = w[j];
+    const double* qp = dp + j * m;
+    const double* qm = dm + j * m;
+    int r = 0;
+    double best = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double g = fabs(qp[i] + qm[i] - (d[i] - lam));
+      if (i == 0 || g < best) { best = g; r = i; }
+    }
+    Z[r * m + j] = 1.0;
+    double s = 1.0, z = 1.0;
+    for (int i = r - 1; i >= 0; --i) {
+      z *= -e[i] * td_rcp(qp[i]);
+      Z[i * m + j] = z;
+      s = fma(z, z, s);
+    }
+    z = 1.0;
+    for (int i = r + 1; i < m; ++i) {
+      z *= -e[i - 1] * td_rcp(qm[i]);
+      Z[i * m + j] = z;
+      s = fma(z, z, s);
+    }
+    const double inv = 1.0 / sqrt(s);
+    for (int i = 0; i < m; ++i) Z[i * m + j] *= inv;
+  }
+  __syncthreads();
+  // 4. two Gram-Schmidt passes inside clusters, in ascending order
+  const double ortol = fmax(1e-3 * tnorm, 2.2250738585072014e-308);
+  int degenerate = 0;
+  for (int j = 1; j < m; ++j) {
+    int c0 = j;
+    while (c0 > 0 && w[c0] - w[c0 - 1] < ortol) --c0;
+    if (c0 == j) continue;
+    const int nb = j - c0;
+    for (int pass = 0; pass < 2; ++pass) {
+      {
+        const int i = t >> 3, sub = t & 7;
+        double s = 0.0;
+        if (i < nb)
+          for (int row = sub; row < m; row += 8) s = fma(Z[row * m + c0 + i], Z[row * m + j], s);
+        s += __shfl_xor(s, 4, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 1, 8);
+        if (i < nb && sub == 0) p[i] = s;
+      }
+      __syncthreads();
+      for (int row = t; row < m; row += nt) {
+        double s = 0.0;
+        for (int i = 0; i < nb; ++i) s = fma(p[i], Z[row * m + c0 + i], s);
+        Z[row * m + j] -= s;
+      }
+      __syncthreads();
+    }
+    if (t < 64) {
+      const double z = lane < m ? Z[lane * m + j] : 0.0;
+      const double s = wave_sum(z * z);
+      if (lane < m) Z[lane * m + j] = z / sqrt(s);
+      if (lane == 0) misc[3] = (s >= 0.25) ? 0.0 : 1.0;
+    }
+    __syncthreads();
+    if (misc[3] != 0.0) degenerate = 1;
+    __syncthreads();
+  }
+  if (degenerate) return 1;
+  // 5. Y = Q Z
+  for (int idx = t; idx < m * m; idx += nt) {
+    const int r = idx / m, c = idx - (idx / m) * m;
+    const double* qr = Qw + r * lda;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < MT; i += 2) {
+      s0 = fma(i < m ? qr[i] : 0.0, i < m ? Z[(i < m ? i : 0) * m + c] : 0.0, s0);
+      s1 = fma(i + 1 < m ? qr[i + 1] : 0.0, i + 1 < m ? Z[(i + 1 < m ? i + 1 : 0) * m + c] : 0.0, s1);
+    }
+    Y[idx] = s0 + s1;
+  }
+  __syncthreads();
+  return 0;
+}
+
 }  // namespace usbs

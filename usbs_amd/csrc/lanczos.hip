@@ -57,9 +57,13 @@ int lanczos_heavy_tile_rows(int n) {
 // through LDS (block_reduce_cols): that scratch is larger.
 #define LZ_BIG (USBS_RCH * USBS_NT + USBS_RCH * USBS_NPART + 64)
 
+// Ritz eigensolve: tridiagonalisation + multisection for m <= 32 (LZ_TRIDIAG_MAX),
+// parallel Jacobi above.
+#define LZ_TRIDIAG_MAX 32
 static inline size_t ritz_lds_bytes(int m) {
   const int M = m + (m & 1);
-  return sizeof(double) * (size_t)(2 * M * (M + 1) + m * m + 2 * M + 16 + (8 + M + (M - 1) * M) / 2 + 16);
+  const size_t base = (size_t)(2 * M * (M + 1) + m * m + 2 * M + 16 + (8 + M + (M - 1) * M) / 2 + 16);
+  return sizeof(double) * (base + (m <= LZ_TRIDIAG_MAX ? tridiag_scratch_doubles(m, 256) : 0));
 }
 static inline size_t tile_lds_bytes(int maxc) { return sizeof(double) * (size_t)(LZ_BIG + maxc + 8); }
 
@@ -168,6 +172,26 @@ __device__ void tile_rotate(const LzInst<T>& L, int tl, int knext, const double*
 }
 
 
+// The arrowhead T of restart k (lanczos.py:128-146): diagonal alpha, arrow
+// beta[a] at (a, k) for a < k, tridiagonal beta[a] at (a, a+1) for a >= k.
+__device__ void lz_build_T(int m, int k, const double* alpha, const double* beta, double* Tw,
+                           int M, int lda) {
+  for (int idx = threadIdx.x; idx < M * lda; idx += blockDim.x) {
+    const int i = idx / lda, j = idx - (idx / lda) * lda;
+    double v = 0.0;
+    if (i < m && j < m) {
+      if (i == j) {
+        v = alpha[i];
+      } else {
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        if (b == k && a < k) v = beta[a];
+        else if (b == a + 1 && a >= k) v = beta[a];
+      }
+    }
+    Tw[idx] = v;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Ritz step (one block): arrowhead T, eigh, convergence count, saved set,
 // reorder, new alpha/beta and the rotation Yr (lanczos.py:128-168, 202-221).
@@ -185,22 +209,16 @@ __device__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alp
   int* part = sh + 8;               // m ints: saved-set order
   int* sched = part + M;            // (M-1) * M ints: Jacobi pairing
   const int k = ctrl[LZ_K];
-  for (int idx = t; idx < M * lda; idx += nt) {
-    const int i = idx / lda, j = idx - (idx / lda) * lda;
-    double v = 0.0;
-    if (i < m && j < m) {
-      if (i == j) {
-        v = alpha[i];
-      } else {
-        const int a = i < j ? i : j, b = i < j ? j : i;
-        if (b == k && a < k) v = beta[a];
-        else if (b == a + 1 && a >= k) v = beta[a];
-      }
-    }
-    Tw[idx] = v;
-  }
+  lz_build_T(m, k, alpha, beta, Tw, M, lda);
   __syncthreads();
-  block_eigh(Tw, lda, m, w, Y, Vw, cs, sched, red);
+  bool jacobi = m > LZ_TRIDIAG_MAX;
+  if (!jacobi && tridiag_eigh(Tw, lda, m, w, Y, Vw, (double*)(sched + (M - 1) * M + 16))) {
+    // a degenerate cluster: rebuild T and take the Jacobi solver
+    lz_build_T(m, k, alpha, beta, Tw, M, lda);
+    __syncthreads();
+    jacobi = true;
+  }
+  if (jacobi) block_eigh(Tw, lda, m, w, Y, Vw, cs, sched, red);
   const double bm = beta[m - 1];
   if (t == 0) {
     int nc = m;
