@@ -152,9 +152,14 @@ def roofline(stats):
     achieved = per_launch / avg_s / 1e9
     # counters were taken on the torus workload in FP32 storage; others report None
     traffic = traffic_from(TRAFFIC_FILE) if (n, nnz, s) == (20000, 100000, 4) else None
+    persistent = stats.get("lz_form", 1) == 1
+    if not persistent:
+        traffic = None
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "kernel": "k_lz_steps_coop (all Lanczos steps of one restart)",
+            "kernel": ("k_lz_steps_coop (all Lanczos steps of one restart)" if persistent else
+                       "per-op Lanczos step kernels k_lz_pre/k_lz_A/k_lz_BC of one restart "
+                       "(launch gaps included)"),
             "avg_launch_us": round(avg_s * 1e6, 2), "launches": L,
             "steps_per_launch": round(steps / L, 3), "algorithmic_bytes_per_launch": int(per_launch)}
 

@@ -228,10 +228,12 @@ int usbs_objective_infeas(usbs_ctx* ctx, int32_t k, int32_t kc, const double* F,
 
 /*
  * Live kernel timing (measurement support; no counterpart in the reference).
- * While enabled, usbs_solve brackets every persistent Lanczos step launch
- * with HIP events on the context's stream and accumulates the duration of
- * the launches that ran at least one step, with their step counts, so that a
- * caller can divide algorithmic bytes by measured kernel time.
+ * While enabled, usbs_solve brackets the Lanczos step kernels of every
+ * restart with HIP events on the context's stream -- the one cooperative
+ * launch in the persistent form, the restart's step launches (gaps included)
+ * in the per-op form -- and accumulates the duration of the restarts that ran
+ * at least one step, with their step counts, so that a caller can divide
+ * algorithmic bytes by measured kernel time.
  */
 typedef struct usbs_stats {
   double lz_kernel_ms;  /* summed duration of those launches                        */
@@ -240,7 +242,7 @@ typedef struct usbs_stats {
   int64_t lz_sum_cols;  /* sum over those steps of the valid basis column count i+1  */
   int64_t n, nnz;       /* operator size of the uploaded problem (merged pattern)     */
   int32_t value_bytes;  /* bytes per stored basis / operator value of the last solve */
-  int32_t reserved;
+  int32_t lz_form;      /* USBS_EXEC_PEROP or _PERSISTENT: the form the timed calls ran */
 } usbs_stats;
 
 /* Enable (1) or disable (0) the timing; enabling resets the counters. */

@@ -31,7 +31,7 @@ class usbs_stats(ctypes.Structure):
     _fields_ = [("lz_kernel_ms", ctypes.c_double), ("lz_launches", ctypes.c_int64),
                 ("lz_steps", ctypes.c_int64), ("lz_sum_cols", ctypes.c_int64),
                 ("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("value_bytes", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("lz_form", ctypes.c_int32)]
 
 
 class USBSError(RuntimeError):
@@ -140,7 +140,7 @@ class Context:
     def stats(self):
         st = usbs_stats()
         self.check(self._lib.usbs_get_stats(self.h, ctypes.byref(st)))
-        return {f: getattr(st, f) for f, _ in usbs_stats._fields_ if f != "reserved"}
+        return {f: getattr(st, f) for f, _ in usbs_stats._fields_}
 
     def set_problem(self, n, m, C_rows, C_cols, C_vals, A_indices, A_data, b, b_ineq_mask, SCALE_A):
         C_rows, C_cols, C_vals = as_i32(C_rows), as_i32(C_cols), as_f64(C_vals)

@@ -632,11 +632,13 @@ static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
 }
 
 template <typename T, int MAXC>
-static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
+static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm) {
   const dim3 g(L.ntiles), b(USBS_NT);
   int k = 0;
   for (int restart = 0; restart < L.max_restarts; ++restart) {
+    if (tm && hipEventRecord(tm->e0, s) != hipSuccess) return (int)hipErrorUnknown;
     perop_steps(L, k, 0, s);
+    if (tm && hipEventRecord(tm->e1, s) != hipSuccess) return (int)hipErrorUnknown;
     k_lz_ritz<T><<<1, LZ_RITZ_NT, ritz_lds_bytes(L.m), s>>>(L);
     k_lz_rotate<T, MAXC><<<g, b, rot_lds_bytes(L.m), s>>>(L);
     hipError_t e = hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(int) * LZ_NCTRL,
@@ -644,6 +646,14 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
     if (e != hipSuccess) return (int)e;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return (int)e;
+    if (tm && k < L.m) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, tm->e0, tm->e1) != hipSuccess) return (int)hipErrorUnknown;
+      tm->ms += ms;
+      tm->launches += 1;
+      tm->steps += L.m - k;
+      for (int i = k; i < L.m; ++i) tm->sum_cols += i + 1;
+    }
     k = host_ctrl[LZ_K];
     if (host_ctrl[LZ_DONE]) break;
   }
@@ -663,10 +673,10 @@ template int orth_perop<double>(const LzInst<double>&, hipStream_t);
 template int orth_perop<float>(const LzInst<float>&, hipStream_t);
 
 template <typename T>
-int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl) {
-  if (L.m + 1 <= 17) return perop_impl<T, 17>(L, s, host_ctrl);
-  if (L.m + 1 <= 33) return perop_impl<T, 33>(L, s, host_ctrl);
-  if (L.m + 1 <= 49) return perop_impl<T, 49>(L, s, host_ctrl);
+int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm) {
+  if (L.m + 1 <= 17) return perop_impl<T, 17>(L, s, host_ctrl, tm);
+  if (L.m + 1 <= 33) return perop_impl<T, 33>(L, s, host_ctrl, tm);
+  if (L.m + 1 <= 49) return perop_impl<T, 49>(L, s, host_ctrl, tm);
   return (int)hipErrorInvalidValue;
 }
 
@@ -892,8 +902,8 @@ int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cach
 template int lanczos_plan_persistent<double>(LzInst<double>&, int, int, size_t*);
 template int lanczos_plan_persistent<float>(LzInst<float>&, int, int, size_t*);
 
-template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*);
-template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*);
+template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*, LzTiming*);
+template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*, LzTiming*);
 template int lanczos_persistent<double>(const LzInst<double>*, int, int, int, int, hipStream_t,
                                         const int*, const int*, int, unsigned long long, size_t,
                                         hipEvent_t*, int, const LzStop*);

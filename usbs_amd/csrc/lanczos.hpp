@@ -69,8 +69,15 @@ struct LzStop {
 };
 
 // Host launchers (lanczos.hip).  All return hipError_t as int.
+// Measured time of the step kernels of per-op restarts (HIP events around
+// each restart's step launches, launch gaps included).
+struct LzTiming {
+  hipEvent_t e0, e1;
+  double ms;
+  long long launches, steps, sum_cols;
+};
 template <typename T>
-int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl);
+int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm = nullptr);
 // Persistent form for ninst instances sharing the basis size m: per restart
 // one cooperative launch of all Lanczos steps (dev_wgmap: int2 per workgroup
 // = instance, local workgroup), then the Ritz kernel (one block per

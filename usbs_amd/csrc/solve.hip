@@ -69,6 +69,8 @@ struct Solver {
   size_t cache_bytes = 0;            // LDS per persistent workgroup when tiles are cached
   int host_ctrl[LZ_NCTRL] = {0};
   int lz_restarts = 0;
+  LzTiming ptm{};          // per-op step timing (usbs_set_timing)
+  bool use_ptm = false;
   int lz_hint = 2;        // restarts the last persistent call needed: its first early-stop check
   int tev_launched = 0;   // restarts launched by the last timed persistent call
   hipStream_t s2 = nullptr;          // side stream: the lower-bound IPM overlaps the Lanczos call
@@ -82,6 +84,8 @@ struct Solver {
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+    if (ptm.e0) (void)hipEventDestroy(ptm.e0);
+    if (ptm.e1) (void)hipEventDestroy(ptm.e1);
     ctx_free_pool(pool);
   }
 
@@ -168,14 +172,21 @@ struct Solver {
         hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
-    if (p.exec_mode == USBS_EXEC_PERSISTENT && c->timing) {
-      tev.assign(2 * lz.max_restarts, nullptr);
-      lstat_h.assign(2 * lz.max_restarts, 0);
-      for (auto& e : tev)
-        if (hipEventCreate(&e) != hipSuccess) return set_err(c, USBS_ERR_HIP, "event creation failed");
+    if (c->timing) {
+      if (p.exec_mode == USBS_EXEC_PERSISTENT) {
+        tev.assign(2 * lz.max_restarts, nullptr);
+        lstat_h.assign(2 * lz.max_restarts, 0);
+        for (auto& e : tev)
+          if (hipEventCreate(&e) != hipSuccess) return set_err(c, USBS_ERR_HIP, "event creation failed");
+      } else {
+        if (hipEventCreate(&ptm.e0) != hipSuccess || hipEventCreate(&ptm.e1) != hipSuccess)
+          return set_err(c, USBS_ERR_HIP, "event creation failed");
+        use_ptm = true;
+      }
       c->stats.n = n;
       c->stats.nnz = P->nnz;
       c->stats.value_bytes = (int)sizeof(T);
+      c->stats.lz_form = p.exec_mode;
     }
     if (p.exec_mode == USBS_EXEC_PERSISTENT) {
       if (lanczos_plan_persistent<T>(lz, c->device, c->persist_wgs, &cache_bytes))
@@ -216,8 +227,18 @@ struct Solver {
     k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
     CK((int)hipGetLastError());
     if (p.exec_mode == USBS_EXEC_PEROP) {
-      CK(lanczos_perop<T>(L, s, host_ctrl));
+      if (use_ptm) {
+        ptm.ms = 0.0;
+        ptm.launches = ptm.steps = ptm.sum_cols = 0;
+      }
+      CK(lanczos_perop<T>(L, s, host_ctrl, use_ptm ? &ptm : nullptr));
       lz_restarts = host_ctrl[LZ_RESTART];
+      if (use_ptm) {
+        c->stats.lz_kernel_ms += ptm.ms;
+        c->stats.lz_launches += ptm.launches;
+        c->stats.lz_steps += ptm.steps;
+        c->stats.lz_sum_cols += ptm.sum_cols;
+      }
     } else {
       if (tev_pending) {
         const int rc = collect_timing();
