@@ -41,6 +41,10 @@ extern "C" {
 #define USBS_EXEC_PERSISTENT 1 /* one cooperative launch per Lanczos call            */
 #define USBS_EXEC_AUTO 2       /* persistent where every row tile fits in LDS for a
                                   restart (light rows, <= 256-row tiles), else per-op */
+#define USBS_EXEC_TUNE 3       /* usbs_solve: time the first Lanczos call in both
+                                  forms (same results) and keep the faster; the
+                                  choice is kept per context until the next problem;
+                                  elsewhere as USBS_EXEC_AUTO                       */
 
 typedef struct usbs_ctx usbs_ctx;
 

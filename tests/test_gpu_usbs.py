@@ -95,3 +95,33 @@ def test_usbs_mixed_precision_close_to_fp64(gpu_ctx):
                                 ctx=gpu_ctx)
     for a, b in zip(log_o[1:], log_g[1:]):
         np.testing.assert_allclose(b["primal_obj"], a["primal_obj"], rtol=1e-5)
+
+
+def test_usbs_tune_form_same_results(gpu_ctx):
+    """exec_mode "tune" times the first Lanczos call per-op and persistent and
+    keeps the faster; both forms give bitwise the same Lanczos results, so the
+    whole solve equals the per-op and the persistent solves exactly."""
+    W = problems.erdos_renyi(150, 0.08, seed=12)
+    outs = {}
+    for mode in ("perop", "persistent", "tune"):
+        print("case: ER(150,0.08) 3 its mixed exec=%s" % mode, flush=True)
+        outs[mode] = _gpu_only(W, 3, 4, 1, 5, 1.0, "mixed", mode, gpu_ctx)
+    base_log, base = outs["perop"]
+    for mode in ("persistent", "tune"):
+        log_g, out = outs[mode]
+        assert np.array_equal(out.y, base.y) and np.array_equal(out.z, base.z), mode
+        assert out.primal_obj == base.primal_obj, mode
+        assert [r["obj_gap"] for r in log_g] == [r["obj_gap"] for r in base_log], mode
+
+
+def _gpu_only(W, iters, k_curr, k_past, r, rho, precision, exec_mode, ctx):
+    n = W.shape[0]
+    Om = problems.gaussian_matrix(n, r, 1)
+    q0 = problems.unit_start_vector(n, 2)
+    sg = maxcut_helpers.initialize_state(W, r, Omega=Om)
+    log_g = []
+    out = usbs(sg, n, n, 2.0 * n, HP["trace_factor"], rho, HP["beta"], k_curr, k_past, iters,
+               HP["max_time"], -1.0, -1.0, -1.0, min(n, 32), 10, 1e-7, 100, 1.0,
+               maxcut_helpers.compute_max_cut, None, sg.C, precision=precision, exec_mode=exec_mode,
+               q0=q0, log=log_g, context=ctx)
+    return log_g, out

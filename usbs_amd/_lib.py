@@ -16,7 +16,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of device memory",
           -4: "non-finite input", -5: "bad call order", -6: "device wait expired"}
 
 PRECISION = {"fp64": 0, "mixed": 1}
-EXEC_MODE = {"perop": 0, "persistent": 1, "auto": 2}
+EXEC_MODE = {"perop": 0, "persistent": 1, "auto": 2, "tune": 3}
 
 _lib = None
 

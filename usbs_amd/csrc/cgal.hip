@@ -448,7 +448,7 @@ extern "C" int usbs_cgal_solve(usbs_ctx* h, const usbs_cgal_params* prm, const d
   Ctx* c = &h->c;
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
   usbs_cgal_params p = *prm;
-  if (p.exec_mode == USBS_EXEC_AUTO)
+  if (p.exec_mode == USBS_EXEC_AUTO || p.exec_mode == USBS_EXEC_TUNE)
     p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision);
   if (!y0 || !z0 || !q0) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   if (p.lanczos_inner_iterations < 1 || p.lanczos_inner_iterations > 32 ||

@@ -135,6 +135,7 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
     c->keep_r = -1;
   }
   c->has_problem = false;
+  c->tuned_mode = -1;
   Problem& P = c->P;
   P = Problem();
   P.n = (int)n;
@@ -502,7 +503,8 @@ int usbs_lanczos_smallest(usbs_ctx* h, double C_coef, const double* y, const dou
   if (max_restarts < 1) return set_err(c, USBS_ERR_INVALID, "max_restarts must be >= 1");
   if (precision != USBS_PREC_FP64 && precision != USBS_PREC_MIXED)
     return set_err(c, USBS_ERR_INVALID, "unknown precision");
-  if (exec_mode == USBS_EXEC_AUTO) exec_mode = resolve_exec_mode(c->P, inner_iterations, precision);
+  if (exec_mode == USBS_EXEC_AUTO || exec_mode == USBS_EXEC_TUNE)
+    exec_mode = resolve_exec_mode(c->P, inner_iterations, precision);
   if (exec_mode != USBS_EXEC_PEROP && exec_mode != USBS_EXEC_PERSISTENT)
     return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
   for (int i = 0; i < c->P.m; ++i)

@@ -76,6 +76,9 @@ struct Ctx {
   std::vector<DevBuf> prob_bufs;  // freed on new problem / destroy
   std::vector<DevBuf> work_bufs;  // solve workspaces
   int persist_wgs = 0;            // workgroups per persistent Lanczos instance (0 = one per tile)
+  int tuned_mode = -1;            // USBS_EXEC_TUNE: the form measured faster for this problem
+  int tuned_m = 0, tuned_prec = -1;  // ... at this Lanczos basis size and precision
+  float tune_ms[2] = {0.f, 0.f};     // the measured first calls: per-op, persistent
   bool timing = false;            // bracket persistent Lanczos launches with events
   // y, z, P (column-major n x r, ld = keep_ld) of the last usbs_solve, kept
   // on the device for warm starts; survive usbs_set_problem while n, m agree

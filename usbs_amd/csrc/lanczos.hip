@@ -25,6 +25,8 @@
 // forms give bitwise the same numbers.  The persistent form keeps its tile's
 // rows of Q, the work vector and the CSR slice in LDS for the whole restart
 // when they fit.
+#include <cstring>
+#include <tuple>
 #include <vector>
 
 #include "lanczos.hpp"
@@ -321,9 +323,9 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
   for (int j = 0; j < MAXC; ++j) acc[j] = 0.0;
   const T* xq = L.Q + (long long)i * L.ldq;
   if (mode == 0 && L.heavy) {
-    // a wavefront per row, four rows of a wavefront at a time: lane l takes
-    // entries l, l+64, ... of each row, four per row per round, so sixteen
-    // independent loads are in flight per lane; each row's lane sums add in
+    // a wavefront per row, LZ_HEAVY_RG rows of a wavefront at a time: lane l
+    // takes entries l, l+64, ... of each row, LZ_HEAVY_UN per row per round,
+    // all loads of a round in flight together; each row's lane sums add in
     // entry order, then a butterfly over the wavefront, and lane 0 writes.
     // Heavy tiles have at most 256 rows (column-wise dots).
     constexpr int RG = LZ_HEAVY_RG, UN = LZ_HEAVY_UN;
@@ -601,43 +603,109 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate_multi(const LzInst<T>* __
   tile_rotate<T, MAXC>(L, mp.y, knext, lds);
 }
 
+// Destination of the per-op step launches: the stream, or (graph set) a
+// chain of kernel nodes appended to a graph built node by node (no stream
+// capture, so other threads' synchronous HIP calls are unaffected).
+struct StepSink {
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphNode_t last = nullptr;
+  hipError_t err = hipSuccess;
+
+  template <typename... P, typename... A>
+  void launch(void (*kern)(P...), dim3 grid, dim3 block, size_t lds, A... args) {
+    if (!graph) {
+      kern<<<grid, block, lds, s>>>(args...);
+      return;
+    }
+    if (err != hipSuccess) return;
+    std::tuple<P...> vals(args...);
+    void* ptrs[sizeof...(P)];
+    std::apply([&ptrs](auto&... v) {
+      int j = 0;
+      ((ptrs[j++] = (void*)&v), ...);
+    }, vals);
+    hipKernelNodeParams kp{};
+    kp.func = (void*)kern;
+    kp.gridDim = grid;
+    kp.blockDim = block;
+    kp.sharedMemBytes = (unsigned)lds;
+    kp.kernelParams = ptrs;
+    kp.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    err = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &kp);
+    if (err == hipSuccess) last = node;
+  }
+};
+
 // Steps i = i0..m-1 of one restart (or of the QR, mode 2) as per-op launches,
 // then the normalisation of Q[:, m].
 template <typename T, int MAXC, bool RW>
-static void perop_steps_m(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
+static void perop_steps_m(const LzInst<T>& L, int i0, int mode, StepSink& k) {
   if (i0 >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
   const size_t lds = tile_lds_bytes(L.m + 1);
-  const dim3 g(L.ntiles), b(USBS_NT);
+  const dim3 g(L.ntiles), b(USBS_NT), one(1);
   for (int i = i0; i < L.m; ++i) {
-    if (i > i0) k_lz_pre<T><<<1, b, lds, s>>>(L, i);
-    k_lz_A<T, MAXC, RW><<<g, b, lds, s>>>(L, i, i == i0 ? 1 : 0, mode);
-    k_lz_BC<T, MAXC, RW><<<g, b, lds, s>>>(L, i, 0);
-    k_lz_BC<T, MAXC, RW><<<g, b, lds, s>>>(L, i, 1);
+    if (i > i0) k.launch(k_lz_pre<T>, one, b, lds, L, i);
+    k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, i, i == i0 ? 1 : 0, mode);
+    k.launch(k_lz_BC<T, MAXC, RW>, g, b, lds, L, i, 0);
+    k.launch(k_lz_BC<T, MAXC, RW>, g, b, lds, L, i, 1);
   }
-  k_lz_pre<T><<<1, b, lds, s>>>(L, L.m);
-  k_lz_A<T, MAXC, RW><<<g, b, lds, s>>>(L, L.m, 0, 3);
+  k.launch(k_lz_pre<T>, one, b, lds, L, L.m);
+  k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, L.m, 0, 3);
 }
 
 template <typename T, bool RW>
-static void perop_steps_rw(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
-  if (L.m + 1 <= 17) perop_steps_m<T, 17, RW>(L, i0, mode, s);
-  else if (L.m + 1 <= 33) perop_steps_m<T, 33, RW>(L, i0, mode, s);
-  else perop_steps_m<T, 49, RW>(L, i0, mode, s);
+static void perop_steps_rw(const LzInst<T>& L, int i0, int mode, StepSink& k) {
+  if (L.m + 1 <= 17) perop_steps_m<T, 17, RW>(L, i0, mode, k);
+  else if (L.m + 1 <= 33) perop_steps_m<T, 33, RW>(L, i0, mode, k);
+  else perop_steps_m<T, 49, RW>(L, i0, mode, k);
+}
+
+template <typename T>
+static void perop_steps(const LzInst<T>& L, int i0, int mode, StepSink& k) {
+  if (L.tile > 256) perop_steps_rw<T, true>(L, i0, mode, k);
+  else perop_steps_rw<T, false>(L, i0, mode, k);
 }
 
 template <typename T>
 static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
-  if (L.tile > 256) perop_steps_rw<T, true>(L, i0, mode, s);
-  else perop_steps_rw<T, false>(L, i0, mode, s);
+  StepSink k;
+  k.s = s;
+  perop_steps(L, i0, mode, k);
 }
 
 template <typename T, int MAXC>
-static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm) {
+static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm,
+                      LzGraphs* graphs) {
   const dim3 g(L.ntiles), b(USBS_NT);
+  if (graphs) {
+    static_assert(sizeof(LzInst<T>) <= sizeof(graphs->key), "LzInst key too large");
+    if (graphs->key_len != (int)sizeof(L) || memcmp(graphs->key, &L, sizeof(L)) != 0) {
+      lanczos_graphs_clear(graphs);
+      memcpy(graphs->key, &L, sizeof(L));
+      graphs->key_len = (int)sizeof(L);
+    }
+  }
   int k = 0;
   for (int restart = 0; restart < L.max_restarts; ++restart) {
     if (tm && hipEventRecord(tm->e0, s) != hipSuccess) return (int)hipErrorUnknown;
-    perop_steps(L, k, 0, s);
+    if (graphs && k < L.m && k < LzGraphs::N) {
+      if (!graphs->exec[k]) {
+        StepSink sink;
+        hipError_t e = hipGraphCreate(&sink.graph, 0);
+        if (e != hipSuccess) return (int)e;
+        perop_steps(L, k, 0, sink);
+        e = sink.err;
+        if (e == hipSuccess) e = hipGraphInstantiate(&graphs->exec[k], sink.graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(sink.graph);
+        if (e != hipSuccess) { graphs->exec[k] = nullptr; return (int)e; }
+      }
+      const hipError_t e = hipGraphLaunch(graphs->exec[k], s);
+      if (e != hipSuccess) return (int)e;
+    } else {
+      perop_steps(L, k, 0, s);
+    }
     if (tm && hipEventRecord(tm->e1, s) != hipSuccess) return (int)hipErrorUnknown;
     k_lz_ritz<T><<<1, LZ_RITZ_NT, ritz_lds_bytes(L.m), s>>>(L);
     k_lz_rotate<T, MAXC><<<g, b, rot_lds_bytes(L.m), s>>>(L);
@@ -673,10 +741,11 @@ template int orth_perop<double>(const LzInst<double>&, hipStream_t);
 template int orth_perop<float>(const LzInst<float>&, hipStream_t);
 
 template <typename T>
-int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm) {
-  if (L.m + 1 <= 17) return perop_impl<T, 17>(L, s, host_ctrl, tm);
-  if (L.m + 1 <= 33) return perop_impl<T, 33>(L, s, host_ctrl, tm);
-  if (L.m + 1 <= 49) return perop_impl<T, 49>(L, s, host_ctrl, tm);
+int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm,
+                  LzGraphs* graphs) {
+  if (L.m + 1 <= 17) return perop_impl<T, 17>(L, s, host_ctrl, tm, graphs);
+  if (L.m + 1 <= 33) return perop_impl<T, 33>(L, s, host_ctrl, tm, graphs);
+  if (L.m + 1 <= 49) return perop_impl<T, 49>(L, s, host_ctrl, tm, graphs);
   return (int)hipErrorInvalidValue;
 }
 
@@ -902,8 +971,15 @@ int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cach
 template int lanczos_plan_persistent<double>(LzInst<double>&, int, int, size_t*);
 template int lanczos_plan_persistent<float>(LzInst<float>&, int, int, size_t*);
 
-template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*, LzTiming*);
-template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*, LzTiming*);
+template int lanczos_perop<double>(const LzInst<double>&, hipStream_t, int*, LzTiming*, LzGraphs*);
+template int lanczos_perop<float>(const LzInst<float>&, hipStream_t, int*, LzTiming*, LzGraphs*);
+
+void lanczos_graphs_clear(LzGraphs* g) {
+  if (!g) return;
+  for (auto& e : g->exec)
+    if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+  g->key_len = 0;
+}
 template int lanczos_persistent<double>(const LzInst<double>*, int, int, int, int, hipStream_t,
                                         const int*, const int*, int, unsigned long long, size_t,
                                         hipEvent_t*, int, const LzStop*);

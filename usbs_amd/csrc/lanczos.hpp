@@ -41,13 +41,16 @@ struct LzInst {
   double* Yr;    // m x m row-major: saved Ritz vectors (columns) of T
   int* ctrl;     // LZ_NCTRL
   unsigned* bar; // barrier counters, one per restart (persistent launch)
-  int nbar;      // number of counters in bar (>= max_restarts)
   int* lstat;    // 2 x nbar: per persistent launch r, steps done and the sum
                  // over those steps of the valid column count i+1
+  int nbar;      // number of counters in bar (>= max_restarts)
   int tile_nnz_max;  // largest number of operator entries in one row tile
   int heavy;         // 1: a wavefront per operator row in the SpMV (rows of
                      // 32+ entries on average), else a thread per row
+  int reserved;      // keeps the struct free of padding bytes (it is compared
+                     // bytewise as the key of the captured per-op graphs)
 };
+static_assert(sizeof(LzInst<double>) == 8 * 4 + 8 + 8 * 17 + 4 * 4, "LzInst has padding");
 
 // Largest LDS block of a persistent workgroup that caches its tile.
 #define LZ_CACHE_MAX_BYTES (64 * 1024)
@@ -76,8 +79,19 @@ struct LzTiming {
   double ms;
   long long launches, steps, sum_cols;
 };
+// Per-op restarts replayed from HIP graphs: the step launches of a restart
+// that starts at step k are captured once per k (for one LzInst) and then
+// launched as one graph.  Owned by the caller; lanczos_graphs_clear frees it.
+struct LzGraphs {
+  static constexpr int N = 49;
+  hipGraphExec_t exec[N] = {};
+  unsigned char key[256] = {};  // bytes of the LzInst the graphs were captured for
+  int key_len = 0;
+};
+void lanczos_graphs_clear(LzGraphs* g);
 template <typename T>
-int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm = nullptr);
+int lanczos_perop(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTiming* tm = nullptr,
+                  LzGraphs* graphs = nullptr);
 // Persistent form for ninst instances sharing the basis size m: per restart
 // one cooperative launch of all Lanczos steps (dev_wgmap: int2 per workgroup
 // = instance, local workgroup), then the Ritz kernel (one block per

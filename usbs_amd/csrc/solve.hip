@@ -71,6 +71,8 @@ struct Solver {
   int lz_restarts = 0;
   LzTiming ptm{};          // per-op step timing (usbs_set_timing)
   bool use_ptm = false;
+  bool tuning = false;     // USBS_EXEC_TUNE: the next Lanczos call picks the form
+  LzGraphs graphs[2];      // per-op restarts replayed as HIP graphs (nd = k, nd = kc)
   int lz_hint = 2;        // restarts the last persistent call needed: its first early-stop check
   int tev_launched = 0;   // restarts launched by the last timed persistent call
   hipStream_t s2 = nullptr;          // side stream: the lower-bound IPM overlaps the Lanczos call
@@ -85,6 +87,7 @@ struct Solver {
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
     if (ptm.e0) (void)hipEventDestroy(ptm.e0);
+    for (auto& g : graphs) lanczos_graphs_clear(&g);
     if (ptm.e1) (void)hipEventDestroy(ptm.e1);
     ctx_free_pool(pool);
   }
@@ -172,23 +175,25 @@ struct Solver {
         hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
+    tuning = p.exec_mode == USBS_EXEC_TUNE;
     if (c->timing) {
-      if (p.exec_mode == USBS_EXEC_PERSISTENT) {
+      if (p.exec_mode == USBS_EXEC_PERSISTENT || tuning) {
         tev.assign(2 * lz.max_restarts, nullptr);
         lstat_h.assign(2 * lz.max_restarts, 0);
         for (auto& e : tev)
           if (hipEventCreate(&e) != hipSuccess) return set_err(c, USBS_ERR_HIP, "event creation failed");
-      } else {
+      }
+      if (p.exec_mode == USBS_EXEC_PEROP || tuning) {
         if (hipEventCreate(&ptm.e0) != hipSuccess || hipEventCreate(&ptm.e1) != hipSuccess)
           return set_err(c, USBS_ERR_HIP, "event creation failed");
-        use_ptm = true;
+        use_ptm = !tuning;
       }
       c->stats.n = n;
       c->stats.nnz = P->nnz;
       c->stats.value_bytes = (int)sizeof(T);
       c->stats.lz_form = p.exec_mode;
     }
-    if (p.exec_mode == USBS_EXEC_PERSISTENT) {
+    if (p.exec_mode == USBS_EXEC_PERSISTENT || tuning) {
       if (lanczos_plan_persistent<T>(lz, c->device, c->persist_wgs, &cache_bytes))
         return set_err(c, USBS_ERR_HIP, "no persistent Lanczos launch fits the device");
       nb = lz.nblocks;
@@ -220,8 +225,55 @@ struct Solver {
     return (int)hipGetLastError();
   }
 
+  // USBS_EXEC_TUNE: run the first Lanczos call per-op, then again persistent
+  // (same inputs, bitwise the same results), timing each with events on the
+  // stream, and keep the faster form for the rest of the solve (and for later
+  // solves of this problem on this context).
+  int tune_lanczos(int nd) {
+    LzInst<T> L = lz;
+    L.nd = nd;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) return set_err(c, USBS_ERR_HIP, "event creation failed");
+    int rc = USBS_OK;
+    float ms_perop = 0.f, ms_pers = 0.f;
+    do {
+      if (hipEventRecord(ev[0], s) != hipSuccess) { rc = USBS_ERR_HIP; break; }
+      k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
+      if (hipGetLastError() != hipSuccess || lanczos_perop<T>(L, s, host_ctrl, nullptr)) { rc = USBS_ERR_HIP; break; }
+      if (hipEventRecord(ev[1], s) != hipSuccess) { rc = USBS_ERR_HIP; break; }
+      k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
+      int* ctrl_ptr = lz.ctrl;
+      const LzStop stop{&ctrl_ptr, 2, nullptr};
+      if (hipGetLastError() != hipSuccess ||
+          lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
+                                d_map + 2 * nb, lz.ntiles, 1ull << 26, cache_bytes, nullptr,
+                                lz.tile > 256 ? 1 : 0, &stop)) { rc = USBS_ERR_HIP; break; }
+      if (hipEventRecord(ev[2], s) != hipSuccess || hipEventSynchronize(ev[2]) != hipSuccess ||
+          hipEventElapsedTime(&ms_perop, ev[0], ev[1]) != hipSuccess ||
+          hipEventElapsedTime(&ms_pers, ev[1], ev[2]) != hipSuccess) rc = USBS_ERR_HIP;
+    } while (false);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return set_err(c, rc, "Lanczos form tuning failed");
+    const int mode = ms_perop < ms_pers ? USBS_EXEC_PEROP : USBS_EXEC_PERSISTENT;
+    p.exec_mode = mode;
+    tuning = false;
+    c->tuned_mode = mode;
+    c->tuned_m = mL;
+    c->tuned_prec = p.precision;
+    c->tune_ms[0] = ms_perop;
+    c->tune_ms[1] = ms_pers;
+    if (c->timing) {
+      c->stats.lz_form = mode;
+      use_ptm = mode == USBS_EXEC_PEROP;
+    }
+    lz_restarts = host_ctrl[LZ_RESTART];
+    return USBS_OK;
+  }
+
   // Lanczos on the assembled operator (op64/op32 already hold -C + A*(z)).
   int lanczos(int nd) {
+    if (tuning) return tune_lanczos(nd);
     LzInst<T> L = lz;
     L.nd = nd;
     k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
@@ -231,7 +283,7 @@ struct Solver {
         ptm.ms = 0.0;
         ptm.launches = ptm.steps = ptm.sum_cols = 0;
       }
-      CK(lanczos_perop<T>(L, s, host_ctrl, use_ptm ? &ptm : nullptr));
+      CK(lanczos_perop<T>(L, s, host_ctrl, use_ptm ? &ptm : nullptr, &graphs[nd == k ? 0 : 1]));
       lz_restarts = host_ctrl[LZ_RESTART];
       if (use_ptm) {
         c->stats.lz_kernel_ms += ptm.ms;
@@ -619,6 +671,9 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   usbs_params p = *prm;
   if (p.exec_mode == USBS_EXEC_AUTO)
     p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision);
+  if (p.exec_mode == USBS_EXEC_TUNE && c->tuned_mode >= 0 &&
+      c->tuned_m == p.lanczos_inner_iterations && c->tuned_prec == p.precision)
+    p.exec_mode = c->tuned_mode;
   if (!q0 || (!p.warm && (!y0 || !z0))) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   const int k = p.k_curr + p.k_past;
   if (p.k_curr < 1 || p.k_past < 0 || k > 11)
@@ -637,7 +692,8 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
     return set_err(c, USBS_ERR_INVALID, "rho, trace_ub and trace_factor must be positive");
   if (p.precision != USBS_PREC_FP64 && p.precision != USBS_PREC_MIXED)
     return set_err(c, USBS_ERR_INVALID, "unknown precision");
-  if (p.exec_mode != USBS_EXEC_PEROP && p.exec_mode != USBS_EXEC_PERSISTENT)
+  if (p.exec_mode != USBS_EXEC_PEROP && p.exec_mode != USBS_EXEC_PERSISTENT &&
+      p.exec_mode != USBS_EXEC_TUNE)
     return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
   if (!p.warm)
     for (int i = 0; i < c->P.m; ++i)

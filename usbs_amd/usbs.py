@@ -10,8 +10,10 @@ after the reference's select the execution:
   precision "fp64" (plain FP64) or "mixed" (FP32 Lanczos / sketch storage,
             FP64 accumulation, Ritz solve, IPM, duals and convergence test)
   exec_mode "perop" (a kernel per Lanczos phase), "persistent" (one
-            cooperative launch per Lanczos restart) or "auto" (persistent
-            where every row tile stays in LDS for a restart, else per-op)
+            cooperative launch per Lanczos restart), "auto" (persistent
+            where every row tile stays in LDS for a restart, else per-op) or
+            "tune" (time the first Lanczos call in both forms, keep the
+            faster; the results are the same either way)
   q0        Lanczos start vector (default: seeded normal, normalised; the
             reference draws it from jax.random with PRNGKey(0))
   log       a list that receives one dict per condition check
