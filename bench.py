@@ -288,8 +288,9 @@ def main():
     ap.add_argument("--steps", type=int, default=300, help="timed USBS iterations")
     ap.add_argument("--warmup", type=int, default=20, help="untimed USBS iterations")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--exec-mode", default="auto", choices=["auto", "tune", "persistent", "perop"],
-                    help="auto: persistent Lanczos where its row tiles stay in LDS, else per-op")
+    ap.add_argument("--exec-mode", default="tune", choices=["tune", "auto", "persistent", "perop"],
+                    help="tune: time the first Lanczos call in both forms and keep the faster; "
+                         "auto: persistent where row tiles stay in LDS and fit one per CU, else per-op")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--workload", default="torus",
                     choices=["torus", "rgg", "qap", "qap-kron", "serving"],

@@ -37,10 +37,13 @@ extern "C" {
                           /* FP64 Ritz solve, IPM, duals and convergence test       */
 #define USBS_PREC_BF16 2  /* usbs_qap_kron_matvec only: BF16 operands, FP32 accum.  */
 
-#define USBS_EXEC_PEROP 0      /* one kernel per phase, host reads control per restart */
+#define USBS_EXEC_PEROP 0      /* one kernel per phase, host reads control per restart;
+                                  usbs_solve and usbs_cgal replay each restart's
+                                  launches from a HIP graph                          */
 #define USBS_EXEC_PERSISTENT 1 /* one cooperative launch per Lanczos call            */
 #define USBS_EXEC_AUTO 2       /* persistent where every row tile fits in LDS for a
-                                  restart (light rows, <= 256-row tiles), else per-op */
+                                  restart (light rows, <= 256-row tiles) and there
+                                  are no more tiles than compute units, else per-op */
 #define USBS_EXEC_TUNE 3       /* usbs_solve: time the first Lanczos call in both
                                   forms (same results) and keep the faster; the
                                   choice is kept per context until the next problem;

@@ -158,8 +158,12 @@ struct CgalSolver {
   size_t cache_bytes = 0;
   int host_ctrl[LZ_NCTRL] = {0};
   std::vector<double> hpart;
+  LzGraphs graphs;  // per-op restarts replayed as HIP graphs
 
-  ~CgalSolver() { ctx_free_pool(pool); }
+  ~CgalSolver() {
+    lanczos_graphs_clear(&graphs);
+    ctx_free_pool(pool);
+  }
 
   template <typename X>
   X* alloc(size_t count) {
@@ -225,7 +229,7 @@ struct CgalSolver {
     k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(lz, q0);
     CK((int)hipGetLastError());
     if (p.exec_mode == USBS_EXEC_PEROP) {
-      CK(lanczos_perop<T>(lz, s, host_ctrl));
+      CK(lanczos_perop<T>(lz, s, host_ctrl, nullptr, &graphs));
     } else {
       int* ctrl_ptr = lz.ctrl;
       const LzStop stop{&ctrl_ptr, 2, nullptr};
@@ -449,7 +453,7 @@ extern "C" int usbs_cgal_solve(usbs_ctx* h, const usbs_cgal_params* prm, const d
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
   usbs_cgal_params p = *prm;
   if (p.exec_mode == USBS_EXEC_AUTO || p.exec_mode == USBS_EXEC_TUNE)
-    p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision);
+    p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision, c->num_cus);
   if (!y0 || !z0 || !q0) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   if (p.lanczos_inner_iterations < 1 || p.lanczos_inner_iterations > 32 ||
       p.lanczos_inner_iterations > c->P.n)

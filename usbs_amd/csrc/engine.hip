@@ -448,6 +448,9 @@ int usbs_create(usbs_ctx** out, int device) {
     return USBS_ERR_HIP;
   }
   if (const char* e = getenv("USBS_PERSIST_WGS")) h->c.persist_wgs = atoi(e);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    h->c.num_cus = cus;
   *out = h;
   return USBS_OK;
 }
@@ -504,7 +507,7 @@ int usbs_lanczos_smallest(usbs_ctx* h, double C_coef, const double* y, const dou
   if (precision != USBS_PREC_FP64 && precision != USBS_PREC_MIXED)
     return set_err(c, USBS_ERR_INVALID, "unknown precision");
   if (exec_mode == USBS_EXEC_AUTO || exec_mode == USBS_EXEC_TUNE)
-    exec_mode = resolve_exec_mode(c->P, inner_iterations, precision);
+    exec_mode = resolve_exec_mode(c->P, inner_iterations, precision, c->num_cus);
   if (exec_mode != USBS_EXEC_PEROP && exec_mode != USBS_EXEC_PERSISTENT)
     return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
   for (int i = 0; i < c->P.m; ++i)

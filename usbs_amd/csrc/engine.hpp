@@ -56,10 +56,13 @@ struct Problem {
 
 int lanczos_prefers_persistent(int tile, int heavy, int m, int tile_nnz_max, int value_bytes);
 // USBS_EXEC_AUTO: the persistent form where it keeps each tile in LDS for the
-// whole restart (light rows, small tiles), per-op launches otherwise (heavy
-// rows, and tiles streamed from HBM, where the device-wide barriers cost more
-// than launches).  m: Lanczos basis size; precision: USBS_PREC_*.
-inline int resolve_exec_mode(const Problem& P, int m, int precision) {
+// whole restart (light rows, small tiles) and the tiles fit one workgroup per
+// compute unit, per-op launches otherwise (heavy rows, tiles streamed from
+// HBM, or more tiles than compute units: there the device-wide barriers cost
+// more than the launches of graph-replayed restarts).  m: Lanczos basis size;
+// precision: USBS_PREC_*; num_cus: compute units of the device.
+inline int resolve_exec_mode(const Problem& P, int m, int precision, int num_cus) {
+  if (P.lz_ntiles > num_cus) return 0;
   return lanczos_prefers_persistent(P.lz_tile, P.lz_heavy, m, P.tile_nnz_max,
                                     precision == 1 ? 4 : 8);
 }
@@ -75,6 +78,7 @@ struct Ctx {
   Problem P;
   std::vector<DevBuf> prob_bufs;  // freed on new problem / destroy
   std::vector<DevBuf> work_bufs;  // solve workspaces
+  int num_cus = 256;             // compute units of the device
   int persist_wgs = 0;            // workgroups per persistent Lanczos instance (0 = one per tile)
   int tuned_mode = -1;            // USBS_EXEC_TUNE: the form measured faster for this problem
   int tuned_m = 0, tuned_prec = -1;  // ... at this Lanczos basis size and precision

@@ -485,6 +485,12 @@ __device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int 
   }
 }
 
+// counter_normal kept out of line: the breakdown path is cold, and inlined its
+// transcendental code would raise the register count of the step kernels.
+__device__ __attribute__((noinline)) double counter_normal_cold(uint64_t key, uint64_t idx) {
+  return counter_normal(key, idx);
+}
+
 // Breakdown restart vector for the residual w_{i-1} (lanczos.py:107-117):
 // counter normals keyed by step i, orthogonalised twice against columns
 // 0..i-1 and left in w_{i-1} with |.|^2 partials in part3.  sub = 0, 1, 2 are
@@ -497,7 +503,7 @@ __device__ void breakdown_sub(const LzInst<T>& L, int tl, int i, int sub, double
   T* gb = L.w + (long long)((i + 1) & 1) * L.ldq + r0;
   const T* qb = L.Q + r0;
   if (sub == 0) {
-    for (int rl = t; rl < R; rl += USBS_NT) gb[rl] = (T)counter_normal((uint64_t)i, (uint64_t)(r0 + rl));
+    for (int rl = t; rl < R; rl += USBS_NT) gb[rl] = (T)counter_normal_cold((uint64_t)i, (uint64_t)(r0 + rl));
     __syncthreads();
     col_dots(qb, L.ldq, (const T*)gb, R, i, red, L.part1 + (long long)tl * L.pstride);
   } else {

@@ -225,30 +225,37 @@ struct Solver {
     return (int)hipGetLastError();
   }
 
-  // USBS_EXEC_TUNE: run the first Lanczos call per-op, then again persistent
-  // (same inputs, bitwise the same results), timing each with events on the
-  // stream, and keep the faster form for the rest of the solve (and for later
-  // solves of this problem on this context).
+  // USBS_EXEC_TUNE: run the first Lanczos call in both forms once untimed
+  // (the per-op run records its graphs), then again in both forms timed with
+  // events on the stream (same inputs, bitwise the same results), and keep
+  // the faster form for the rest of the solve (and for later solves of this
+  // problem on this context).
   int tune_lanczos(int nd) {
     LzInst<T> L = lz;
     L.nd = nd;
+    LzGraphs* gr = &graphs[nd == k ? 0 : 1];
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     for (auto& e : ev)
       if (hipEventCreate(&e) != hipSuccess) return set_err(c, USBS_ERR_HIP, "event creation failed");
     int rc = USBS_OK;
     float ms_perop = 0.f, ms_pers = 0.f;
+    int* ctrl_ptr = lz.ctrl;
+    const LzStop stop{&ctrl_ptr, 2, nullptr};
+    auto perop = [&]() {
+      k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
+      return hipGetLastError() != hipSuccess || lanczos_perop<T>(L, s, host_ctrl, nullptr, gr);
+    };
+    auto persistent = [&]() {
+      k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
+      return hipGetLastError() != hipSuccess ||
+             lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
+                                   d_map + 2 * nb, lz.ntiles, 1ull << 26, cache_bytes, nullptr,
+                                   lz.tile > 256 ? 1 : 0, &stop);
+    };
     do {
-      if (hipEventRecord(ev[0], s) != hipSuccess) { rc = USBS_ERR_HIP; break; }
-      k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
-      if (hipGetLastError() != hipSuccess || lanczos_perop<T>(L, s, host_ctrl, nullptr)) { rc = USBS_ERR_HIP; break; }
-      if (hipEventRecord(ev[1], s) != hipSuccess) { rc = USBS_ERR_HIP; break; }
-      k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
-      int* ctrl_ptr = lz.ctrl;
-      const LzStop stop{&ctrl_ptr, 2, nullptr};
-      if (hipGetLastError() != hipSuccess ||
-          lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
-                                d_map + 2 * nb, lz.ntiles, 1ull << 26, cache_bytes, nullptr,
-                                lz.tile > 256 ? 1 : 0, &stop)) { rc = USBS_ERR_HIP; break; }
+      if (perop() || persistent()) { rc = USBS_ERR_HIP; break; }
+      if (hipEventRecord(ev[0], s) != hipSuccess || perop()) { rc = USBS_ERR_HIP; break; }
+      if (hipEventRecord(ev[1], s) != hipSuccess || persistent()) { rc = USBS_ERR_HIP; break; }
       if (hipEventRecord(ev[2], s) != hipSuccess || hipEventSynchronize(ev[2]) != hipSuccess ||
           hipEventElapsedTime(&ms_perop, ev[0], ev[1]) != hipSuccess ||
           hipEventElapsedTime(&ms_pers, ev[1], ev[2]) != hipSuccess) rc = USBS_ERR_HIP;
@@ -670,7 +677,7 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
   usbs_params p = *prm;
   if (p.exec_mode == USBS_EXEC_AUTO)
-    p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision);
+    p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision, c->num_cus);
   if (p.exec_mode == USBS_EXEC_TUNE && c->tuned_mode >= 0 &&
       c->tuned_m == p.lanczos_inner_iterations && c->tuned_prec == p.precision)
     p.exec_mode = c->tuned_mode;
