@@ -35,6 +35,7 @@ import numpy as np  # noqa: E402
 METRIC = "time-to-1e-5 rel gap: mixed/batched vs plain FP64 on GPU; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lz_steps_coop_torus100x200_traffic.txt")
+TRAFFIC_FILE_PEROP = os.path.join(ROOT, "profiles", "lz_perop_steps_torus100x200_traffic.txt")
 KRON_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_kron_bf16_qap48_8192vec_traffic.txt")
 
 
@@ -137,9 +138,10 @@ def gap_marks(log):
 
 
 def roofline(stats):
-    """Algorithmic bytes per launch of the persistent Lanczos step kernel:
-    per step the CSR SpMV nnz*(s+4) + 4(n+1) + 2 n s plus CGS2 over c valid
-    columns (4c + 6) n s, with s the bytes per stored value."""
+    """Algorithmic bytes per launch of the Lanczos step kernels (the
+    persistent kernel, or one restart's per-op launches): per step the CSR
+    SpMV nnz*(s+4) + 4(n+1) + 2 n s plus CGS2 over c valid columns
+    (4c + 6) n s, with s the bytes per stored value."""
     L = stats["lz_launches"]
     if not L or stats["lz_kernel_ms"] <= 0:
         return None
@@ -151,10 +153,9 @@ def roofline(stats):
     avg_s = stats["lz_kernel_ms"] / L / 1e3
     achieved = per_launch / avg_s / 1e9
     # counters were taken on the torus workload in FP32 storage; others report None
-    traffic = traffic_from(TRAFFIC_FILE) if (n, nnz, s) == (20000, 100000, 4) else None
     persistent = stats.get("lz_form", 1) == 1
-    if not persistent:
-        traffic = None
+    traffic = (traffic_from(TRAFFIC_FILE if persistent else TRAFFIC_FILE_PEROP)
+               if (n, nnz, s) == (20000, 100000, 4) else None)
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "kernel": ("k_lz_steps_coop (all Lanczos steps of one restart)" if persistent else
