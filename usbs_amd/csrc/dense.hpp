@@ -234,66 +234,12 @@ __device__ int block_eigh(double* A, int lda, int m, double* w, double* Y, doubl
 }
 
 // ---------------------------------------------------------------------------
-// LDL^T of a symmetric d x d matrix without pivoting (the IPM's SPD Newton
-// block), in place: on return the strict lower triangle holds the unit factor
-// L and rD[k] = 1 / D[k].  Right-looking: step k updates the trailing lower
-// triangle with column k (16 threads per row).  A zero pivot yields inf/NaN,
-// as an LU solve of a singular matrix does.
-__device__ void dense_ldlt(double* A, int d, double* rD) {
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int tr = t >> 4, nr = nt >> 4, tc = t & 15;
-  for (int k = 0; k < d - 1; ++k) {
-    const double rdk = 1.0 / A[k * d + k];
-    // trailing update A[i][j] -= (A[i][k] / D_k) A[j][k] for k < j <= i
-    for (int i = k + 1 + tr; i < d; i += nr) {
-      const double lik = A[i * d + k] * rdk;
-      for (int j = k + 1 + tc; j <= i; j += 16) A[i * d + j] -= lik * A[j * d + k];
-    }
-    __syncthreads();
-  }
-  for (int k = t; k < d; k += nt) rD[k] = 1.0 / A[k * d + k];
-  __syncthreads();
-  for (int i = tr; i < d; i += nr)
-    for (int j = tc; j < i; j += 16) A[i * d + j] *= rD[j];
-  __syncthreads();
-}
-
 // Broadcast lane `src` (uniform) of a double to the whole wavefront.
 __device__ __forceinline__ double readlane_d(double v, int src) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, src);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// Solve (L D L^T) X = B with the dense_ldlt factor for nrhs right-hand sides
-// (B row-major d x ldb, columns 0..nrhs-1 overwritten), d <= 128.  One
-// wavefront per right-hand side holds it in registers (rows lane and
-// lane + 64); forward and backward substitution are column-oriented, so each
-// step is one broadcast and one FMA per row.
-__device__ void dense_ldlt_solve(const double* A, int d, const double* rD, double* B, int ldb,
-                                 int nrhs) {
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (w < nrhs) {
-    const int i0 = lane, i1 = lane + 64;
-    double y0 = (i0 < d) ? B[i0 * ldb + w] : 0.0;
-    double y1 = (i1 < d) ? B[i1 * ldb + w] : 0.0;
-    for (int k = 0; k < d; ++k) {
-      const double yk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
-      if (i0 > k && i0 < d) y0 -= A[i0 * d + k] * yk;
-      if (i1 > k && i1 < d) y1 -= A[i1 * d + k] * yk;
-    }
-    if (i0 < d) y0 *= rD[i0];
-    if (i1 < d) y1 *= rD[i1];
-    for (int k = d - 1; k > 0; --k) {
-      const double xk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
-      if (i0 < k) y0 -= A[k * d + i0] * xk;
-      if (i1 < k) y1 -= A[k * d + i1] * xk;
-    }
-    if (i0 < d) B[i0 * ldb + w] = y0;
-    if (i1 < d) B[i1 * ldb + w] = y1;
-  }
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

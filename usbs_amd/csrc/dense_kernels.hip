@@ -129,7 +129,7 @@ struct IpmLds {
   double *S, *T, *Si, *dS, *dT, *tk, *red;
   double* R;     // d x 3 right-hand sides of the Woodbury solve
   double* rD;    // d reciprocal pivots of the LDL^T factor
-  double* cb;    // 2 x 128 published columns of the LDL^T
+  double* lp;    // d x 8 scaled panel columns of the LDL^T
   int* ired;
   int *pb, *pc;  // svec index -> (b, c)
 };
@@ -145,7 +145,7 @@ __device__ IpmLds ipm_layout(double* lds, int k) {
   L.tk = p; p += kk;
   L.R = p; p += 3 * d;
   L.rD = p; p += d;
-  L.cb = p; p += 256;
+  L.lp = p; p += 8 * d;
   L.red = p; p += 16;
   L.ired = (int*)p; p += 16;
   L.pb = (int*)p; p += (d + 1) / 2 + 1;
@@ -156,7 +156,7 @@ __device__ IpmLds ipm_layout(double* lds, int k) {
 static size_t ipm_lds_bytes(int k) {
   const int d = k * (k + 1) / 2, kk = k * k;
   const int big = d * d > 5 * kk + 4 * k + 64 ? d * d : 5 * kk + 4 * k + 64;
-  return sizeof(double) * (big + 14 * d + 256 + 6 * kk + 16 + 16 + d + 4);
+  return sizeof(double) * (big + 14 * d + 8 * d + 6 * kk + 16 + 16 + d + 4);
 }
 
 __device__ void ipm_tables(const IpmLds& L, int k) {
@@ -191,75 +191,90 @@ __device__ __forceinline__ double symkron(const IpmLds& L, int k, int a, int b) 
 }
 
 
-// LDL^T (no pivoting) of the SPD Newton block B (D x D row-major in LDS),
-// held in registers by the four wavefronts of the block: lane l owns rows l
-// and l + 64, wave w owns the columns j = w (mod 4).  Step k: the owner of
-// column k publishes it (double-buffered LDS column, one barrier per step),
-// every lane scales its entry by 1/D_k and updates its columns j > k.  On
-// return the strict lower triangle of B holds the unit factor L (stored once
-// at the end, scaled by the reciprocal pivots) and rD[k] = 1/D_k.  A zero pivot yields inf/NaN, as a dense solve of a singular
-// matrix does.  cb: 256 doubles of LDS.
+// LDL^T (no pivoting) of the SPD Newton block A (D x D row-major in LDS), in
+// place: on return the strict lower triangle holds the unit factor L and
+// rD[k] = 1 / D_k.  Panels of 8 columns: wavefront 0 factors the panel in
+// registers (a lane per row, the pivot and the panel rows broadcast with
+// readlane), then every thread applies the panel's updates to the trailing
+// lower triangle, column by column in order, so each entry sees exactly the
+// right-looking sequence A[i][j] -= (A[i][k] / D_k) A[j][k], k ascending.
+// Two barriers per panel instead of one per column.  A zero pivot yields
+// inf/NaN, as an LU solve of a singular matrix does.  Lp: D x 8 doubles of
+// LDS (the panel's scaled column values).
 template <int D>
-__device__ void ipm_ldlt(double* B, double* rD, double* cb) {
-  constexpr int NC = (D + 3) / 4;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int r0 = lane, r1 = lane + 64;
-  const int r0c = r0 < D ? r0 : D - 1, r1c = r1 < D ? r1 : D - 1;
-  double a0[NC], a1[NC];
+__device__ __forceinline__ void ipm_ldlt(double* A, double* rD, double* Lp) {
+  constexpr int PW = 8;
+  constexpr int NC = (D + 15) / 16;
+  const int t = threadIdx.x, tr = t >> 4, tc = t & 15, lane = t & 63, w = t >> 6;
+  for (int k0 = 0; k0 < D - 1; k0 += PW) {
+    const int pw = min(PW, D - 1 - k0);
+    if (w == 0) {
+      const int r0 = k0 + lane, r1 = k0 + 64 + lane;
+      double p0[PW], p1[PW];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const int j = w + 4 * c;
-    a0[c] = (j < D && r0 < D) ? B[r0 * D + j] : 0.0;
-    a1[c] = (D > 64 && j < D && r1 < D) ? B[r1 * D + j] : 0.0;
-  }
-  __syncthreads();
-  for (int k = 0; k < D; ++k) {
-    double* col = cb + (k & 1) * 128;
-    if (w == (k & 3)) {
-      // a0[k >> 2] without dynamic register indexing (which would spill)
-      double v0 = 0.0, v1 = 0.0;
+      for (int c = 0; c < PW; ++c) {
+        p0[c] = (c < pw && r0 < D && k0 + c <= r0) ? A[r0 * D + k0 + c] : 0.0;
+        p1[c] = (c < pw && r1 < D) ? A[r1 * D + k0 + c] : 0.0;
+      }
 #pragma unroll
-      for (int c = 0; c < NC; ++c)
-        if (c == (k >> 2)) { v0 = a0[c]; v1 = a1[c]; }
-      if (r0 < D) col[r0] = v0;
-      if (D > 64 && r1 < D) col[r1] = v1;
-    }
-    __syncthreads();
-    // every load of the step is issued before the first use, and the
-    // updates are selects rather than branches
-    const double piv = col[k];
-    const double c0 = col[r0c];
-    const double c1 = (D > 64) ? col[r1c] : 0.0;
-    double cv[NC];
+      for (int c = 0; c < PW; ++c) {
+        if (c < pw) {
+          const double rdk = 1.0 / readlane_d(p0[c], c);
+          const double l0 = p0[c] * rdk, l1 = p1[c] * rdk;
+          if (r0 > k0 + c && r0 < D) Lp[r0 * PW + c] = l0;
+          if (r1 < D) Lp[r1 * PW + c] = l1;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int j = w + 4 * c;
-      cv[c] = col[j < D ? j : D - 1];
-    }
-    const double rk = 1.0 / piv;
-    if (threadIdx.x == 0) rD[k] = rk;
-    const double l0 = c0 * rk;
-    const double l1 = c1 * rk;
+          for (int j = c + 1; j < PW; ++j) {
+            if (j < pw) {
+              const double ajc = readlane_d(p0[c], j);  // A[k0 + j][k0 + c]
+              if (r0 >= k0 + j) p0[j] = p0[j] - l0 * ajc;
+              p1[j] = p1[j] - l1 * ajc;
+            }
+          }
+        }
+      }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int j = w + 4 * c;
-      const bool live = (j > k) && (j < D);
-      const double n0 = a0[c] - l0 * cv[c];
-      a0[c] = (live && j <= r0) ? n0 : a0[c];
-      if (D > 64) {
-        const double n1 = a1[c] - l1 * cv[c];
-        a1[c] = (live && j <= r1) ? n1 : a1[c];
+      for (int c = 0; c < PW; ++c) {
+        if (c < pw) {
+          if (r0 < D && r0 >= k0 + c) A[r0 * D + k0 + c] = p0[c];
+          if (r1 < D) A[r1 * D + k0 + c] = p1[c];
+        }
       }
     }
-  }
-  __syncthreads();
-  // column j of the lower triangle is final after step j - 1: scale and store
+    __syncthreads();
+    const int kt = k0 + pw;
+    if (kt < D) {
+      double aj[NC][PW];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const int j = w + 4 * c;
-    if (j < D && j < r0 && r0 < D) B[r0 * D + j] = a0[c] * rD[j];
-    if (D > 64 && j < D && j < r1 && r1 < D) B[r1 * D + j] = a1[c] * rD[j];
+      for (int cc = 0; cc < NC; ++cc) {
+        const int j = kt + tc + 16 * cc;
+        const int jc = j < D ? j : D - 1;
+#pragma unroll
+        for (int c = 0; c < PW; ++c) aj[cc][c] = (c < pw) ? A[jc * D + k0 + c] : 0.0;
+      }
+      for (int i = kt + tr; i < D; i += 16) {
+        double li[PW];
+#pragma unroll
+        for (int c = 0; c < PW; ++c) li[c] = (c < pw) ? Lp[i * PW + c] : 0.0;
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+          const int j = kt + tc + 16 * cc;
+          if (j <= i) {
+            double v = A[i * D + j];
+#pragma unroll
+            for (int c = 0; c < PW; ++c)
+              if (c < pw) v = v - li[c] * aj[cc][c];
+            A[i * D + j] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
+  for (int x = t; x < D; x += blockDim.x) rD[x] = 1.0 / A[x * D + x];
+  __syncthreads();
+  for (int i = tr; i < D; i += 16)
+    for (int j = tc; j < i; j += 16) A[i * D + j] *= rD[j];
   __syncthreads();
 }
 
@@ -268,7 +283,7 @@ __device__ void ipm_ldlt(double* B, double* rD, double* cb) {
 // registers (rows lane, lane + 64); both substitutions are column-oriented:
 // one broadcast (readlane) and one FMA per row and step.
 template <int D>
-__device__ void ipm_ldlt_solve(const double* Lm, const double* rD, double* R, int nrhs) {
+__device__ __forceinline__ void ipm_ldlt_solve(const double* Lm, const double* rD, double* R, int nrhs) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (w < nrhs) {
     const int i0 = lane, i1 = lane + 64;
@@ -309,16 +324,16 @@ __device__ void ipm_ldlt_solve(const double* Lm, const double* rD, double* R, in
 // solution as a dense solve of the full matrix, up to rounding.  r in/out;
 // R: D x 3 LDS scratch; B is destroyed.
 template <int D>
-__device__ void woodbury_solve(double* B, double* r, const double* u1, const double* u2,
+__device__ __forceinline__ void woodbury_solve(double* B, double* r, const double* u1, const double* u2,
                                const double* w1, const double* w2, int nu, double* R, double* rD,
-                               double* cb, double* red) {
+                               double* lp, double* red) {
   const int t = threadIdx.x;
   for (int a = t; a < D; a += blockDim.x) {
     R[a * 3 + 0] = r[a];
     R[a * 3 + 1] = u1[a];
     R[a * 3 + 2] = (nu > 1) ? u2[a] : 0.0;
   }
-  dense_ldlt(B, D, rD);
+  ipm_ldlt<D>(B, rD, lp);
   ipm_ldlt_solve<D>(B, rD, R, nu + 1);
   if (t < 64) {
     double g0 = 0.0, g1 = 0.0, m00 = 0.0, m01 = 0.0, m10 = 0.0, m11 = 0.0;
@@ -513,7 +528,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
       L.ordv[a] = (L.q12[a] - k2 * L.sI[a]) / den;
     }
     __syncthreads();
-    woodbury_solve<D>(L.coeff, L.dsS, L.q12, L.sI, L.dsT, L.ordv, 2, L.R, L.rD, L.cb, L.red);
+    woodbury_solve<D>(L.coeff, L.dsS, L.q12, L.sI, L.dsT, L.ordv, 2, L.R, L.rD, L.lp, L.red);
     // back-substitution for the other directions (usbs.py:111-122)
     for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = k1 * L.q12[a] + L.sI[a];
     __syncthreads();
@@ -615,7 +630,7 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
       L.dsS[a] = v;
     }
     __syncthreads();
-    woodbury_solve<D>(L.coeff, L.dsS, L.sI, L.sI, L.ordv, L.ordv, 1, L.R, L.rD, L.cb, L.red);
+    woodbury_solve<D>(L.coeff, L.dsS, L.sI, L.sI, L.ordv, L.ordv, 1, L.R, L.rD, L.lp, L.red);
     const double p1 = dotd(L.sI, L.dsS, d, L.red);
     const double d_eta = (p1 - k1 * (mu / eta - g2 - omega) + mu / omega + trS + eta - 1) / (-k1 * ze - 1);
     double d_omega = -ze * d_eta + mu / eta;
