@@ -66,7 +66,7 @@ __device__ __forceinline__ bool lane_chol(const double* M, const double* dM, dou
 
 // S^{-1} for SPD S: lane j < K solves L L^T x = e_j for column j.
 template <int K>
-__device__ void lane_spd_inverse(const double* S, double* Si) {
+__device__ __forceinline__ void lane_spd_inverse(const double* S, double* Si) {
   const int j = threadIdx.x;
   if (j < K) {
     double L[K * (K + 1) / 2], rd[K];
@@ -97,7 +97,7 @@ __device__ void lane_spd_inverse(const double* S, double* Si) {
 // per candidate (each thread tests one candidate of a batch of blockDim).
 // pw0 = 0.9^threadIdx.x, the first batch's candidate of this thread.
 template <int K>
-__device__ double pd_step(const double* M, const double* dM, int max_iters, double pw0,
+__device__ __forceinline__ double pd_step(const double* M, const double* dM, int max_iters, double pw0,
                           double* red) {
   const int t = threadIdx.x;
   for (int j0 = 0; j0 < max_iters; j0 += blockDim.x) {
@@ -119,7 +119,7 @@ __device__ double pd_step(const double* M, const double* dM, int max_iters, doub
   return 0.0;
 }
 
-__device__ void mat_from_svec(const double* v, int k, double* M) {
+__device__ __forceinline__ void mat_from_svec(const double* v, int k, double* M) {
   dense_svec_inv(v, k, M);
   __syncthreads();
 }
@@ -364,7 +364,7 @@ __device__ __forceinline__ void woodbury_solve(double* B, double* r, const doubl
 }
 
 // mu of the current point (solver/usbs.py:75-77, 163-165)
-__device__ double ipm_mu(const IpmLds& L, int k, int d, double eta, double zeta, double omega) {
+__device__ __forceinline__ double ipm_mu(const IpmLds& L, int k, int d, double eta, double zeta, double omega) {
   dense_svec(L.S, k, L.sS);
   dense_svec(L.T, k, L.sT);
   __syncthreads();
@@ -374,7 +374,7 @@ __device__ double ipm_mu(const IpmLds& L, int k, int d, double eta, double zeta,
 }
 
 // Initial T, zeta, omega from T0 (in L.T) and zeta0 (usbs.py:67-74 / 228-235).
-__device__ void ipm_center(const IpmLds& L, int k, double& zeta, double& omega) {
+__device__ __forceinline__ void ipm_center(const IpmLds& L, int k, double& zeta, double& omega) {
   double mx = zeta, mn = zeta;
   for (int i = 0; i < k; ++i) {
     const double v = L.T[i * k + i];
@@ -393,7 +393,7 @@ __device__ void ipm_center(const IpmLds& L, int k, double& zeta, double& omega) 
 
 // Shared step: sizes, line search, update; returns the new mu and step.
 template <int K>
-__device__ void ipm_step(const IpmLds& L, int max_iters, double pw0, double& eta, double& zeta,
+__device__ __forceinline__ void ipm_step(const IpmLds& L, int max_iters, double pw0, double& eta, double& zeta,
                          double& omega, double& mu, double& step, double d_eta, double d_zeta,
                          double d_omega) {
   constexpr int D = K * (K + 1) / 2;
@@ -428,7 +428,7 @@ __device__ void ipm_step(const IpmLds& L, int max_iters, double pw0, double& eta
 
 // out = (cq Gq) v for the d x d matrix Gq in global memory: 4 threads per
 // row, each summing every 4th column, combined in a fixed order.
-__device__ void qmatvec(const double* Gq, double cq, const double* v, int d, double* out) {
+__device__ __forceinline__ void qmatvec(const double* Gq, double cq, const double* v, int d, double* out) {
   const int t = threadIdx.x, q = t & 3;
   for (int a0 = 0; a0 < d; a0 += IPM_NT / 4) {
     const int a = a0 + (t >> 2);
