@@ -197,7 +197,7 @@ __device__ void lz_build_T(int m, int k, const double* alpha, const double* beta
 // ----------------------------------------------------------------------------
 // Ritz step (one block): arrowhead T, eigh, convergence count, saved set,
 // reorder, new alpha/beta and the rotation Yr (lanczos.py:128-168, 202-221).
-__device__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alpha, double* beta,
+__device__ __forceinline__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alpha, double* beta,
                         double* Yr, int* ctrl, double* lds) {
   const int t = threadIdx.x, nt = blockDim.x;
   const int M = m + (m & 1), lda = M + 1;
