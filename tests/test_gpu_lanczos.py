@@ -74,19 +74,45 @@ def test_lanczos_mixed_precision_error_stated(gpu_ctx, exec_mode):
     assert np.abs(ev_g[0] - dense[0]) <= 2e-6 * scale
 
 
-def test_krylov_breakdown_small_n(gpu_ctx):
+@pytest.mark.parametrize("exec_mode", ["persistent", "perop"])
+def test_krylov_breakdown_small_n(gpu_ctx, exec_mode):
     """n <= 32: the Krylov space is exhausted inside the first restart, which
-    forces the random-restart branch (lanczos.py:107-117)."""
+    forces the random-restart branch (lanczos.py:107-117).  The per-op form
+    meets it in a step's phase A, stops the restart there and resumes after
+    building the restart vector."""
     W = problems.erdos_renyi(24, 0.3, seed=4)
     negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 9)
     n = negC.shape[0]
+    print("case: ER(24,0.3) breakdown exec=%s" % exec_mode, flush=True)
     ev_o, _ = olz.eigsh_smallest(n, negC, A_data, A_indices, y, q0, 4, n, 3, 1e-7)
     ev_g, V_g, info = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 4, n, 3, 1e-7,
-                                     return_info=True, context=gpu_ctx)
+                                     exec_mode=exec_mode, return_info=True, context=gpu_ctx)
     dense = np.linalg.eigvalsh((negC + scipy.sparse.diags(y)).toarray())
     np.testing.assert_allclose(ev_g, dense[:4], rtol=1e-9, atol=1e-9)
     np.testing.assert_allclose(ev_o, dense[:4], rtol=1e-9, atol=1e-9)
     assert info["breakdowns"] >= 1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_krylov_breakdown_forms_bitwise(gpu_ctx, precision):
+    """Breakdowns in both forms: the per-op resume after a flagged step gives
+    bitwise the persistent form's Ritz pairs, breakdown and step counts.  (In
+    FP32 storage the exhausted Krylov space leaves residuals at rounding level,
+    above the 1e-15 breakdown threshold, so only FP64 breaks down.)"""
+    out = {}
+    for n, p, seed in ((24, 0.3, 4), (30, 0.25, 7)):
+        W = problems.erdos_renyi(n, p, seed=seed)
+        negC, A_data, A_indices, y, q0 = _maxcut_operator(W, 9)
+        for mode in ("persistent", "perop"):
+            print("case: ER(%d,%g) breakdown %s exec=%s" % (n, p, precision, mode), flush=True)
+            out[(n, mode)] = eigsh_smallest(n, negC, A_data, A_indices, y, q0, 4, n, 3, 1e-7,
+                                            precision=precision, exec_mode=mode,
+                                            return_info=True, context=gpu_ctx)
+        ev_a, V_a, info_a = out[(n, "persistent")]
+        ev_b, V_b, info_b = out[(n, "perop")]
+        assert np.array_equal(ev_a, ev_b) and np.array_equal(V_a, V_b)
+        assert info_a == info_b
+        assert info_a["breakdowns"] >= (1 if precision == "fp64" else 0)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

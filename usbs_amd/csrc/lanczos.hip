@@ -543,15 +543,38 @@ __device__ TileCtx<T, false> plain_tile(const LzInst<T>& L, int tl) {
   return X;
 }
 
+// Phase A of step i.  record (restart steps without a k_lz_pre launch before
+// them): every block computes beta_{i-1} from the part3 partials, block 0
+// records it, and on a breakdown the step is not taken: block 0 marks it in
+// ctrl[LZ_PENDING] and the host resumes there (lanczos_perop).
 template <typename T, int MAXC, bool RW>
-__global__ void __launch_bounds__(USBS_NT) k_lz_A(LzInst<T> L, int i, int first, int mode) {
+__global__ void __launch_bounds__(USBS_NT) k_lz_A(LzInst<T> L, int i, int first, int mode,
+                                                  int record) {
   extern __shared__ double lds[];
-  phaseA<T, false, MAXC, RW>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, first != 0, mode, lds);
+  if (L.ctrl[LZ_PENDING]) return;
+  double beta = -1.0;
+  if (record && !first) {
+    beta = lz_beta(L, lds);
+    __syncthreads();
+    const bool brk = beta < 1e-15;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (brk) {
+        L.ctrl[LZ_PENDING] = i;
+      } else {
+        L.beta[i - 1] = beta;
+        L.ctrl[LZ_BREAKDOWN] = 0;
+      }
+    }
+    if (brk) return;
+  }
+  phaseA<T, false, MAXC, RW>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, first != 0, mode, lds,
+                             beta);
 }
 
 template <typename T, int MAXC, bool RW>
 __global__ void __launch_bounds__(USBS_NT) k_lz_BC(LzInst<T> L, int i, int final_pass) {
   extern __shared__ double lds[];
+  if (L.ctrl[LZ_PENDING]) return;
   phaseBC<T, false, MAXC, RW>(L, plain_tile(L, blockIdx.x), blockIdx.x, i, final_pass, lds);
 }
 
@@ -560,6 +583,7 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_BC(LzInst<T> L, int i, int final
 template <typename T>
 __global__ void __launch_bounds__(USBS_NT) k_lz_pre(LzInst<T> L, int i) {
   extern __shared__ double lds[];
+  if (threadIdx.x == 0) L.ctrl[LZ_PENDING] = 0;
   if (!lz_record_beta(L, i, true, lds)) return;
   for (int sub = 0; sub < 3; ++sub) {
     for (int tl = 0; tl < L.ntiles; ++tl) breakdown_sub(L, tl, i, sub, lds);
@@ -571,6 +595,7 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_pre(LzInst<T> L, int i) {
 template <typename T>
 __global__ void __launch_bounds__(LZ_RITZ_NT) k_lz_ritz(LzInst<T> L) {
   extern __shared__ double lds[];
+  if (L.ctrl[LZ_PENDING]) return;
   lz_ritz(L.m, L.nd, L.tol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
 }
 
@@ -587,6 +612,7 @@ __global__ void __launch_bounds__(LZ_RITZ_NT) k_lz_ritz_multi(const LzInst<T>* _
 template <typename T, int MAXC>
 __global__ void __launch_bounds__(USBS_NT) k_lz_rotate(LzInst<T> L) {
   extern __shared__ double lds[];
+  if (L.ctrl[LZ_PENDING]) return;
   const int knext = L.ctrl[LZ_K];
   const int mm = L.m * L.m;
   for (int idx = threadIdx.x; idx < mm; idx += USBS_NT) lds[idx] = L.Yr[idx];
@@ -645,40 +671,56 @@ struct StepSink {
 };
 
 // Steps i = i0..m-1 of one restart (or of the QR, mode 2) as per-op launches,
-// then the normalisation of Q[:, m].
+// then the normalisation of Q[:, m].  Two sequences:
+//  * checked: a k_lz_pre launch before every step after the first (beta and,
+//    on a breakdown, the restart vector), so the steps never stop;
+//  * optimistic (restarts of lanczos_perop): phase A records beta itself and
+//    flags a breakdown in ctrl[LZ_PENDING], three launches a step; the host
+//    resumes a flagged restart with perop_resume.
+// resume (optimistic only): step i0 follows a breakdown fixed by k_lz_pre,
+// so it normalises w_{i0-1} like any later step.
 template <typename T, int MAXC, bool RW>
-static void perop_steps_m(const LzInst<T>& L, int i0, int mode, StepSink& k) {
-  if (i0 >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
+static void perop_steps_m(const LzInst<T>& L, int i0, int mode, StepSink& k, bool optimistic,
+                          bool resume) {
+  if (i0 >= L.m && !resume) return;  // every basis column saved: no step, Q[:, m] unchanged
   const size_t lds = tile_lds_bytes(L.m + 1);
   const dim3 g(L.ntiles), b(USBS_NT), one(1);
+  const int rec = optimistic ? 1 : 0;
+  if (resume) k.launch(k_lz_pre<T>, one, b, lds, L, i0);
   for (int i = i0; i < L.m; ++i) {
-    if (i > i0) k.launch(k_lz_pre<T>, one, b, lds, L, i);
-    k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, i, i == i0 ? 1 : 0, mode);
+    if (i > i0 && !optimistic) k.launch(k_lz_pre<T>, one, b, lds, L, i);
+    // the resumed step normalises the vector k_lz_pre built (beta already recorded)
+    k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, i, (i == i0 && !resume) ? 1 : 0, mode,
+             (resume && i == i0) ? 0 : rec);
     k.launch(k_lz_BC<T, MAXC, RW>, g, b, lds, L, i, 0);
     k.launch(k_lz_BC<T, MAXC, RW>, g, b, lds, L, i, 1);
   }
-  k.launch(k_lz_pre<T>, one, b, lds, L, L.m);
-  k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, L.m, 0, 3);
+  if (!optimistic) k.launch(k_lz_pre<T>, one, b, lds, L, L.m);
+  // after a resume at i0 = m the breakdown vector is already built and normalised here
+  k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, L.m, 0, 3, (resume && i0 == L.m) ? 0 : rec);
 }
 
 template <typename T, bool RW>
-static void perop_steps_rw(const LzInst<T>& L, int i0, int mode, StepSink& k) {
-  if (L.m + 1 <= 17) perop_steps_m<T, 17, RW>(L, i0, mode, k);
-  else if (L.m + 1 <= 33) perop_steps_m<T, 33, RW>(L, i0, mode, k);
-  else perop_steps_m<T, 49, RW>(L, i0, mode, k);
+static void perop_steps_rw(const LzInst<T>& L, int i0, int mode, StepSink& k, bool optimistic,
+                           bool resume) {
+  if (L.m + 1 <= 17) perop_steps_m<T, 17, RW>(L, i0, mode, k, optimistic, resume);
+  else if (L.m + 1 <= 33) perop_steps_m<T, 33, RW>(L, i0, mode, k, optimistic, resume);
+  else perop_steps_m<T, 49, RW>(L, i0, mode, k, optimistic, resume);
 }
 
 template <typename T>
-static void perop_steps(const LzInst<T>& L, int i0, int mode, StepSink& k) {
-  if (L.tile > 256) perop_steps_rw<T, true>(L, i0, mode, k);
-  else perop_steps_rw<T, false>(L, i0, mode, k);
+static void perop_steps(const LzInst<T>& L, int i0, int mode, StepSink& k, bool optimistic = false,
+                        bool resume = false) {
+  if (L.tile > 256) perop_steps_rw<T, true>(L, i0, mode, k, optimistic, resume);
+  else perop_steps_rw<T, false>(L, i0, mode, k, optimistic, resume);
 }
 
 template <typename T>
-static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s) {
+static void perop_steps(const LzInst<T>& L, int i0, int mode, hipStream_t s,
+                        bool optimistic = false, bool resume = false) {
   StepSink k;
   k.s = s;
-  perop_steps(L, i0, mode, k);
+  perop_steps(L, i0, mode, k, optimistic, resume);
 }
 
 template <typename T, int MAXC>
@@ -701,7 +743,7 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTimin
         StepSink sink;
         hipError_t e = hipGraphCreate(&sink.graph, 0);
         if (e != hipSuccess) return (int)e;
-        perop_steps(L, k, 0, sink);
+        perop_steps(L, k, 0, sink, true);
         e = sink.err;
         if (e == hipSuccess) e = hipGraphInstantiate(&graphs->exec[k], sink.graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(sink.graph);
@@ -710,7 +752,7 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTimin
       const hipError_t e = hipGraphLaunch(graphs->exec[k], s);
       if (e != hipSuccess) return (int)e;
     } else {
-      perop_steps(L, k, 0, s);
+      perop_steps(L, k, 0, s, true);
     }
     if (tm && hipEventRecord(tm->e1, s) != hipSuccess) return (int)hipErrorUnknown;
     k_lz_ritz<T><<<1, LZ_RITZ_NT, ritz_lds_bytes(L.m), s>>>(L);
@@ -720,6 +762,18 @@ static int perop_impl(const LzInst<T>& L, hipStream_t s, int* host_ctrl, LzTimin
     if (e != hipSuccess) return (int)e;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return (int)e;
+    // a breakdown stopped the restart at step p: build its vector and go on
+    // from there (each pass ends at a later step, so at most m + 1 passes)
+    for (int pass = 0; host_ctrl[LZ_PENDING] && pass <= L.m; ++pass) {
+      perop_steps(L, host_ctrl[LZ_PENDING], 0, s, true, true);
+      k_lz_ritz<T><<<1, LZ_RITZ_NT, ritz_lds_bytes(L.m), s>>>(L);
+      k_lz_rotate<T, MAXC><<<g, b, rot_lds_bytes(L.m), s>>>(L);
+      e = hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(int) * LZ_NCTRL, hipMemcpyDeviceToHost, s);
+      if (e != hipSuccess) return (int)e;
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return (int)e;
+    }
+    if (host_ctrl[LZ_PENDING]) return (int)hipErrorUnknown;
     if (tm && k < L.m) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, tm->e0, tm->e1) != hipSuccess) return (int)hipErrorUnknown;

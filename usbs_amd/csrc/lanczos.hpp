@@ -18,7 +18,10 @@ enum LzCtrl {
   LZ_ERROR = 5,      // nonzero: a bounded wait expired (persistent launch)
   LZ_NBREAK = 6,     // number of breakdowns seen
   LZ_STEPS = 7,      // Lanczos steps taken (operator applications)
-  LZ_NCTRL = 8
+  LZ_PENDING = 8,    // per-op: step i >= 1 met a breakdown in its phase A; the
+                     // restart's later kernels return at once until the host
+                     // has built the breakdown vector and resumed at step i
+  LZ_NCTRL = 9
 };
 
 template <typename T>
