@@ -68,19 +68,19 @@ WORKLOADS = {
 }
 
 
-def workload(name="torus"):
+def workload(name="torus", seed=0):
     from usbs_amd import problems
     if name == "rgg":
         W = problems.random_geometric(1 << 20, 1.3504e-3, 0)
     else:
-        W = problems.torus(100, 200, seed=0, signed=True)
+        W = problems.torus(100, 200, seed=seed, signed=True)
     n = W.shape[0]
-    Om = problems.gaussian_matrix(n, SOLVER["sketch_dim"], 1)
-    q0 = problems.unit_start_vector(n, 2)
+    Om = problems.gaussian_matrix(n, SOLVER["sketch_dim"], 1 + 10 * seed)
+    q0 = problems.unit_start_vector(n, 2 + 10 * seed)
     return W, n, Om, q0
 
 
-def make_problem(name, oracle=False):
+def make_problem(name, oracle=False, seed=0):
     """The named workload as (state, n, m, q0, settings, trace_ub, callback):
     max-cut states come from the graph, the QAP state from its builder
     (library host code; ``oracle`` selects the oracle's max-cut builder and
@@ -98,7 +98,7 @@ def make_problem(name, oracle=False):
         s = QAP_SOLVER
         return (st, N, st.b.size, problems.unit_start_vector(N, 2), s,
                 s["trace_factor"] * (l + 1) * st.SCALE_X, None)
-    W, n, Om, q0 = workload(name)
+    W, n, Om, q0 = workload(name, seed)
     s = SOLVER
     if oracle:
         from oracle import maxcut as omc
@@ -248,6 +248,89 @@ def kron_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_bench(args):
+    """B torus 100x200 max-cut SDPs (seeds 0..B-1), K iterations each: the
+    mixed-precision form with the B solves in flight together (a thread and
+    a context each, so their launches overlap on the device) against the
+    plain FP64 per-op form solving them one after another.  Also the mixed
+    form one after another, which separates the gain of batching from the
+    gain of precision."""
+    import threading
+    # one hardware queue per solve in flight (HIP reads this when it starts;
+    # nothing here has touched the device yet)
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, args.batch)))
+    from usbs_amd import _lib
+    from usbs_amd.usbs import upload_problem
+    B, K, W0 = args.batch, args.steps, args.warmup
+    probs = [make_problem("torus", seed=b) for b in range(B)]
+
+    def concurrent(precision, exec_mode):
+        ctxs = [_lib.Context(0) for _ in range(B)]
+        for ctx, pr in zip(ctxs, probs):
+            upload_problem(ctx, pr[0], pr[1], pr[2])
+        gate = threading.Barrier(B)
+        spans, errs = [None] * B, []
+
+        def work(b):
+            try:
+                if W0 > 0:
+                    run_gpu(ctxs[b], probs[b], W0, precision, exec_mode)
+                gate.wait()
+                t0 = time.perf_counter()
+                _, log, _ = run_gpu(ctxs[b], probs[b], K, precision, exec_mode)
+                spans[b] = (t0, time.perf_counter(), int(log[-1]["t"]))
+            except Exception as e:  # noqa: BLE001 - reported below
+                errs.append(e)
+                gate.abort()
+
+        th = [threading.Thread(target=work, args=(b,)) for b in range(B)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        if errs:
+            raise errs[0]
+        for ctx in ctxs:
+            ctx.close()
+        wall = max(s[1] for s in spans) - min(s[0] for s in spans)
+        return sum(s[2] for s in spans) / wall, wall
+
+    def sequential(precision, exec_mode):
+        ctx = _lib.Context(0)
+        its, wall = 0, 0.0
+        for pr in probs:
+            upload_problem(ctx, pr[0], pr[1], pr[2])
+            if W0 > 0:
+                run_gpu(ctx, pr, W0, precision, exec_mode)
+            el, log, _ = run_gpu(ctx, pr, K, precision, exec_mode)
+            its += int(log[-1]["t"])
+            wall += el
+        ctx.close()
+        return its / wall, wall
+
+    # the solves in flight use the per-op form (what tune picks at this size);
+    # persistent launches would compete for co-residency across the contexts
+    rate_b, wall_b = concurrent("mixed", "perop")
+    rate_s, wall_s = sequential("mixed", "perop")
+    rate_64, wall_64 = sequential("fp64", "perop")
+    out = {"metric": METRIC, "value": round(rate_b, 3), "unit": "USBS iterations/s (all solves)",
+           "n_gpus": 1, "steps": B * K, "warmup": W0, "ms_per_step": round(1e3 / rate_b, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 storage / f64 accumulate",
+           "data": "synthetic (seeded graphs, seeded Omega and q0)",
+           "config": {"workload": "batch of %d max-cut SDPs, torus 100x200 (n=20000, w in {-1,+1}, "
+                                  "seeds 0..%d), %d iterations each" % (B, B - 1, K),
+                      "solves_in_flight": B, "precision": "mixed", "exec_mode": "perop",
+                      **SOLVER},
+           "mixed_sequential": {"value": round(rate_s, 3), "wall_s": round(wall_s, 3)},
+           "plain_fp64": {"exec_mode": "perop", "order": "one solve after another",
+                          "value": round(rate_64, 3), "wall_s": round(wall_64, 3),
+                          "speedup_of_value": round(rate_b / rate_64, 3)},
+           "wall_s": round(wall_b, 3), "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def serving_bench(args):
     """The serving workload: Poisson arrivals, sizes log-uniform in
     [1e3, 1e5], ER mean degree 10 with +-1 weights, 30 % follow-ups flipping
@@ -294,9 +377,11 @@ def main():
                          "auto: persistent where row tiles stay in LDS and fit one per CU, else per-op")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--workload", default="torus",
-                    choices=["torus", "rgg", "qap", "qap-kron", "serving"],
+                    choices=["torus", "rgg", "qap", "qap-kron", "serving", "torus-batch"],
                     help="torus 100x200 (default), RGG 2^20, QAP l=48, the QAP Kronecker "
-                         "matvec alone, or the serving stream")
+                         "matvec alone, the serving stream, or a batch of torus solves in flight "
+                         "together against plain FP64 one after another")
+    ap.add_argument("--batch", type=int, default=16, help="torus-batch: solves in the batch")
     ap.add_argument("--kron-vectors", type=int, default=8192,
                     help="qap-kron: vectors per launch")
     ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
@@ -327,6 +412,9 @@ def main():
 
     if args.workload == "serving":
         serving_bench(args)
+        return
+    if args.workload == "torus-batch":
+        batch_bench(args)
         return
     if args.workload == "qap-kron":
         kron_bench(args)
