@@ -159,7 +159,7 @@ def roofline(stats):
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "kernel": ("k_lz_steps_coop (all Lanczos steps of one restart)" if persistent else
-                       "per-op Lanczos step kernels k_lz_pre/k_lz_A/k_lz_BC of one restart "
+                       "per-op Lanczos step kernels k_lz_A/k_lz_BC of one restart "
                        "(launch gaps included)"),
             "avg_launch_us": round(avg_s * 1e6, 2), "launches": L,
             "steps_per_launch": round(steps / L, 3), "algorithmic_bytes_per_launch": int(per_launch)}
