@@ -80,7 +80,7 @@ def workload(name="torus", seed=0):
     return W, n, Om, q0
 
 
-def make_problem(name, oracle=False, seed=0):
+def make_problem(name, oracle=False, seed=0, reorder="none"):
     """The named workload as (state, n, m, q0, settings, trace_ub, callback):
     max-cut states come from the graph, the QAP state from its builder
     (library host code; ``oracle`` selects the oracle's max-cut builder and
@@ -106,6 +106,10 @@ def make_problem(name, oracle=False, seed=0):
                 omc.compute_max_cut)
     from usbs_amd import maxcut_helpers
     st = maxcut_helpers.initialize_state(W, s["sketch_dim"], Omega=Om)
+    if reorder == "rcm":
+        # node renumbering for SpMV locality (host setup, outside the timed region)
+        from usbs_amd.reorder import permute_state
+        st, q0, _ = permute_state(st, q0)
     return st, n, n, q0, s, s["trace_factor"] * n, maxcut_helpers.compute_max_cut
 
 
@@ -382,6 +386,9 @@ def main():
                          "matvec alone, the serving stream, or a batch of torus solves in flight "
                          "together against plain FP64 one after another")
     ap.add_argument("--batch", type=int, default=16, help="torus-batch: solves in the batch")
+    ap.add_argument("--reorder", default="auto", choices=["auto", "rcm", "none"],
+                    help="node renumbering before upload: reverse Cuthill-McKee (auto: for the "
+                         "randomly numbered RGG only)")
     ap.add_argument("--kron-vectors", type=int, default=8192,
                     help="qap-kron: vectors per launch")
     ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
@@ -429,7 +436,8 @@ def main():
     except ImportError:
         sync = lambda: None  # noqa: E731
 
-    prob = make_problem(args.workload)
+    reorder = args.reorder if args.reorder != "auto" else ("rcm" if args.workload == "rgg" else "none")
+    prob = make_problem(args.workload, reorder=reorder)
     state, n, m = prob[0], prob[1], prob[2]
     ctx = _lib.Context(0)
     upload_problem(ctx, state, n, m)
@@ -449,6 +457,7 @@ def main():
            "data": "synthetic (seeded graph, seeded Omega and q0)",
            "config": {"workload": WORKLOADS[args.workload], "n": n, "m": m,
                       "precision": args.precision, "exec_mode": args.exec_mode,
+                      "node_order": "reverse Cuthill-McKee" if reorder == "rcm" else "as generated",
                       "step": "one USBS iteration", **prob[4]}}
     last = log[-1]
     out["gap"] = {"obj_gap_end": last["obj_gap"], "infeas_gap_end": last["infeas_gap"],

@@ -125,3 +125,32 @@ def _gpu_only(W, iters, k_curr, k_past, r, rho, precision, exec_mode, ctx):
                maxcut_helpers.compute_max_cut, None, sg.C, precision=precision, exec_mode=exec_mode,
                q0=q0, log=log_g, context=ctx)
     return log_g, out
+
+
+def test_usbs_reordered_nodes_same_sdp(gpu_ctx):
+    """A solve of the RCM-renumbered problem is a solve of the same SDP: the
+    first iterations' objective and dual values agree with the solve in the
+    input numbering to rounding, and the sketch comes back in input order."""
+    from usbs_amd.reorder import permute_state, unpermute_rows
+    n = 300
+    W = problems.random_geometric(n, 0.12, 3)
+    Om = problems.gaussian_matrix(n, 5, 1)
+    q0 = problems.unit_start_vector(n, 2)
+    st = maxcut_helpers.initialize_state(W, 5, Omega=Om)
+    st2, q02, perm = permute_state(st, q0)
+    assert not np.array_equal(perm, np.arange(n))
+    logs, outs = [], []
+    for s, q in ((st, q0), (st2, q02)):
+        print("case: RGG(300) 3 its fp64 perop, numbering %d" % len(outs), flush=True)
+        log = []
+        outs.append(usbs(s, n, n, 2.0 * n, HP["trace_factor"], 1.0, HP["beta"], 4, 1, 3,
+                         HP["max_time"], -1.0, -1.0, -1.0, 32, 10, 1e-7, 100, 1.0,
+                         maxcut_helpers.compute_max_cut, None, s.C, precision="fp64",
+                         exec_mode="perop", q0=q, log=log, context=gpu_ctx))
+        logs.append(log)
+    for a, b in zip(logs[0][1:], logs[1][1:]):
+        np.testing.assert_allclose(b["primal_obj"], a["primal_obj"], rtol=1e-8)
+        np.testing.assert_allclose(b["dual_obj"], a["dual_obj"], rtol=1e-8)
+    np.testing.assert_allclose(outs[1].y, outs[0].y, rtol=1e-7, atol=1e-9)
+    P0, P1 = outs[0].P, unpermute_rows(outs[1].P, perm)
+    np.testing.assert_allclose(P1, P0, rtol=1e-6, atol=1e-8 * np.abs(P0).max())
