@@ -152,9 +152,13 @@ __device__ double lz_beta(const LzInst<T>& L, double* red) {
 }
 
 // Rotation Q[:, :k_next] = Q[:, :m] Yr; Q[:, k_next] = Q[:, m] (lanczos.py:217-221).
+// Four output columns at a time, four independent FMA chains, each summed
+// over x in order.  Yr: m x m in LDS followed by 4 readable doubles (reads
+// past the last column feed only outputs that are not stored).
 template <typename T, int MAXC>
 __device__ void tile_rotate(const LzInst<T>& L, int tl, int knext, const double* Yr) {
   const int m = L.m;
+  const int kk = knext < m ? knext : m;
   const int r0 = tl * L.tile;
   const int r1 = min(L.n, r0 + L.tile);
   for (int r = r0 + (int)threadIdx.x; r < r1; r += USBS_NT) {
@@ -162,12 +166,22 @@ __device__ void tile_rotate(const LzInst<T>& L, int tl, int knext, const double*
 #pragma unroll
     for (int x = 0; x < MAXC; ++x) q[x] = (x < m) ? (double)L.Q[r + (long long)x * L.ldq] : 0.0;
     const T qm = L.Q[r + (long long)m * L.ldq];
-    for (int j = 0; j < knext && j < m; ++j) {
-      double s = 0.0;
+    for (int j0 = 0; j0 < kk; j0 += 4) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
       for (int x = 0; x < MAXC; ++x)
-        if (x < m) s += q[x] * Yr[x * m + j];
-      L.Q[r + (long long)j * L.ldq] = (T)s;
+        if (x < m) {
+          const double* y = Yr + x * m + j0;
+          s0 += q[x] * y[0];
+          s1 += q[x] * y[1];
+          s2 += q[x] * y[2];
+          s3 += q[x] * y[3];
+        }
+      T* out = L.Q + r + (long long)j0 * L.ldq;
+      out[0] = (T)s0;
+      if (j0 + 1 < kk) out[L.ldq] = (T)s1;
+      if (j0 + 2 < kk) out[2 * L.ldq] = (T)s2;
+      if (j0 + 3 < kk) out[3 * L.ldq] = (T)s3;
     }
     if (knext < m) L.Q[r + (long long)knext * L.ldq] = qm;
   }
@@ -616,6 +630,7 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate(LzInst<T> L) {
   const int knext = L.ctrl[LZ_K];
   const int mm = L.m * L.m;
   for (int idx = threadIdx.x; idx < mm; idx += USBS_NT) lds[idx] = L.Yr[idx];
+  if (threadIdx.x < 8) lds[mm + threadIdx.x] = 0.0;
   __syncthreads();
   tile_rotate<T, MAXC>(L, blockIdx.x, knext, lds);
 }
@@ -631,6 +646,7 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate_multi(const LzInst<T>* __
   const int knext = L.ctrl[LZ_K];
   const int mm = L.m * L.m;
   for (int idx = threadIdx.x; idx < mm; idx += USBS_NT) lds[idx] = L.Yr[idx];
+  if (threadIdx.x < 8) lds[mm + threadIdx.x] = 0.0;
   __syncthreads();
   tile_rotate<T, MAXC>(L, mp.y, knext, lds);
 }
