@@ -152,8 +152,8 @@ __device__ double lz_beta(const LzInst<T>& L, double* red) {
 }
 
 // Rotation Q[:, :k_next] = Q[:, :m] Yr; Q[:, k_next] = Q[:, m] (lanczos.py:217-221).
-// Four output columns at a time, four independent FMA chains, each summed
-// over x in order.  Yr: m x m in LDS followed by 4 readable doubles (reads
+// Eight output columns at a time, eight independent FMA chains, each summed
+// over x in order.  Yr: m x m in LDS followed by 8 readable doubles (reads
 // past the last column feed only outputs that are not stored).
 template <typename T, int MAXC>
 __device__ void tile_rotate(const LzInst<T>& L, int tl, int knext, const double* Yr) {
@@ -166,22 +166,19 @@ __device__ void tile_rotate(const LzInst<T>& L, int tl, int knext, const double*
 #pragma unroll
     for (int x = 0; x < MAXC; ++x) q[x] = (x < m) ? (double)L.Q[r + (long long)x * L.ldq] : 0.0;
     const T qm = L.Q[r + (long long)m * L.ldq];
-    for (int j0 = 0; j0 < kk; j0 += 4) {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int j0 = 0; j0 < kk; j0 += 8) {
+      double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int x = 0; x < MAXC; ++x)
         if (x < m) {
           const double* y = Yr + x * m + j0;
-          s0 += q[x] * y[0];
-          s1 += q[x] * y[1];
-          s2 += q[x] * y[2];
-          s3 += q[x] * y[3];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s[u] += q[x] * y[u];
         }
       T* out = L.Q + r + (long long)j0 * L.ldq;
-      out[0] = (T)s0;
-      if (j0 + 1 < kk) out[L.ldq] = (T)s1;
-      if (j0 + 2 < kk) out[2 * L.ldq] = (T)s2;
-      if (j0 + 3 < kk) out[3 * L.ldq] = (T)s3;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (j0 + u < kk) out[u * L.ldq] = (T)s[u];
     }
     if (knext < m) L.Q[r + (long long)knext * L.ldq] = qm;
   }
