@@ -36,6 +36,7 @@ METRIC = "time-to-1e-5 rel gap: mixed/batched vs plain FP64 on GPU; p50/p99 solv
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lz_steps_coop_torus100x200_traffic.txt")
 TRAFFIC_FILE_PEROP = os.path.join(ROOT, "profiles", "lz_perop_steps_torus100x200_traffic.txt")
+TRAFFIC_FILE_RGG = os.path.join(ROOT, "profiles", "lz_perop_steps_rgg2e20_rcm_traffic.txt")
 KRON_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_kron_bf16_qap48_8192vec_traffic.txt")
 
 
@@ -141,7 +142,7 @@ def gap_marks(log):
     return marks
 
 
-def roofline(stats):
+def roofline(stats, rgg_rcm=False):
     """Algorithmic bytes per launch of the Lanczos step kernels (the
     persistent kernel, or one restart's per-op launches): per step the CSR
     SpMV nnz*(s+4) + 4(n+1) + 2 n s plus CGS2 over c valid columns
@@ -156,10 +157,14 @@ def roofline(stats):
     per_launch = total / L
     avg_s = stats["lz_kernel_ms"] / L / 1e3
     achieved = per_launch / avg_s / 1e9
-    # counters were taken on the torus workload in FP32 storage; others report None
+    # counters were taken on the torus workload (both forms) and on the RGG in
+    # reverse Cuthill-McKee order (per-op), in FP32 storage; others report None
     persistent = stats.get("lz_form", 1) == 1
-    traffic = (traffic_from(TRAFFIC_FILE if persistent else TRAFFIC_FILE_PEROP)
-               if (n, nnz, s) == (20000, 100000, 4) else None)
+    traffic = None
+    if (n, nnz, s) == (20000, 100000, 4):
+        traffic = traffic_from(TRAFFIC_FILE if persistent else TRAFFIC_FILE_PEROP)
+    elif rgg_rcm and n == (1 << 20) and s == 4 and not persistent:
+        traffic = traffic_from(TRAFFIC_FILE_RGG)
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "kernel": ("k_lz_steps_coop (all Lanczos steps of one restart)" if persistent else
@@ -475,7 +480,8 @@ def main():
                              "obj_gap_end": log64[-1]["obj_gap"],
                              "primal_rel_diff_end": abs(log64[-1]["primal_obj"] - last["primal_obj"])
                              / max(1.0, abs(log64[-1]["primal_obj"]))}
-    out["roofline"] = roofline(stats) if stats else None
+    out["roofline"] = (roofline(stats, rgg_rcm=(args.workload == "rgg" and reorder == "rcm"))
+                       if stats else None)
     out["cpu_baseline"] = (cpu_baseline(args.cpu_iters, args.workload) if args.cpu_iters > 0
                            else None)
     print(json.dumps(out), flush=True)
