@@ -34,9 +34,14 @@
 
 namespace usbs {
 
+// Row tiles: 256 rows up to n = 65536 (at most 256 tiles); above, the
+// smallest power of two >= 512 rows that gives at most 512 tiles, so that a
+// compute unit runs two workgroups of the row-wise kernels (every block sums
+// all tile partials, which keeps the count from growing further).
 int tile_rows_for(int n) {
-  int t = 256;
-  while ((n + t - 1) / t > 256) t *= 2;
+  if (n <= 256 * 256) return 256;
+  int t = 512;
+  while ((n + t - 1) / t > 512) t *= 2;
   return t;
 }
 
