@@ -888,26 +888,35 @@ k_cut_values(int n, int tile, int r, const int* __restrict__ row_ptr, const int*
   }
 }
 
-// out[j] = cut value of candidate j, out[r] = max_j out[j].
+// out[j] = cut value of candidate j, out[r] = max_j out[j].  Lane j sums
+// candidate j's tile partials in tile order (eight loads in flight); lane 0
+// then takes the maximum in candidate order.  r <= 48 < blockDim (64).
 __global__ void k_reduce_cut(const long long* __restrict__ ipart, const double* __restrict__ dpart,
                              int ntiles, int r, int use_int, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double best = -INFINITY;
-  for (int j = 0; j < r; ++j) {
+  __shared__ double vals[64];
+  const int j = threadIdx.x;
+  if (j < r) {
     double v;
     if (use_int) {
       long long s = 0;
+#pragma unroll 8
       for (int t = 0; t < ntiles; ++t) s += ipart[(long long)t * r + j];
       v = (double)s * 0.25;
     } else {
       double s = 0.0;
+#pragma unroll 8
       for (int t = 0; t < ntiles; ++t) s += dpart[(long long)t * r + j];
       v = s;
     }
     out[j] = v;
-    best = (j == 0 || v > best) ? v : best;
+    vals[j] = v;
   }
-  out[r] = best;
+  __syncthreads();
+  if (j == 0) {
+    double best = -INFINITY;
+    for (int q = 0; q < r; ++q) best = (q == 0 || vals[q] > best) ? vals[q] : best;
+    out[r] = best;
+  }
 }
 
 template <typename T>
