@@ -278,6 +278,7 @@ template int launch_tsmm<double, float>(int, const double*, long long, int, cons
 // Each thread owns 4 x 4 output blocks (upper block triangle only); the
 // reduce kernel mirrors the strictly lower blocks.
 #define QB_BS 4
+#define QB_KMAX 16  // k <= 15 (d <= 124)
 template <typename TV>
 __global__ void __launch_bounds__(USBS_NT)
 k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
@@ -288,8 +289,14 @@ k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
   const int d = k * (k + 1) / 2;
   const int dp = (d + QB_BS - 1) / QB_BS * QB_BS;  // padded width
   double* G = lds;  // GRAM_CH x dp
-  int* pb = (int*)(G + GRAM_CH * dp);
+  double* rd = G + GRAM_CH * dp;      // chunk row: data of its first triple
+  double* Vi = rd + GRAM_CH;          // chunk row: V row i of its first triple (QB_KMAX)
+  double* Vj = Vi + GRAM_CH * QB_KMAX;  // and V row j
+  int* pb = (int*)(Vj + GRAM_CH * QB_KMAX);
   int* pc = pb + dp;
+  int* rcnt = pc + dp;                // chunk row: triple count
+  int* ri = rcnt + GRAM_CH;
+  int* rj = ri + GRAM_CH;
   const int nbk = dp / QB_BS;
   const int nblk = nbk * (nbk + 1) / 2;  // upper block triangle
   const int tl = blockIdx.x;
@@ -329,10 +336,46 @@ k_qbase(int m, int mtile, int k, const TV* __restrict__ V, long long ldv,
   __syncthreads();
   for (int c0 = c0t; c0 < c1t; c0 += GRAM_CH) {
     const int rows = min(GRAM_CH, c1t - c0);
+    // A constraint with one triple (max-cut's diagonal, most QAP rows) reads
+    // its two V rows once, into LDS, with every load of the chunk in flight;
+    // others loop over their triples.  The arithmetic is the same either way.
+    for (int rr = threadIdx.x; rr < GRAM_CH; rr += USBS_NT) {
+      int cv = 0, iv = 0, jv = 0;
+      double dv = 0.0;
+      if (rr < rows) {
+        const int c = c0 + rr;
+        const int e0 = con_ptr[c];
+        cv = con_ptr[c + 1] - e0;
+        if (cv >= 1) { iv = ct_i[e0]; jv = ct_j[e0]; dv = ct_d[e0]; }
+      }
+      rcnt[rr] = cv; ri[rr] = iv; rj[rr] = jv; rd[rr] = dv;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < GRAM_CH * k; idx += USBS_NT) {
+      const int rr = idx / k, x = idx - (idx / k) * k;
+      double vi = 0.0, vj = 0.0;
+      if (rcnt[rr] == 1) {
+        vi = (double)V[(long long)ri[rr] + x * ldv];
+        vj = (double)V[(long long)rj[rr] + x * ldv];
+      }
+      Vi[rr * QB_KMAX + x] = vi;
+      Vj[rr * QB_KMAX + x] = vj;
+    }
+    __syncthreads();
     for (int idx = threadIdx.x; idx < GRAM_CH * dp; idx += USBS_NT) {
       const int rr = idx / dp, a = idx - (idx / dp) * dp;
       double v = 0.0;
-      if (rr < rows && a < d) {
+      if (rr < rows && a < d && rcnt[rr] == 1) {
+        const int bb = pb[a], cc = pc[a];
+        const double td = rd[rr];
+        const double vib = Vi[rr * QB_KMAX + bb], vjc = Vj[rr * QB_KMAX + cc];
+        if (bb == cc) {
+          v += td * (vib * vjc);
+        } else {
+          const double vic = Vi[rr * QB_KMAX + cc], vjb = Vj[rr * QB_KMAX + bb];
+          v += td * (vib * vjc) * u + td * (vic * vjb) * u;
+        }
+      } else if (rr < rows && a < d) {
         const int bb = pb[a], cc = pc[a];
         const int c = c0 + rr;
         for (int e = con_ptr[c]; e < con_ptr[c + 1]; ++e) {
@@ -408,8 +451,9 @@ int launch_qbase(const Problem& P, int k, const TV* V, long long ldv, double* pa
   const int d = k * (k + 1) / 2;
   const int dp = (d + QB_BS - 1) / QB_BS * QB_BS;
   const int nbk = dp / QB_BS;
-  if (nbk * (nbk + 1) / 2 > 2 * USBS_NT) return (int)hipErrorInvalidValue;
-  const size_t lds = sizeof(double) * GRAM_CH * dp + sizeof(int) * 2 * dp;
+  if (nbk * (nbk + 1) / 2 > 2 * USBS_NT || k >= QB_KMAX) return (int)hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * (GRAM_CH * dp + GRAM_CH + 2 * GRAM_CH * QB_KMAX) +
+                     sizeof(int) * (2 * dp + 3 * GRAM_CH);
   const int qt = qbase_tile_rows(P.m, d), qnt = (P.m + qt - 1) / qt;
   k_qbase<TV><<<qnt, USBS_NT, lds, s>>>(P.m, qt, k, V, ldv, P.con_ptr, P.ct_i, P.ct_j, P.ct_d,
                                         partial);
