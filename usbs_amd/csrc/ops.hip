@@ -904,23 +904,43 @@ k_cut_values(int n, int tile, int r, const int* __restrict__ row_ptr, const int*
   const int tl = blockIdx.x;
   const int r0 = tl * tile, r1 = min(n, r0 + tile);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int j = 0; j < r; ++j) {
-    long long ia = 0;
-    double da = 0.0;
-    const signed char* s = sg + (long long)j * n;
+  // eight candidates per pass over the tile's CSR (each candidate's sum keeps
+  // its order: rows, then entries, in storage order)
+  constexpr int CG = 8;
+  for (int j0 = 0; j0 < r; j0 += CG) {
+    const int jn = min(CG, r - j0);
+    long long ia[CG];
+    double da[CG];
+#pragma unroll
+    for (int u = 0; u < CG; ++u) { ia[u] = 0; da[u] = 0.0; }
     for (int i = r0 + (int)threadIdx.x; i < r1; i += USBS_NT) {
-      const int si = s[i];
+      int si[CG];
+#pragma unroll
+      for (int u = 0; u < CG; ++u) si[u] = u < jn ? (int)sg[(long long)(j0 + u) * n + i] : 0;
       for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
-        const int sk = s[col[e]];
-        if (use_int) ia += (long long)(cw4[e] * si * sk);
-        else da += cval[e] * (double)(si * sk);
+        const int c = col[e];
+        const int w = use_int ? cw4[e] : 0;
+        const double cvv = use_int ? 0.0 : cval[e];
+#pragma unroll
+        for (int u = 0; u < CG; ++u) {
+          if (u < jn) {
+            const int sk = sg[(long long)(j0 + u) * n + c];
+            if (use_int) ia[u] += (long long)(w * si[u] * sk);
+            else da[u] += cvv * (double)(si[u] * sk);
+          }
+        }
       }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      ia += __shfl_xor(ia, off, 64);
-      da += __shfl_xor(da, off, 64);
+#pragma unroll
+    for (int u = 0; u < CG; ++u) {
+      long long a = ia[u];
+      double b = da[u];
+      for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+      }
+      if (lane == 0 && u < jn) { ired[wid * 48 + j0 + u] = a; dred[wid * 48 + j0 + u] = b; }
     }
-    if (lane == 0) { ired[wid * 48 + j] = ia; dred[wid * 48 + j] = da; }
   }
   __syncthreads();
   for (int j = threadIdx.x; j < r; j += USBS_NT) {
