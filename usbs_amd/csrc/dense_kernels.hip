@@ -210,11 +210,14 @@ __device__ __forceinline__ void ipm_ldlt(double* A, double* rD, double* Lp) {
     const int pw = min(PW, D - 1 - k0);
     if (w == 0) {
       const int r0 = k0 + lane, r1 = k0 + 64 + lane;
+      const int r0c = r0 < D ? r0 : D - 1, r1c = r1 < D ? r1 : D - 1;
       double p0[PW], p1[PW];
 #pragma unroll
       for (int c = 0; c < PW; ++c) {
-        p0[c] = (c < pw && r0 < D && k0 + c <= r0) ? A[r0 * D + k0 + c] : 0.0;
-        p1[c] = (c < pw && r1 < D) ? A[r1 * D + k0 + c] : 0.0;
+        const int col = k0 + c < D ? k0 + c : D - 1;
+        const double v0 = A[r0c * D + col], v1 = A[r1c * D + col];
+        p0[c] = (c < pw && r0 < D && k0 + c <= r0) ? v0 : 0.0;
+        p1[c] = (c < pw && r1 < D) ? v1 : 0.0;
       }
 #pragma unroll
       for (int c = 0; c < PW; ++c) {
@@ -244,28 +247,41 @@ __device__ __forceinline__ void ipm_ldlt(double* A, double* rD, double* Lp) {
     __syncthreads();
     const int kt = k0 + pw;
     if (kt < D) {
+      // Loads are unconditional from clamped in-bounds addresses and masked
+      // by selects (no branch per load); a masked term is 0 * 0, which
+      // leaves v unchanged, so every entry sees the same FMAs as above.
       double aj[NC][PW];
 #pragma unroll
       for (int cc = 0; cc < NC; ++cc) {
         const int j = kt + tc + 16 * cc;
         const int jc = j < D ? j : D - 1;
 #pragma unroll
-        for (int c = 0; c < PW; ++c) aj[cc][c] = (c < pw) ? A[jc * D + k0 + c] : 0.0;
+        for (int c = 0; c < PW; ++c) {
+          const int col = k0 + c < D ? k0 + c : D - 1;
+          const double a = A[jc * D + col];
+          aj[cc][c] = (c < pw) ? a : 0.0;
+        }
       }
       for (int i = kt + tr; i < D; i += 16) {
-        double li[PW];
+        double li[PW], v[NC];
 #pragma unroll
-        for (int c = 0; c < PW; ++c) li[c] = (c < pw) ? Lp[i * PW + c] : 0.0;
+        for (int c = 0; c < PW; ++c) {
+          const double l = Lp[i * PW + c];
+          li[c] = (c < pw) ? l : 0.0;
+        }
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
           const int j = kt + tc + 16 * cc;
-          if (j <= i) {
-            double v = A[i * D + j];
+          v[cc] = A[i * D + (j < D ? j : D - 1)];
+        }
 #pragma unroll
-            for (int c = 0; c < PW; ++c)
-              if (c < pw) v = v - li[c] * aj[cc][c];
-            A[i * D + j] = v;
-          }
+        for (int c = 0; c < PW; ++c)
+#pragma unroll
+          for (int cc = 0; cc < NC; ++cc) v[cc] = v[cc] - li[c] * aj[cc][c];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+          const int j = kt + tc + 16 * cc;
+          if (j <= i) A[i * D + j] = v[cc];
         }
       }
     }
@@ -281,7 +297,10 @@ __device__ __forceinline__ void ipm_ldlt(double* A, double* rD, double* Lp) {
 // Solve (L D L^T) X = R for nrhs <= 4 right-hand sides (R row-major D x 3)
 // with the ipm_ldlt factor.  One wavefront per right-hand side keeps it in
 // registers (rows lane, lane + 64); both substitutions are column-oriented:
-// one broadcast (readlane) and one FMA per row and step.
+// one broadcast (readlane) and one FMA per row and step.  The factor entries
+// of eight steps are loaded together, unconditionally from clamped in-bounds
+// addresses, and masked by a select, so the loads do not wait on the
+// substitution chain.  Same operations in the same order as one step at a time.
 template <int D>
 __device__ __forceinline__ void ipm_ldlt_solve(const double* Lm, const double* rD, double* R, int nrhs) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -290,26 +309,45 @@ __device__ __forceinline__ void ipm_ldlt_solve(const double* Lm, const double* r
     const int i0c = i0 < D ? i0 : D - 1, i1c = i1 < D ? i1 : D - 1;
     double y0 = (i0 < D) ? R[i0 * 3 + w] : 0.0;
     double y1 = (D > 64 && i1 < D) ? R[i1 * 3 + w] : 0.0;
+    constexpr int G = 8;  // factor entries loaded per group, ahead of the group's steps
 #pragma unroll
-    for (int k = 0; k < D - 1; ++k) {
-      const double yk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
-      const double la = (i0 > k && i0 < D) ? Lm[i0c * D + k] : 0.0;
-      y0 -= la * yk;
-      if (D > 64) {
-        const double lb = (i1 > k && i1 < D) ? Lm[i1c * D + k] : 0.0;
-        y1 -= lb * yk;
+    for (int k0 = 0; k0 < D - 1; k0 += G) {
+      double la[G], lb[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int k = k0 + g < D - 1 ? k0 + g : D - 2;
+        la[g] = Lm[i0c * D + k];
+        lb[g] = (D > 64) ? Lm[i1c * D + k] : 0.0;
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int k = k0 + g;
+        if (k < D - 1) {
+          const double yk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
+          y0 -= ((i0 > k && i0 < D) ? la[g] : 0.0) * yk;
+          if (D > 64) y1 -= ((i1 > k && i1 < D) ? lb[g] : 0.0) * yk;
+        }
       }
     }
     y0 *= rD[i0c];
     if (D > 64) y1 *= rD[i1c];
 #pragma unroll
-    for (int k = D - 1; k > 0; --k) {
-      const double xk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
-      const double la = (i0 < k) ? Lm[k * D + i0] : 0.0;
-      y0 -= la * xk;
-      if (D > 64) {
-        const double lb = (i1 < k) ? Lm[k * D + (i1 < D ? i1 : 0)] : 0.0;
-        y1 -= lb * xk;
+    for (int k0 = D - 1; k0 > 0; k0 -= G) {
+      double la[G], lb[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int k = k0 - g > 0 ? k0 - g : 1;
+        la[g] = Lm[k * D + i0c];
+        lb[g] = (D > 64) ? Lm[k * D + i1c] : 0.0;
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int k = k0 - g;
+        if (k > 0) {
+          const double xk = (k < 64) ? readlane_d(y0, k) : readlane_d(y1, k - 64);
+          y0 -= ((i0 < k) ? la[g] : 0.0) * xk;
+          if (D > 64) y1 -= ((i1 < k) ? lb[g] : 0.0) * xk;
+        }
       }
     }
     if (i0 < D) R[i0 * 3 + w] = y0;
