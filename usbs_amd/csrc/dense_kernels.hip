@@ -39,16 +39,23 @@ __device__ __forceinline__ double symkron(const double* T, const double* Si, int
 
 // ---------------------------------------------------------------------------
 // Per-lane K x K Cholesky in registers (K is a compile-time constant so the
-// packed factor stays in VGPRs).  Returns false on a non-positive pivot.
+// packed factor stays in VGPRs).  Returns false on a non-positive pivot.  A
+// lane that meets one carries on with pivot 1 (its factor is discarded)
+// instead of returning, so no load sits behind a branch and the next
+// column's operands load while the square root and division run.
 template <int K>
 __device__ __forceinline__ bool lane_chol(const double* M, const double* dM, double s, double* L,
                                           double* rdiag) {
+  bool ok = true;
 #pragma unroll
   for (int c = 0; c < K; ++c) {
     double d = M[c * K + c] + s * dM[c * K + c];
 #pragma unroll
     for (int p = 0; p < c; ++p) d -= L[c * (c + 1) / 2 + p] * L[c * (c + 1) / 2 + p];
-    if (!(d > 0.0)) return false;
+    if (!(d > 0.0)) {
+      ok = false;
+      d = 1.0;
+    }
     d = sqrt(d);
     const double rd = 1.0 / d;
     L[c * (c + 1) / 2 + c] = d;
@@ -61,7 +68,7 @@ __device__ __forceinline__ bool lane_chol(const double* M, const double* dM, dou
       L[i * (i + 1) / 2 + c] = v * rd;
     }
   }
-  return true;
+  return ok;
 }
 
 // S^{-1} for SPD S: lane j < K solves L L^T x = e_j for column j.
@@ -171,7 +178,11 @@ __device__ void ipm_tables(const IpmLds& L, int k) {
   __syncthreads();
 }
 
-// 0.5 * U (T (x) Sinv + Sinv (x) T) U^T, entry (a, b)  (solver/usbs.py:92-93)
+// 0.5 * U (T (x) Sinv + Sinv (x) T) U^T, entry (a, b)  (solver/usbs.py:92-93).
+// All four (x, y) terms are evaluated, with the operands loaded
+// unconditionally (no branch per load); a term outside na x nb gets weight 0,
+// and fma(0, X, s) = s for finite X, so only the na x nb terms contribute, in
+// the same order.
 __device__ __forceinline__ double symkron(const IpmLds& L, int k, int a, int b) {
   const int b1 = L.pb[a], c1 = L.pc[a], b2 = L.pb[b], c2 = L.pc[b];
   const double* T = L.T;
@@ -179,12 +190,16 @@ __device__ __forceinline__ double symkron(const IpmLds& L, int k, int a, int b) 
   const double u = 0.70710678118654752440;
   const int na = (b1 == c1) ? 1 : 2, nb = (b2 == c2) ? 1 : 2;
   const double ua = (na == 1) ? 1.0 : u, ub = (nb == 1) ? 1.0 : u;
+  const double uab = ua * ub;
   double s = 0.0;
-  for (int x = 0; x < na; ++x) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
     const int p = x == 0 ? b1 : c1, q = x == 0 ? c1 : b1;
-    for (int y = 0; y < nb; ++y) {
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
       const int pp = y == 0 ? b2 : c2, qq = y == 0 ? c2 : b2;
-      s += ua * ub * (T[p * k + pp] * Si[q * k + qq] + Si[p * k + pp] * T[q * k + qq]);
+      const double X = T[p * k + pp] * Si[q * k + qq] + Si[p * k + pp] * T[q * k + qq];
+      s = fma((x < na && y < nb) ? uab : 0.0, X, s);
     }
   }
   return 0.5 * s;
@@ -248,8 +263,8 @@ __device__ __forceinline__ void ipm_ldlt(double* A, double* rD, double* Lp) {
     const int kt = k0 + pw;
     if (kt < D) {
       // Loads are unconditional from clamped in-bounds addresses and masked
-      // by selects (no branch per load); a masked term is 0 * 0, which
-      // leaves v unchanged, so every entry sees the same FMAs as above.
+      // by selects (no branch per load); a masked term is 0 * 0 and leaves v
+      // unchanged, so each entry gets its updates k ascending as above.
       double aj[NC][PW];
 #pragma unroll
       for (int cc = 0; cc < NC; ++cc) {
@@ -300,7 +315,7 @@ __device__ __forceinline__ void ipm_ldlt(double* A, double* rD, double* Lp) {
 // one broadcast (readlane) and one FMA per row and step.  The factor entries
 // of eight steps are loaded together, unconditionally from clamped in-bounds
 // addresses, and masked by a select, so the loads do not wait on the
-// substitution chain.  Same operations in the same order as one step at a time.
+// substitution chain.  The steps run in the same order as one at a time.
 template <int D>
 __device__ __forceinline__ void ipm_ldlt_solve(const double* Lm, const double* rD, double* R, int nrhs) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
