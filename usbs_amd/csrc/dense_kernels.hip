@@ -160,11 +160,13 @@ __device__ IpmLds ipm_layout(double* lds, int k) {
   return L;
 }
 
-static size_t ipm_lds_bytes(int k) {
-  const int d = k * (k + 1) / 2, kk = k * k;
-  const int big = d * d > 5 * kk + 4 * k + 64 ? d * d : 5 * kk + 4 * k + 64;
-  return sizeof(double) * (big + 14 * d + 8 * d + 6 * kk + 16 + 16 + d + 4);
+__host__ __device__ constexpr int ipm_lds_doubles(int k) {
+  return (k * (k + 1) / 2) * (k * (k + 1) / 2) > 5 * k * k + 4 * k + 64
+             ? (k * (k + 1) / 2) * (k * (k + 1) / 2) + 23 * (k * (k + 1) / 2) + 6 * k * k + 36
+             : 5 * k * k + 4 * k + 64 + 23 * (k * (k + 1) / 2) + 6 * k * k + 36;
 }
+
+static size_t ipm_lds_bytes(int k) { return sizeof(double) * ipm_lds_doubles(k); }
 
 __device__ void ipm_tables(const IpmLds& L, int k) {
   const int d = k * (k + 1) / 2;
@@ -479,20 +481,30 @@ __device__ __forceinline__ void ipm_step(const IpmLds& L, int max_iters, double 
   step = st;
 }
 
-// out = (cq Gq) v for the d x d matrix Gq in global memory: 4 threads per
-// row, each summing every 4th column, combined in a fixed order.
-__device__ __forceinline__ void qmatvec(const double* Gq, double cq, const double* v, int d, double* out) {
+// out = (cq Gq) v for the D x D matrix Gq (an LDS copy): 4 threads per row,
+// each summing every 4th column, combined in a fixed order.  The row's
+// operands are loaded unconditionally from clamped in-bounds addresses, all
+// in flight together; columns past D add nothing.
+template <int D>
+__device__ __forceinline__ void qmatvec(const double* Gq, double cq, const double* v, double* out) {
   const int t = threadIdx.x, q = t & 3;
-  for (int a0 = 0; a0 < d; a0 += IPM_NT / 4) {
+  for (int a0 = 0; a0 < D; a0 += IPM_NT / 4) {
     const int a = a0 + (t >> 2);
-    double s = 0.0;
-    if (a < d) {
-      const double* g = Gq + (size_t)a * d;
-      for (int b = q; b < d; b += 4) s += (cq * g[b]) * v[b];
+    const double* g = Gq + (a < D ? a : D - 1) * D;
+    double gv[(D + 3) / 4], vv[(D + 3) / 4];
+#pragma unroll
+    for (int i = 0; i < (D + 3) / 4; ++i) {
+      const int b = q + 4 * i, bc = b < D ? b : D - 1;
+      gv[i] = g[bc];
+      vv[i] = v[bc];
     }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < (D + 3) / 4; ++i)
+      if (q + 4 * i < D) s += (cq * gv[i]) * vv[i];
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
-    if (a < d && q == 0) out[a] = s;
+    if (a < D && q == 0) out[a] = s;
   }
   __syncthreads();
 }
@@ -504,6 +516,21 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
   extern __shared__ double lds[];
   const int t = threadIdx.x;
   IpmLds L = ipm_layout(lds, k);
+  // LDS copy of the D x D block Gq (after the shared layout; the launch adds
+  // its bytes): every IPM iteration reads it three times.
+  double* gq = lds + ipm_lds_doubles(K);
+  {
+    constexpr int NL = (D * D + IPM_NT - 1) / IPM_NT;
+    double g[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int idx = t + IPM_NT * i;
+      g[i] = A.Gq[idx < D * D ? idx : D * D - 1];
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i)
+      if (t + IPM_NT * i < D * D) gq[t + IPM_NT * i] = g[i];
+  }
   ipm_tables(L, k);
   const double pw0 = pow(0.9, (double)t);
   double* sc = A.sc;
@@ -536,7 +563,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
   __syncthreads();
   dense_svec(L.S, k, L.sS);
   __syncthreads();
-  qmatvec(A.Gq, cq, L.sS, d, L.tmpv);
+  qmatvec<D>(gq, cq, L.sS, L.tmpv);
   for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = L.tmpv[a] + eta * L.q12[a] + L.h1[a];
   __syncthreads();
   mat_from_svec(L.tmpv, k, L.T);
@@ -552,7 +579,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
     const double trS = dotd(L.sI, L.sS, d, L.red);
     const double k1 = (1 - trS - eta) / omega;
     const double k2 = (zeta / eta) + q22;
-    qmatvec(A.Gq, cq, L.sS, d, L.F1);
+    qmatvec<D>(gq, cq, L.sS, L.F1);
     for (int a = t; a < d; a += IPM_NT)
       L.F1[a] = (L.F1[a] + eta * L.q12[a] + L.h1[a]) + (-L.sT[a] + omega * L.sI[a]);
     __syncthreads();
@@ -564,7 +591,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
     // U = [q12, svec(I)], W = [k1 q12 + svec(I), q12 - k2 svec(I)] / den
     for (int idx = t; idx < d * d; idx += IPM_NT) {
       const int a = idx / D, b = idx - (idx / D) * D;
-      L.coeff[idx] = cq * A.Gq[idx] + symkron(L, k, a, b);
+      L.coeff[idx] = cq * gq[idx] + symkron(L, k, a, b);
     }
     dense_svec(L.Si, k, L.tmpv);
     for (int a = t; a < d; a += IPM_NT) {
@@ -591,7 +618,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
     __syncthreads();
     const double p2 = dotd(L.tmpv, L.dsS, d, L.red);
     const double d_omega = (-F2 + mu / eta - zeta + k2 * (mu / omega + trS + eta - 1) - p2) / den;
-    qmatvec(A.Gq, cq, L.dsS, d, L.dsT);
+    qmatvec<D>(gq, cq, L.dsS, L.dsT);
     for (int a = t; a < d; a += IPM_NT)
       L.dsT[a] = L.dsT[a] + d_eta * L.q12[a] + d_omega * L.sI[a] + L.F1[a];
     __syncthreads();
@@ -721,7 +748,8 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
   }
 
 int launch_ipm_quad(const IpmArgs& a, hipStream_t s) {
-#define Q_CALL(KK) k_ipm_quad<KK><<<1, IPM_NT, ipm_lds_bytes(KK), s>>>(a)
+#define Q_CALL(KK) \
+  k_ipm_quad<KK><<<1, IPM_NT, ipm_lds_bytes(KK) + sizeof(double) * (KK * (KK + 1) / 2) * (KK * (KK + 1) / 2), s>>>(a)
   USBS_K_SWITCH(a.k, Q_CALL)
 #undef Q_CALL
   return (int)hipGetLastError();
