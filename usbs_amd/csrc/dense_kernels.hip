@@ -126,6 +126,29 @@ __device__ __forceinline__ double pd_step(const double* M, const double* dM, int
   return 0.0;
 }
 
+// pd_step for M = S and M = T together when max_iters <= 128: threads
+// 0..127 test S's candidates, threads 128..255 T's (pwh = 0.9^(threadIdx.x
+// mod 128)), so the two searches share one Cholesky latency.  Same candidates
+// and the same maxima as two pd_step calls.
+template <int K>
+__device__ __forceinline__ void pd_step2(const double* S, const double* dS, const double* T,
+                                         const double* dT, int max_iters, double pwh, double* red,
+                                         double& cS, double& cT) {
+  const int t = threadIdx.x, h = t >> 7;
+  double v = 0.0;
+  if ((t & 127) < max_iters) {
+    double L[K * (K + 1) / 2], rd[K];
+    if (lane_chol<K>(h ? T : S, h ? dT : dS, pwh, L, rd)) v = pwh;
+  }
+  v = wave_max(v);
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  cS = red[1] > red[0] ? red[1] : red[0];
+  cT = red[3] > red[2] ? red[3] : red[2];
+  __syncthreads();
+}
+
 __device__ __forceinline__ void mat_from_svec(const double* v, int k, double* M) {
   dense_svec_inv(v, k, M);
   __syncthreads();
@@ -448,7 +471,7 @@ __device__ __forceinline__ void ipm_center(const IpmLds& L, int k, double& zeta,
 
 // Shared step: sizes, line search, update; returns the new mu and step.
 template <int K>
-__device__ __forceinline__ void ipm_step(const IpmLds& L, int max_iters, double pw0, double& eta, double& zeta,
+__device__ __forceinline__ void ipm_step(const IpmLds& L, int max_iters, double pw0, double pwh, double& eta, double& zeta,
                          double& omega, double& mu, double& step, double d_eta, double d_zeta,
                          double d_omega) {
   constexpr int D = K * (K + 1) / 2;
@@ -462,8 +485,13 @@ __device__ __forceinline__ void ipm_step(const IpmLds& L, int max_iters, double 
   double st = -1.0 / rmin;
   st = st > 1.0 ? 1.0 : st;
   st = 0.99 * ((st <= 0.0) ? 1.0 : st);
-  const double cS = pd_step<K>(L.S, L.dS, max_iters, pw0, L.red);
-  const double cT = pd_step<K>(L.T, L.dT, max_iters, pw0, L.red);
+  double cS, cT;
+  if (max_iters <= IPM_NT / 2) {
+    pd_step2<K>(L.S, L.dS, L.T, L.dT, max_iters, pwh, L.red, cS, cT);
+  } else {
+    cS = pd_step<K>(L.S, L.dS, max_iters, pw0, L.red);
+    cT = pd_step<K>(L.T, L.dT, max_iters, pw0, L.red);
+  }
   const double cand = cS < cT ? cS : cT;
   st = st < cand ? st : cand;
   __syncthreads();
@@ -532,7 +560,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
       if (t + IPM_NT * i < D * D) gq[t + IPM_NT * i] = g[i];
   }
   ipm_tables(L, k);
-  const double pw0 = pow(0.9, (double)t);
+  const double pw0 = pow(0.9, (double)t), pwh = pow(0.9, (double)(t & 127));
   double* sc = A.sc;
   const double tu = A.tu, rho = A.rho;
   const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
@@ -623,7 +651,7 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
       L.dsT[a] = L.dsT[a] + d_eta * L.q12[a] + d_omega * L.sI[a] + L.F1[a];
     __syncthreads();
     const double d_zeta = dotd(L.q12, L.dsS, d, L.red) + d_eta * q22 + d_omega + F2;
-    ipm_step<K>(L, A.max_iters, pw0, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ipm_step<K>(L, A.max_iters, pw0, pwh, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
     ++it;
   }
   // outputs (usbs.py:191-192) and the bundle factor (usbs.py:568-570)
@@ -670,7 +698,7 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
   const int t = threadIdx.x;
   IpmLds L = ipm_layout(lds, k);
   ipm_tables(L, k);
-  const double pw0 = pow(0.9, (double)t);
+  const double pw0 = pow(0.9, (double)t), pwh = pow(0.9, (double)(t & 127));
   const double trXb = sc[SC_TRXB] > 0.0 ? sc[SC_TRXB] : tu;
   const double pb = sc[SC_PB];
   double* g1 = L.h1;
@@ -719,7 +747,7 @@ k_ipm_lb(int max_iters, double tu, double eps, const double* Glb, double* sc) {
       L.dsT[a] = d_omega * L.sI[a] - L.sT[a] + g1[a] + omega * L.sI[a];
     __syncthreads();
     const double d_zeta = d_omega - zeta + g2 + omega;
-    ipm_step<K>(L, max_iters, pw0, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
+    ipm_step<K>(L, max_iters, pw0, pwh, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
     ++it;
   }
   dense_svec(L.S, k, L.sS);
