@@ -18,6 +18,40 @@ __device__ __forceinline__ double dotd(const double* x, const double* y, int d, 
   return block_sum1(s, red);
 }
 
+// Three dot products of length d in one block reduction: each component is
+// reduced exactly as dotd reduces it (same terms, same tree), with one set
+// of barriers for all three.
+__device__ __forceinline__ void dotd3(const double* x1, const double* y1, const double* x2,
+                                      const double* y2, const double* x3, const double* y3, int d,
+                                      double* red, double& r1, double& r2, double& r3) {
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int a = threadIdx.x; a < d; a += blockDim.x) {
+    s1 += x1[a] * y1[a];
+    s2 += x2[a] * y2[a];
+    s3 += x3[a] * y3[a];
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  s3 = wave_sum(s3);
+  const int nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[wid] = s1;
+    red[nw + wid] = s2;
+    red[2 * nw + wid] = s3;
+  }
+  __syncthreads();
+  r1 = 0.0;
+  r2 = 0.0;
+  r3 = 0.0;
+  for (int w = 0; w < nw; ++w) {
+    r1 += red[w];
+    r2 += red[nw + w];
+    r3 += red[2 * nw + w];
+  }
+  __syncthreads();
+}
+
 // 0.5 * U (T (x) Sinv + Sinv (x) T) U^T, entry (a, b)  (solver/usbs.py:92-93)
 __device__ __forceinline__ double symkron(const double* T, const double* Si, int k, int a, int b) {
   int b1, c1, b2, c2;
@@ -638,19 +672,22 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
     __syncthreads();
     woodbury_solve<D>(L.coeff, L.dsS, L.q12, L.sI, L.dsT, L.ordv, 2, L.R, L.rD, L.lp, L.red);
     // back-substitution for the other directions (usbs.py:111-122)
-    for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = k1 * L.q12[a] + L.sI[a];
+    // p1, p2 and q12 . dsS (for d_zeta) in one reduction; tmpv and ordv
+    // hold k1 q12 + svec(I) and q12 - k2 svec(I)
+    for (int a = t; a < d; a += IPM_NT) {
+      L.tmpv[a] = k1 * L.q12[a] + L.sI[a];
+      L.ordv[a] = L.q12[a] - k2 * L.sI[a];
+    }
     __syncthreads();
-    const double p1 = dotd(L.tmpv, L.dsS, d, L.red);
+    double p1, p2, q12dsS;
+    dotd3(L.tmpv, L.dsS, L.ordv, L.dsS, L.q12, L.dsS, d, L.red, p1, p2, q12dsS);
     const double d_eta = (-p1 - (mu / omega) - trS - eta + 1 + k1 * (-F2 + mu / eta - zeta)) / den;
-    for (int a = t; a < d; a += IPM_NT) L.tmpv[a] = L.q12[a] - k2 * L.sI[a];
-    __syncthreads();
-    const double p2 = dotd(L.tmpv, L.dsS, d, L.red);
     const double d_omega = (-F2 + mu / eta - zeta + k2 * (mu / omega + trS + eta - 1) - p2) / den;
     qmatvec<D>(gq, cq, L.dsS, L.dsT);
     for (int a = t; a < d; a += IPM_NT)
       L.dsT[a] = L.dsT[a] + d_eta * L.q12[a] + d_omega * L.sI[a] + L.F1[a];
     __syncthreads();
-    const double d_zeta = dotd(L.q12, L.dsS, d, L.red) + d_eta * q22 + d_omega + F2;
+    const double d_zeta = q12dsS + d_eta * q22 + d_omega + F2;
     ipm_step<K>(L, A.max_iters, pw0, pwh, eta, zeta, omega, mu, step, d_eta, d_zeta, d_omega);
     ++it;
   }
