@@ -480,8 +480,8 @@ __device__ __forceinline__ double ipm_mu(const IpmLds& L, int k, int d, double e
   dense_svec(L.S, k, L.sS);
   dense_svec(L.T, k, L.sT);
   __syncthreads();
-  const double st = dotd(L.sS, L.sT, d, L.red);
-  const double trS = dotd(L.sI, L.sS, d, L.red);
+  double st, trS, unused;
+  dotd3(L.sS, L.sT, L.sI, L.sS, L.sI, L.sS, d, L.red, st, trS, unused);
   return (st + eta * zeta + omega * (1 - trS - eta)) / (2 * k + 4.0);
 }
 
@@ -638,14 +638,15 @@ __global__ void __launch_bounds__(IPM_NT) k_ipm_quad(IpmArgs A) {
   const double eps = A.eps;
   while (it < A.max_iters && mu > eps && step > eps * eps) {
     // sS, sT are current (ipm_mu recomputes them)
-    const double trS = dotd(L.sI, L.sS, d, L.red);
+    double trS, q12sS, unused;
+    dotd3(L.sI, L.sS, L.q12, L.sS, L.q12, L.sS, d, L.red, trS, q12sS, unused);
     const double k1 = (1 - trS - eta) / omega;
     const double k2 = (zeta / eta) + q22;
     qmatvec<D>(gq, cq, L.sS, L.F1);
     for (int a = t; a < d; a += IPM_NT)
       L.F1[a] = (L.F1[a] + eta * L.q12[a] + L.h1[a]) + (-L.sT[a] + omega * L.sI[a]);
     __syncthreads();
-    double F2 = dotd(L.q12, L.sS, d, L.red) + eta * q22 + h2;
+    double F2 = q12sS + eta * q22 + h2;
     F2 = F2 + (-zeta + omega);
     lane_spd_inverse<K>(L.S, L.Si);
     const double den = k1 * k2 + 1;
