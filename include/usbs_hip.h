@@ -121,6 +121,28 @@ int usbs_qap_kron_matvec(usbs_ctx* ctx, int32_t l, const double* W, const double
                          int64_t nvec, const double* Y, int32_t precision, int32_t reps,
                          double* U, double* kernel_ms);
 
+/*
+ * The constraint operator A(.) on the columns of U, summed over them
+ * (apply_A_operator_batched, solver/utils.py:27-31; apply_A_operator_slim,
+ * utils.py:11-16, is p = 1):
+ *   out[c] = sum_t [A_indices[t][0] == c] A_data[t] sum_q U[i_t][q] U[j_t][q],
+ * with (c, i_t, j_t) = A_indices[t] (nA x 3 int32 row-major), U n x p row-major,
+ * out m.  Needs no uploaded problem.
+ */
+int usbs_apply_A_operator(usbs_ctx* ctx, int64_t n, int64_t m, int64_t nA,
+                          const int32_t* A_indices, const double* A_data, int32_t p,
+                          const double* U, double* out);
+
+/*
+ * The adjoint A*(z) applied to the columns of U (apply_A_adjoint_batched,
+ * solver/utils.py:34-37; apply_A_adjoint_slim, utils.py:19-24, is p = 1):
+ *   out[i][q] = sum_t [i_t == i] A_data[t] z[c_t] U[j_t][q];
+ * U and out n x p row-major, z m.  Needs no uploaded problem.
+ */
+int usbs_apply_A_adjoint(usbs_ctx* ctx, int64_t n, int64_t m, int64_t nA,
+                         const int32_t* A_indices, const double* A_data, const double* z,
+                         int32_t p, const double* U, double* out);
+
 /* Parameters of usbs() (solver/usbs.py:436-459), plus the execution choices. */
 typedef struct usbs_params {
   int32_t k_curr, k_past;
@@ -138,6 +160,10 @@ typedef struct usbs_params {
   double max_time, obj_gap_eps, infeas_gap_eps, max_infeas_eps;
   double subprob_eps, cond_exp_base;
   double SCALE_C, SCALE_X;
+  double cut_scale;    /* rounding reports sgn^T (C / cut_scale) sgn for the uploaded */
+                       /* (scaled) C; the reference's callers pass C / SCALE_C      */
+                       /* (scripts/warm_start_maxcut_usbs.py:121), i.e. SCALE_C;    */
+                       /* 0 means SCALE_C                                           */
 } usbs_params;
 
 /* One condition-check record: the fields of the reference's log line
@@ -179,6 +205,7 @@ typedef struct usbs_cgal_params {
   double max_time, obj_gap_eps, infeas_gap_eps, max_infeas_eps;
   double subprob_eps, cond_exp_base;
   double SCALE_C, SCALE_X;
+  double cut_scale;    /* as usbs_params.cut_scale                                  */
 } usbs_cgal_params;
 
 /*

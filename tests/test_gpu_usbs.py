@@ -154,3 +154,59 @@ def test_usbs_reordered_nodes_same_sdp(gpu_ctx):
     np.testing.assert_allclose(outs[1].y, outs[0].y, rtol=1e-7, atol=1e-9)
     P0, P1 = outs[0].P, unpermute_rows(outs[1].P, perm)
     np.testing.assert_allclose(P1, P0, rtol=1e-6, atol=1e-8 * np.abs(P0).max())
+
+
+def test_usbs_scaled_state_and_cut_scale(gpu_ctx):
+    """A state scaled as the reference's warm starts scale it (SCALE_X = 1/n,
+    SCALE_C = 1/|C|_F, utils/maxcut_helpers.py:134-135), with the rounding
+    callback given C / SCALE_C as the reference's scripts pass it
+    (scripts/warm_start_maxcut_usbs.py:121): the records (unscaled primal
+    objective) follow the oracle, and every cut value is the oracle's exact
+    integer cut of the unscaled graph."""
+    from oracle import common as ocommon
+    from usbs_amd import common
+    W = problems.erdos_renyi(120, 0.08, seed=11)
+    n = W.shape[0]
+    Om = problems.gaussian_matrix(n, 5, 1)
+    q0 = problems.unit_start_vector(n, 2)
+    C0 = omc.laplacian_cost(W)
+    sc, sx = 1.0 / np.linalg.norm(C0.data), 1.0 / n
+    so = ocommon.scale_sdp_state(omc.initialize_state(W, 5, Om)._replace(SCALE_C=sc, SCALE_X=sx))
+    sg = common.scale_sdp_state(maxcut_helpers.initialize_state(W, 5, Omega=Om)._replace(
+        SCALE_C=sc, SCALE_X=sx))
+    tu = 2.0 * n * sx
+    log_o, log_g = [], []
+    ousbs.usbs(so, n, n, tu, 2.0, 1.0, 0.25, 4, 1, 4, 1e9, -1.0, -1.0, -1.0, 32, 10, 1e-7, 100,
+               1.0, omc.compute_max_cut, None, so.C / sc, q0, log=log_o)
+    usbs(sg, n, n, tu, 2.0, 1.0, 0.25, 4, 1, 4, 1e9, -1.0, -1.0, -1.0, 32, 10, 1e-7, 100, 1.0,
+         maxcut_helpers.compute_max_cut, None, sg.C / sc, q0=q0, log=log_g, context=gpu_ctx)
+    assert len(log_o) == len(log_g) == 5
+    for a, b in zip(log_o[1:], log_g[1:]):
+        for key in ("primal_obj", "obj_ub", "obj_gap", "infeas_gap"):
+            np.testing.assert_allclose(b[key], a[key], rtol=1e-5, atol=1e-9, err_msg=key)
+        assert b["callback_val"] == a["callback_val"]
+        assert b["callback_val"] == int(b["callback_val"]) and b["callback_val"] > 0
+
+
+def test_usbs_geometric_condition_schedule(gpu_ctx):
+    """cond_exp_base = 2 (the QAP scripts' default): condition checks, and so
+    records and roundings, at t = 0, 1, 3, 7, 15 for max_iters = 15, as
+    utils/loop.py:85-98 schedules them; the early records follow the oracle."""
+    W = problems.erdos_renyi(120, 0.08, seed=11)
+    n = W.shape[0]
+    Om = problems.gaussian_matrix(n, 5, 1)
+    q0 = problems.unit_start_vector(n, 2)
+    so = omc.initialize_state(W, 5, Om)
+    sg = maxcut_helpers.initialize_state(W, 5, Omega=Om)
+    log_o, log_g = [], []
+    ousbs.usbs(so, n, n, 2.0 * n, 2.0, 1.0, 0.25, 4, 1, 15, 1e9, -1.0, -1.0, -1.0, 32, 10, 1e-7,
+               100, 2.0, omc.compute_max_cut, None, so.C, q0, log=log_o)
+    out = usbs(sg, n, n, 2.0 * n, 2.0, 1.0, 0.25, 4, 1, 15, 1e9, -1.0, -1.0, -1.0, 32, 10, 1e-7,
+               100, 2.0, maxcut_helpers.compute_max_cut, None, sg.C, q0=q0, log=log_g,
+               context=gpu_ctx)
+    assert [int(r["t"]) for r in log_o] == [0, 1, 3, 7, 15]
+    assert [int(r["t"]) for r in log_g] == [0, 1, 3, 7, 15]
+    for a, b in zip(log_o[1:3], log_g[1:3]):
+        np.testing.assert_allclose(b["primal_obj"], a["primal_obj"], rtol=1e-5)
+        assert b["callback_val"] == a["callback_val"]
+    assert np.isfinite(out.primal_obj)

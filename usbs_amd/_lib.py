@@ -4,8 +4,11 @@ The library is built in-tree by ``python -m usbs_amd.build``.  There is no
 CPU fallback: if the shared library is missing or fails to load, importing
 anything that needs it raises ``RuntimeError``.
 """
+import atexit
 import ctypes
 import os
+import threading
+import weakref
 
 import numpy as np
 
@@ -99,8 +102,30 @@ def as_i32(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
 
 
+# Every live context, so that they are destroyed (streams, graphs, device
+# memory) at interpreter exit while the HIP runtime is still up, rather than
+# from __del__ during module teardown.
+_live = weakref.WeakSet()
+_live_lock = threading.Lock()
+
+
+def close_all():
+    """Destroy every context that is still open (registered with atexit)."""
+    with _live_lock:
+        ctxs = list(_live)
+    for c in ctxs:
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+atexit.register(close_all)
+
+
 class Context:
-    """An owned device context (one HIP stream + resident problem data)."""
+    """An owned device context (one HIP stream + resident problem data).
+    Usable as a context manager; close() is idempotent."""
 
     def __init__(self, device=0):
         self._lib = lib()
@@ -111,6 +136,15 @@ class Context:
         self.h = h
         self.n = None
         self.m = None
+        with _live_lock:
+            _live.add(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def check(self, rc):
         if rc != USBS_OK:

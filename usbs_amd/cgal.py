@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib, problems
 from .common import SDPState
-from .usbs import RECORD_FIELDS, _finite, upload_problem, usbs_record
+from .usbs import RECORD_FIELDS, _finite, cut_scale_of, upload_problem, usbs_record
 
 
 class usbs_cgal_params(ctypes.Structure):
@@ -23,7 +23,8 @@ class usbs_cgal_params(ctypes.Structure):
                  "rounding", "precision", "exec_mode", "reserved")] + \
                [(name, ctypes.c_double) for name in
                 ("trace_ub", "beta0", "max_time", "obj_gap_eps", "infeas_gap_eps",
-                 "max_infeas_eps", "subprob_eps", "cond_exp_base", "SCALE_C", "SCALE_X")]
+                 "max_infeas_eps", "subprob_eps", "cond_exp_base", "SCALE_C", "SCALE_X",
+                 "cut_scale")]
 
 
 _lib.EXTRA_SIGNATURES["usbs_cgal_solve"] = [
@@ -65,7 +66,9 @@ def cgal(sdp_state, n, m, trace_ub, beta0, max_iters, max_time, obj_gap_eps, inf
                            infeas_gap_eps=_finite(infeas_gap_eps, -1e300),
                            max_infeas_eps=_finite(max_infeas_eps, -1e300), subprob_eps=subprob_eps,
                            cond_exp_base=cond_exp_base, SCALE_C=float(sdp_state.SCALE_C),
-                           SCALE_X=float(sdp_state.SCALE_X))
+                           SCALE_X=float(sdp_state.SCALE_X),
+                           cut_scale=(cut_scale_of(sdp_state, callback_nonstatic_args) if rounding
+                                      else 0.0))
     y0, z0 = _lib.as_f64(sdp_state.y), _lib.as_f64(sdp_state.z)
     if q0 is None:
         q0 = problems.unit_start_vector(n, 0)

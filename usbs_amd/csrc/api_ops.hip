@@ -144,7 +144,7 @@ int usbs_maxcut_round(usbs_ctx* h, int32_t r, const double* P, const double* Ome
   double* out = S.alloc<double>(r + 1);
   if (!sg || !ip || !dp || !out) return USBS_ERR_OOM;
   hipStream_t s = c->stream;
-  if (launch_round<double>(Pb, r, PO, PO + r * ld, ld, G, sc + SC_NYS_RHO, sg, ip, dp, out, s))
+  if (launch_round<double>(Pb, cut_weights_of(Pb), r, PO, PO + r * ld, ld, G, sc + SC_NYS_RHO, sg, ip, dp, out, s))
     return set_err(c, USBS_ERR_HIP, "rounding launch failed");
   std::vector<double> hv(r + 1);
   std::vector<signed char> hs;

@@ -151,6 +151,7 @@ struct CgalSolver {
   float* op32 = nullptr;
   T* PO = nullptr;  // P | Omega, n x 2r column-major
   signed char* sg = nullptr;
+  CutWeights cutw{};  // weights of the rounding's cut values (C / cut_scale)
   LzInst<T> lz{};
   LzInst<T>* d_lz = nullptr;
   int* d_map = nullptr;
@@ -204,6 +205,10 @@ struct CgalSolver {
         !(sc = alloc<double>(SC_COUNT)) || !(q0 = alloc<double>(n)) ||
         !(sg = alloc<signed char>((size_t)n * r)))
       return USBS_ERR_OOM;
+    if (p.rounding) {
+      const int rcw = make_cut_weights(c, pool, p.cut_scale > 0.0 ? p.cut_scale : p.SCALE_C, &cutw);
+      if (rcw) return rcw;
+    }
     int rc = make_lanczos<T>(c, pool, *P, mL, 1, p.lanczos_max_restarts, p.subprob_eps,
                              sizeof(T) == 4 ? (const T*)op32 : (const T*)op64, lz);
     if (rc) return rc;
@@ -244,7 +249,7 @@ struct CgalSolver {
     if (r <= 0 || !p.rounding) return USBS_OK;
     CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, P_(), ld, r, PO, ld, 2 * r, partial, PtPO, s)));
     CK(launch_nystrom_core(n, r, PtPO, OtO, Gn, lam, sc, s));
-    CK(launch_round<T>(*P, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, sg, ipart, dpart, cut, s));
+    CK(launch_round<T>(*P, cutw, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, sg, ipart, dpart, cut, s));
     return USBS_OK;
   }
 
@@ -472,6 +477,9 @@ extern "C" int usbs_cgal_solve(usbs_ctx* h, const usbs_cgal_params* prm, const d
     return set_err(c, USBS_ERR_INVALID, "unknown exec_mode");
   for (int i = 0; i < c->P.m; ++i)
     if (!std::isfinite(y0[i]) || !std::isfinite(z0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite y0/z0");
+  CoopGuard coop(p.exec_mode == USBS_EXEC_PERSISTENT ? (const void*)c : nullptr);
+  if (p.exec_mode == USBS_EXEC_PERSISTENT && !coop.ok())
+    return set_err(c, USBS_ERR_STATE, COOP_BUSY_MSG);
   if (hipSetDevice(c->device) != hipSuccess) return set_err(c, USBS_ERR_HIP, "hipSetDevice");
   if (p.precision == USBS_PREC_FP64)
     return run_cgal<double>(c, p, y0, z0, tr_X0, primal_obj0, Omega, P0, q0, y_out, z_out, P_out,

@@ -156,6 +156,7 @@ static int build_problem(Ctx* c, int64_t n, int64_t m, int64_t nnzC, const int32
   P.diag_A = diag;
   P.has_ineq = ineq;
   P.cut_integer = cut_int;
+  P.cval_host = cval;
   auto& pool = c->prob_bufs;
   std::vector<double> tdata(Ad, Ad + nA);
   if (!(P.row_ptr = upload(c, pool, row_ptr.data(), row_ptr.size()))) return USBS_ERR_OOM;
@@ -514,6 +515,8 @@ int usbs_lanczos_smallest(usbs_ctx* h, double C_coef, const double* y, const dou
     if (!std::isfinite(y[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite adjoint_left_vec");
   for (int i = 0; i < c->P.n; ++i)
     if (!std::isfinite(q0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite q0");
+  CoopGuard coop(exec_mode == USBS_EXEC_PERSISTENT ? (const void*)c : nullptr);
+  if (exec_mode == USBS_EXEC_PERSISTENT && !coop.ok()) return set_err(c, USBS_ERR_STATE, COOP_BUSY_MSG);
   if (hipSetDevice(c->device) != hipSuccess) return set_err(c, USBS_ERR_HIP, "hipSetDevice");
   if (precision == USBS_PREC_FP64)
     return run_lanczos_standalone<double>(c, C_coef, y, q0, num_desired, m, max_restarts, tolerance,
@@ -546,6 +549,8 @@ int usbs_lanczos_smallest_batch(usbs_ctx* const* ctxs, int ninst, double C_coef,
   }
   if (precision != USBS_PREC_FP64 && precision != USBS_PREC_MIXED)
     return set_err(c0, USBS_ERR_INVALID, "unknown precision");
+  CoopGuard coop(c0);
+  if (!coop.ok()) return set_err(c0, USBS_ERR_STATE, COOP_BUSY_MSG);
   if (hipSetDevice(c0->device) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "hipSetDevice");
   for (int q = 1; q < ninst; ++q)  // work queued on other contexts' streams finishes first
     if (hipStreamSynchronize(cs[q]->stream) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "synchronise");

@@ -8,7 +8,12 @@
 #include <vector>
 
 #include "../../include/usbs_hip.h"
+#include "coop_guard.hpp"
 #include "lanczos.hpp"
+
+#define COOP_BUSY_MSG                                                                      \
+  "a persistent (cooperative) Lanczos launch is in flight on another context of this "   \
+  "process; concurrent solves must use exec_mode per-op"
 
 namespace usbs {
 
@@ -39,6 +44,7 @@ struct Problem {
   int* col = nullptr;           // nnz
   double* cval = nullptr;       // nnz: C in the merged pattern (0 where C has no entry)
   int* cw4 = nullptr;           // nnz: round(4*C) when cut_integer
+  std::vector<double> cval_host;  // C in the merged pattern (host copy: cut weights of C / s)
   int* slot_ptr = nullptr;      // nnz+1: triples per pattern slot
   int* slot_trip = nullptr;     // nA
   int* tcon = nullptr;          // nA: triple constraint index

@@ -2,7 +2,10 @@
 // assembly, SpMM, tall-skinny products and Grams, the sketch update, the
 // constraint map A(F F^T) with the dual/infeasibility reductions, the
 // objective, and the max-cut rounding.
-#include "engine.hpp"
+#include "solve.hpp"
+
+#include <cmath>
+#include <vector>
 
 namespace usbs {
 
@@ -956,7 +959,8 @@ k_cut_values(int n, int tile, int r, const int* __restrict__ row_ptr, const int*
 // candidate j's tile partials in tile order (eight loads in flight); lane 0
 // then takes the maximum in candidate order.  r <= 48 < blockDim (64).
 __global__ void k_reduce_cut(const long long* __restrict__ ipart, const double* __restrict__ dpart,
-                             int ntiles, int r, int use_int, double* __restrict__ out) {
+                             int ntiles, int r, int use_int, double inv_scale,
+                             double* __restrict__ out) {
   __shared__ double vals[64];
   const int j = threadIdx.x;
   if (j < r) {
@@ -970,7 +974,7 @@ __global__ void k_reduce_cut(const long long* __restrict__ ipart, const double* 
       double s = 0.0;
 #pragma unroll 8
       for (int t = 0; t < ntiles; ++t) s += dpart[(long long)t * r + j];
-      v = s;
+      v = s * inv_scale;
     }
     out[j] = v;
     vals[j] = v;
@@ -983,26 +987,54 @@ __global__ void k_reduce_cut(const long long* __restrict__ ipart, const double* 
   }
 }
 
+int make_cut_weights(Ctx* c, std::vector<DevBuf>& pool, double scale, CutWeights* out) {
+  const Problem& P = c->P;
+  *out = cut_weights_of(P);
+  if (scale == 1.0) return USBS_OK;
+  if (!(scale > 0.0) || !std::isfinite(scale))
+    return set_err(c, USBS_ERR_INVALID, "cut scale must be positive and finite");
+  std::vector<int> w(P.nnz > 0 ? P.nnz : 1, 0);
+  bool integer = true;
+  for (long long e = 0; e < P.nnz; ++e) {
+    const double v4 = 4.0 * P.cval_host[e] / scale;
+    const double rv = std::nearbyint(v4);
+    if (!(std::fabs(v4) <= 1e9) || std::fabs(v4 - rv) > 1e-9 * std::fmax(1.0, std::fabs(v4))) {
+      integer = false;
+      break;
+    }
+    w[e] = (int)rv;
+  }
+  out->inv_scale = 1.0 / scale;
+  out->use_int = integer ? 1 : 0;
+  if (!integer) return USBS_OK;
+  int* d = (int*)ctx_alloc(c, pool, sizeof(int) * w.size());
+  if (!d) return USBS_ERR_OOM;
+  if (hipMemcpy(d, w.data(), sizeof(int) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return set_err(c, USBS_ERR_HIP, "upload of the cut weights");
+  out->cw4 = d;
+  return USBS_OK;
+}
+
 template <typename T>
-int launch_round(const Problem& P, int r, const T* Pm, const T* Om, long long ld, const double* G,
-                 const double* rho, signed char* sg, long long* ipart, double* dpart, double* out,
-                 hipStream_t s) {
+int launch_round(const Problem& P, const CutWeights& cw, int r, const T* Pm, const T* Om,
+                 long long ld, const double* G, const double* rho, signed char* sg,
+                 long long* ipart, double* dpart, double* out, hipStream_t s) {
   if (r > 48) return (int)hipErrorInvalidValue;
   k_round_signs<T><<<(P.n + USBS_NT - 1) / USBS_NT, USBS_NT, 0, s>>>(P.n, r, Pm, Om, ld, G, rho, sg);
   int rc = (int)hipGetLastError();
   if (rc) return rc;
-  const int use_int = P.cut_integer ? 1 : 0;
-  k_cut_values<<<P.ntiles, USBS_NT, 0, s>>>(P.n, P.tile, r, P.row_ptr, P.col, P.cw4, P.cval, use_int,
+  const int use_int = cw.use_int;
+  k_cut_values<<<P.ntiles, USBS_NT, 0, s>>>(P.n, P.tile, r, P.row_ptr, P.col, cw.cw4, P.cval, use_int,
                                             sg, ipart, dpart);
   rc = (int)hipGetLastError();
   if (rc) return rc;
-  k_reduce_cut<<<1, 64, 0, s>>>(ipart, dpart, P.ntiles, r, use_int, out);
+  k_reduce_cut<<<1, 64, 0, s>>>(ipart, dpart, P.ntiles, r, use_int, cw.inv_scale, out);
   return (int)hipGetLastError();
 }
-template int launch_round<double>(const Problem&, int, const double*, const double*, long long,
+template int launch_round<double>(const Problem&, const CutWeights&, int, const double*, const double*, long long,
                                   const double*, const double*, signed char*, long long*, double*,
                                   double*, hipStream_t);
-template int launch_round<float>(const Problem&, int, const float*, const float*, long long,
+template int launch_round<float>(const Problem&, const CutWeights&, int, const float*, const float*, long long,
                                  const double*, const double*, signed char*, long long*, double*,
                                  double*, hipStream_t);
 

@@ -61,6 +61,7 @@ struct Solver {
          *Epast = nullptr, *S_io = nullptr, *eta_io = nullptr, *PtPO = nullptr, *OtO = nullptr,
          *Gn = nullptr, *lam = nullptr, *cut = nullptr, *sc = nullptr, *q0 = nullptr;
   signed char* sg = nullptr;
+  CutWeights cutw{};       // weights of the rounding's cut values (C / cut_scale)
   LzInst<T> lz{}, qr{};
   LzInst<T>* d_lz_k = nullptr;   // device copies for the persistent form (nd = k, nd = kc)
   LzInst<T>* d_lz_kc = nullptr;
@@ -165,6 +166,10 @@ struct Solver {
         !(cut = alloc<double>(rr + 1)) || !(sc = alloc<double>(SC_COUNT)) || !(q0 = alloc<double>(n)) ||
         !(sg = alloc<signed char>((size_t)n * rr)))
       return USBS_ERR_OOM;
+    if (p.rounding && r > 0) {
+      const int rcw = make_cut_weights(c, pool, p.cut_scale > 0.0 ? p.cut_scale : p.SCALE_C, &cutw);
+      if (rcw) return rcw;
+    }
     int rc = make_lanczos<T>(c, pool, *P, mL, k, p.lanczos_max_restarts, p.subprob_eps,
                              sizeof(T) == 4 ? (const T*)op32 : (const T*)op64, lz);
     if (rc) return rc;
@@ -332,7 +337,7 @@ struct Solver {
     if (r <= 0 || !p.rounding) return USBS_OK;
     CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, P_(), ld, r, PO, ld, 2 * r, partial, PtPO, s)));
     CK(launch_nystrom_core(n, r, PtPO, OtO, Gn, lam, sc, s));
-    CK(launch_round<T>(*P, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, sg, ipart, dpart, cut, s));
+    CK(launch_round<T>(*P, cutw, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, sg, ipart, dpart, cut, s));
     k_copy1<<<1, 64, 0, s>>>(sc + SC_CB, cut + r);
     CK((int)hipGetLastError());
     return USBS_OK;
@@ -705,6 +710,9 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   if (!p.warm)
     for (int i = 0; i < c->P.m; ++i)
       if (!std::isfinite(y0[i]) || !std::isfinite(z0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite y0/z0");
+  const bool coop_form = p.exec_mode == USBS_EXEC_PERSISTENT || p.exec_mode == USBS_EXEC_TUNE;
+  CoopGuard coop(coop_form ? (const void*)c : nullptr);
+  if (coop_form && !coop.ok()) return set_err(c, USBS_ERR_STATE, COOP_BUSY_MSG);
   if (hipSetDevice(c->device) != hipSuccess) return set_err(c, USBS_ERR_HIP, "hipSetDevice");
   if (p.precision == USBS_PREC_FP64)
     return run_solve<double>(c, p, y0, z0, tr_X0, primal_obj0, Omega, P0, q0, y_out, z_out, P_out,

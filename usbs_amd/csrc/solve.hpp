@@ -87,8 +87,28 @@ template <typename TF>
 int launch_upsilon(const Problem& P, int k, const TF* F, long long ldf, const double* eta,
                    const double* zbar, const double* y, double rho, double* ups, double* partial,
                    double* out, hipStream_t s);
+// Weights of the cut values sgn^T (C / s) sgn: integer 4 C_ij / s when every
+// one is an integer (exact int64 sums), else the FP64 values of C scaled by
+// inv_scale = 1 / s at the end.
+struct CutWeights {
+  const int* cw4 = nullptr;
+  int use_int = 0;
+  double inv_scale = 1.0;
+};
+// The uploaded C itself (s = 1).
+inline CutWeights cut_weights_of(const Problem& P) {
+  CutWeights w;
+  w.cw4 = P.cw4;
+  w.use_int = P.cut_integer ? 1 : 0;
+  return w;
+}
+// s != 1: integer weights round(4 C_ij / s) built from the host copy of C into
+// `pool` when each is within 1e-9 relative of an integer (C / s is the
+// unscaled cost of a state whose C was multiplied by SCALE_C = 1 / s), else
+// the FP64 path with inv_scale.  Returns 0 or a USBS_ERR_* code.
+int make_cut_weights(Ctx* c, std::vector<DevBuf>& pool, double s, CutWeights* out);
 template <typename T>
-int launch_round(const Problem& P, int r, const T* Pm, const T* Om, long long ld, const double* G,
+int launch_round(const Problem& P, const CutWeights& cw, int r, const T* Pm, const T* Om, long long ld, const double* G,
                  const double* rho, signed char* sg, long long* ipart, double* dpart, double* out,
                  hipStream_t s);
 

@@ -86,6 +86,13 @@ class SolveServer:
                  k_curr=4, k_past=1, sketch_dim=10, trace_factor=2.0, rho=0.01, beta=0.25,
                  max_iters=20, lanczos_inner=32, lanczos_restarts=10, subprob_eps=1e-7,
                  subprob_max_iters=100, q0_seed=2):
+        if slots > 1 and exec_mode != "perop":
+            # a persistent (cooperative) launch is sized to the whole device; two
+            # of them from concurrent slots could each wait for workgroups the
+            # other holds, so the library admits one at a time and concurrent
+            # slots use per-op launches
+            raise ValueError("exec_mode=%r cannot run in %d concurrent slots: concurrent solves "
+                             "use exec_mode='perop'" % (exec_mode, slots))
         self.ctxs = [_lib.Context(device) for _ in range(slots)]
         self.held = [-1] * slots          # rid whose final state each context keeps
         self.precision, self.exec_mode = precision, exec_mode

@@ -21,6 +21,7 @@ after the reference's select the execution:
 import ctypes
 
 import numpy as np
+import scipy.sparse
 
 from . import _lib, problems
 from .common import SDPState
@@ -35,7 +36,7 @@ class usbs_params(ctypes.Structure):
                [(name, ctypes.c_double) for name in
                 ("trace_ub", "trace_factor", "rho", "beta", "max_time", "obj_gap_eps",
                  "infeas_gap_eps", "max_infeas_eps", "subprob_eps", "cond_exp_base", "SCALE_C",
-                 "SCALE_X")]
+                 "SCALE_X", "cut_scale")]
 
 
 RECORD_FIELDS = ("t", "time", "pen_dual_obj", "cand_pen_dual_obj", "lb_spec_est", "primal_obj",
@@ -55,6 +56,32 @@ _lib.EXTRA_SIGNATURES["usbs_solve"] = [
 
 def _finite(x, default):
     return float(x) if np.isfinite(x) else default
+
+
+def cut_scale_of(sdp_state, callback_nonstatic_args):
+    """The factor s with sdp_state.C = s * callback_nonstatic_args, i.e. the
+    cost the rounding callback prices cuts with (the reference's scripts pass
+    C / SCALE_C, scripts/warm_start_maxcut_usbs.py:121).  None means SCALE_C.
+    The native rounding works on the uploaded C, so any other matrix raises."""
+    if callback_nonstatic_args is None:
+        return float(sdp_state.SCALE_C)
+    r1, c1, v1, sh1 = coo_parts(sdp_state.C)
+    r2, c2, v2, sh2 = coo_parts(callback_nonstatic_args)
+    if sh1 != sh2:
+        raise ValueError("callback_nonstatic_args has shape %s, C has %s" % (sh2, sh1))
+    A = scipy.sparse.csr_matrix((v1, (r1, c1)), shape=sh1)
+    B = scipy.sparse.csr_matrix((v2, (r2, c2)), shape=sh2)
+    A.sum_duplicates()
+    B.sum_duplicates()
+    amax = abs(A).max() if A.nnz else 0.0
+    bmax = abs(B).max() if B.nnz else 0.0
+    if amax == 0.0 or bmax == 0.0:
+        return float(sdp_state.SCALE_C)
+    s = amax / bmax
+    if abs(A - s * B).max() > 1e-12 * amax:
+        raise NotImplementedError("the native max-cut rounding prices cuts with the uploaded C; "
+                                  "callback_nonstatic_args must be C / s for a constant s")
+    return float(s)
 
 
 def upload_problem(ctx, sdp_state, n, m):
@@ -100,7 +127,8 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
                       infeas_gap_eps=_finite(infeas_gap_eps, -1e300),
                       max_infeas_eps=_finite(max_infeas_eps, -1e300), subprob_eps=subprob_eps,
                       cond_exp_base=cond_exp_base, SCALE_C=float(sdp_state.SCALE_C),
-                      SCALE_X=float(sdp_state.SCALE_X))
+                      SCALE_X=float(sdp_state.SCALE_X),
+                      cut_scale=cut_scale_of(sdp_state, callback_nonstatic_args) if rounding else 0.0)
     y0 = _lib.as_f64(sdp_state.y)
     z0 = _lib.as_f64(sdp_state.z)
     if q0 is None:
