@@ -1,24 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the USBS hot path on one MI355X.
 
-Workload: the max-cut SDP of a 2-D toroidal grid 100 x 200 (n = 20 000
-vertices, 40 000 edges, weights uniform on {-1, +1} from seed 0), solved by
-usbs() with k_curr = 10, k_past = 1, sketch_dim = 10, trace_factor = 2,
-rho = 0.01, beta = 0.25, Lanczos 32 inner iterations x 10 restarts at
+Default workload: the max-cut SDP of a 2-D toroidal grid 100 x 200
+(n = 20 000 vertices, 40 000 edges, weights uniform on {-1, +1} from seed 0)
+solved by usbs() with k_curr = 10, k_past = 1, sketch_dim = 10,
+trace_factor = 2, beta = 0.25, Lanczos 32 inner iterations x 10 restarts at
 tolerance 1e-7, subproblem eps 1e-7 / 100 iterations.
 
-A step is one USBS iteration (prox-bundle IPM, sketch update, objective and
-infeasibility, Lanczos oracle on -C + A*(ycand), lower-bound IPM, descent
-test, bundle QR, max-cut rounding and the stop test).  The timed region is
-one usbs() call of K iterations on a context whose problem already sits in
-device memory; W warm-up iterations run first in a separate call.
+The headline ("value") is the metric itself: seconds until the relative
+objective gap (solver/usbs.py:646-647) first reaches 1e-5, for the
+mixed-precision form, on a context whose problem already sits in device
+memory, at rho = 3 (the torus does not reach 1e-5 within minutes at the
+reference's rho = 0.01; rho = 3 reaches it in about 3 000 iterations).  The
+same run reports:
 
-Printed (one JSON line): USBS iterations per second of the mixed-precision
-persistent form ("value"), ms per iteration, the plain FP64 per-op form on
-the same K iterations and the speed-up, the objective gap reached, the
-roofline of the Lanczos step kernel measured with HIP events on the
-solver's stream during the timed call, and the FP64 CPU restatement (the
-oracle) timed on the host on a bounded sample of the same workload.
+* plain FP64 (per-op) from the identical start, to the same gap, and the ratio;
+* a batch of torus problems (seeds 0..B-1) solved together (one context and
+  host thread each, launches overlapping on the device) to a 1e-3 gap,
+  against plain FP64 solving them one after another;
+* p50 / p99 latency and solves/s of a short serving stream;
+* throughput (USBS iterations/s) over K iterations at rho = 0.01, with the
+  roofline of the Lanczos step kernels measured with HIP events on the
+  solver's stream, and the gap both forms reach in those iterations;
+* the FP64 CPU restatement (the oracle) on a bounded sample, with all host
+  cores and with one.
 """
 import argparse
 import json
@@ -40,6 +45,11 @@ TRAFFIC_FILE_RGG = os.path.join(ROOT, "profiles", "lz_perop_steps_rgg2e20_rcm_tr
 KRON_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_kron_bf16_qap48_8192vec_traffic.txt")
 
 
+def progress(msg):
+    """A line on stderr as each leg of a run ends (the JSON result goes to stdout)."""
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
+
+
 def traffic_from(path):
     """traffic_bytes_per_launch recorded in a counter summary, or None."""
     if not os.path.exists(path):
@@ -51,6 +61,10 @@ def traffic_from(path):
 
 SOLVER = dict(trace_factor=2.0, rho=0.01, beta=0.25, k_curr=10, k_past=1, lanczos_inner=32,
               lanczos_restarts=10, subprob_eps=1e-7, subprob_max_iters=100, sketch_dim=10)
+# time-to-gap legs: the same settings at rho = 3
+GAP_RHO = 3.0
+GAP_TARGET = 1e-5
+BATCH_GAP_TARGET = 1e-3
 # The lifted QAP with the QAP script's settings (k_curr=2, k_past=0,
 # sketch_dim=l, rho=0.1, 100 restarts, 15 subproblem iterations) and
 # trace_factor 2 (trace_ub = 2 after SCALE_X = 1/(l+1)).
@@ -114,15 +128,20 @@ def make_problem(name, oracle=False, seed=0, reorder="none"):
     return st, n, n, q0, s, s["trace_factor"] * n, maxcut_helpers.compute_max_cut
 
 
-def run_gpu(ctx, prob, iters, precision, exec_mode, timing=False):
+def run_gpu(ctx, prob, iters, precision, exec_mode, timing=False, rho=None, gap_eps=None,
+            max_time=float("inf")):
+    """usbs() on the resident problem: `iters` iterations, or until the
+    objective gap reaches gap_eps (infeasibility not part of the stop test)."""
     from usbs_amd.usbs import usbs
     state, n, m, q0, s, tu, cb = prob
     log = []
     if timing:
         ctx.set_timing(True)
+    ge = -float("inf") if gap_eps is None else gap_eps
+    other = -float("inf") if gap_eps is None else float("inf")
     t0 = time.perf_counter()
-    usbs(state, n, m, tu, s["trace_factor"], s["rho"], s["beta"], s["k_curr"],
-         s["k_past"], iters, float("inf"), -float("inf"), -float("inf"), -float("inf"),
+    usbs(state, n, m, tu, s["trace_factor"], rho if rho is not None else s["rho"], s["beta"],
+         s["k_curr"], s["k_past"], iters, max_time, ge, other, other,
          s["lanczos_inner"], s["lanczos_restarts"], s["subprob_eps"], s["subprob_max_iters"], 1.0,
          cb, None, state.C, precision=precision, exec_mode=exec_mode,
          q0=q0, log=log, context=ctx, upload=False)
@@ -133,9 +152,9 @@ def run_gpu(ctx, prob, iters, precision, exec_mode, timing=False):
     return el, log, stats
 
 
-def gap_marks(log):
+def gap_marks(log, thresholds=(1e-1, 1e-2, 1e-3, 1e-4, 1e-5)):
     marks = {}
-    for thr in (1e-1, 1e-2, 1e-3, 1e-4, 1e-5):
+    for thr in thresholds:
         hit = [r for r in log if r["t"] > 0 and r["obj_gap"] <= thr]
         marks["%g" % thr] = ({"iteration": int(hit[0]["t"]), "seconds": round(hit[0]["time"], 4),
                               "infeas_gap": hit[0]["infeas_gap"]} if hit else None)
@@ -174,25 +193,52 @@ def roofline(stats, rgg_rcm=False):
             "steps_per_launch": round(steps / L, 3), "algorithmic_bytes_per_launch": int(per_launch)}
 
 
-def cpu_baseline(iters, name="torus"):
-    """The FP64 CPU restatement (oracle) on the same workload, `iters`
-    iterations including the initial Lanczos call, BLAS threads capped."""
+def cpu_baseline(iters, name="torus", iters_single=0):
+    """The FP64 CPU restatement (oracle) on the same workload: `iters`
+    iterations including the initial Lanczos call with BLAS on every core of
+    this process's affinity, and `iters_single` iterations on one core."""
     import threadpoolctl
     from oracle import usbs as ousbs
-    cores = min(16, len(os.sched_getaffinity(0)))
-    with threadpoolctl.threadpool_limits(limits=cores):
-        st, n, m, q0, s, tu, cb = make_problem(name, oracle=True)
-        log = []
-        t0 = time.perf_counter()
-        ousbs.usbs(st, n, m, tu, s["trace_factor"], s["rho"], s["beta"],
-                   s["k_curr"], s["k_past"], iters, 1e300, -1e300, -1e300, -1e300,
-                   s["lanczos_inner"], s["lanczos_restarts"], s["subprob_eps"],
-                   s["subprob_max_iters"], 1.0, cb, None, st.C, q0, log=log)
-        el = time.perf_counter() - t0
-    return {"value": round(iters / el, 4), "unit": "USBS iterations/s", "cores": cores,
-            "kind": "port",
-            "sample": "FP64 NumPy/SciPy restatement, %s, %d iterations incl. the initial "
-                      "Lanczos call (%.1f s), BLAS threads %d" % (name, iters, el, cores)}
+    # every core this process may use, within the thread budget the host sets
+    # (OMP_NUM_THREADS: the GPU's share of the host's cores)
+    cores = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+
+    def timed(n_iter, threads):
+        with threadpoolctl.threadpool_limits(limits=threads):
+            st, n, m, q0, s, tu, cb = make_problem(name, oracle=True)
+            log = []
+            t0 = time.perf_counter()
+            ousbs.usbs(st, n, m, tu, s["trace_factor"], s["rho"], s["beta"],
+                       s["k_curr"], s["k_past"], n_iter, 1e300, -1e300, -1e300, -1e300,
+                       s["lanczos_inner"], s["lanczos_restarts"], s["subprob_eps"],
+                       s["subprob_max_iters"], 1.0, cb, None, st.C, q0, log=log)
+            return time.perf_counter() - t0
+
+    el = timed(iters, cores)
+    out = {"value": round(iters / el, 4), "unit": "USBS iterations/s", "cores": cores,
+           "kind": "port", "host_cpus": os.cpu_count(),
+           "affinity_cpus": len(os.sched_getaffinity(0)),
+           "blas": [dict(api=d.get("internal_api"), threads=d.get("num_threads"))
+                    for d in threadpoolctl.threadpool_info()],
+           "sample": "FP64 NumPy/SciPy restatement, %s, %d iterations incl. the initial "
+                     "Lanczos call (%.1f s), BLAS threads %d" % (name, iters, el, cores)}
+    if iters_single > 0:
+        el1 = timed(iters_single, 1)
+        single = {"value": round(iters_single / el1, 4), "unit": "USBS iterations/s", "cores": 1,
+                  "sample": "%d iterations (%.1f s)" % (iters_single, el1)}
+        if single["value"] > out["value"]:
+            # BLAS threads do not pay at this size: the baseline is the faster
+            # single-core figure, the all-core one is kept beside it
+            multi = {k: out[k] for k in ("value", "cores", "sample")}
+            out.update(value=single["value"], cores=1,
+                       sample="FP64 NumPy/SciPy restatement, %s, %s, BLAS threads 1"
+                              % (name, single["sample"]))
+            out["all_cores"] = multi
+        else:
+            out["single_thread"] = single
+    return out
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
@@ -375,6 +421,169 @@ def serving_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def gap_leg(ctx, prob, precision, exec_mode, target, max_iters, max_time):
+    """One solve from the problem's start until the objective gap reaches
+    `target` (or max_iters / max_time): the wall time of the usbs() call."""
+    el, log, _ = run_gpu(ctx, prob, max_iters, precision, exec_mode, rho=GAP_RHO, gap_eps=target,
+                         max_time=max_time)
+    last = log[-1]
+    reached = bool(last["obj_gap"] <= target)
+    t = int(last["t"])
+    return {"precision": precision, "exec_mode": exec_mode, "reached": reached,
+            "seconds": round(el, 4) if reached else None, "wall_s": round(el, 4), "iterations": t,
+            "ms_per_iteration": round(1e3 * el / max(t, 1), 4), "obj_gap": last["obj_gap"],
+            "infeas_gap": last["infeas_gap"], "max_infeas": last["max_infeas"],
+            "primal_obj": last["primal_obj"], "obj_ub": last["obj_ub"],
+            "cut_value": None if last["callback_val"] != last["callback_val"] else last["callback_val"],
+            "marks": gap_marks(log)}
+
+
+def gap_batch_leg(B, target, max_iters, max_time):
+    """B torus problems (seeds 0..B-1) to `target`: the mixed form with every
+    solve in flight together (a context and host thread each) against plain
+    FP64 per-op solving them one after another."""
+    import threading
+    from usbs_amd import _lib
+    from usbs_amd.usbs import upload_problem
+    probs = [make_problem("torus", seed=b) for b in range(B)]
+    ctxs = [_lib.Context(0) for _ in range(B)]
+    try:
+        for ctx, pr in zip(ctxs, probs):
+            upload_problem(ctx, pr[0], pr[1], pr[2])
+            run_gpu(ctx, pr, 3, "mixed", "perop", rho=GAP_RHO)   # module load, first launches
+            run_gpu(ctx, pr, 3, "fp64", "perop", rho=GAP_RHO)
+        gate = threading.Barrier(B)
+        res, errs = [None] * B, []
+
+        def work(b):
+            try:
+                gate.wait()
+                t0 = time.perf_counter()
+                el, log, _ = run_gpu(ctxs[b], probs[b], max_iters, "mixed", "perop", rho=GAP_RHO,
+                                     gap_eps=target, max_time=max_time)
+                res[b] = (t0, time.perf_counter(), int(log[-1]["t"]), log[-1]["obj_gap"] <= target)
+            except Exception as e:  # noqa: BLE001 - reported below
+                errs.append(e)
+                gate.abort()
+
+        th = [threading.Thread(target=work, args=(b,)) for b in range(B)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        if errs:
+            raise errs[0]
+        wall_b = max(r[1] for r in res) - min(r[0] for r in res)
+        progress("batched leg: %d mixed solves in flight, %.2f s" % (B, wall_b))
+        seq = []
+        for b in range(B):
+            el, log, _ = run_gpu(ctxs[b], probs[b], max_iters, "fp64", "perop", rho=GAP_RHO,
+                                 gap_eps=target, max_time=max_time)
+            seq.append((el, int(log[-1]["t"]), log[-1]["obj_gap"] <= target))
+            progress("batched leg: fp64 solve %d of %d in %.2f s" % (b + 1, B, el))
+    finally:
+        for ctx in ctxs:
+            ctx.close()
+    wall_s = sum(x[0] for x in seq)
+    return {"problems": B, "target_gap": target, "rho": GAP_RHO,
+            "workload": "torus 100x200, w in {-1,+1}, seeds 0..%d" % (B - 1),
+            "mixed_in_flight": {"wall_s": round(wall_b, 3), "iterations": [r[2] for r in res],
+                                "all_reached": all(r[3] for r in res), "exec_mode": "perop"},
+            "plain_fp64_one_after_another": {"wall_s": round(wall_s, 3),
+                                             "iterations": [x[1] for x in seq],
+                                             "all_reached": all(x[2] for x in seq),
+                                             "exec_mode": "perop"},
+            "speedup": round(wall_s / wall_b, 3)}
+
+
+def serving_leg(requests, rate, slots, iters):
+    from usbs_amd import serving
+    reqs = serving.request_stream(requests, rate, 1000, 100000, 10.0, 0.3, 0.01, seed=0)
+    srv = serving.SolveServer(slots=slots, precision="mixed", exec_mode="perop", max_iters=iters)
+    try:
+        srv.run(serving.request_stream(slots, 1e6, 1000, 1001, seed=1), realtime=False)
+        srv.held = [-1] * slots
+        results, summ = srv.run(reqs, realtime=True)
+    finally:
+        for c in srv.ctxs:
+            c.close()
+    return {"requests": len(results), "rate_per_s": rate, "slots": slots,
+            "iterations_per_solve": iters, "solves_per_s": round(summ["solves_per_s"], 4),
+            "p50_latency_s": round(summ["p50_latency_s"], 4),
+            "p99_latency_s": round(summ["p99_latency_s"], 4),
+            "warm_starts": {"device": summ["warm_device"], "host": summ["warm_host"]},
+            "sizes": [r.n for r in results]}
+
+
+def gap_bench(args):
+    """The default run: time to a 1e-5 objective gap (mixed vs plain FP64),
+    the batched leg, the serving leg, throughput with the roofline, and the
+    CPU baseline."""
+    from usbs_amd import _lib
+    from usbs_amd.usbs import upload_problem
+    prob = make_problem("torus")
+    state, n, m = prob[0], prob[1], prob[2]
+    ctx = _lib.Context(0)
+    try:
+        upload_problem(ctx, state, n, m)
+        # warm-up: module load, tune's choice of the Lanczos form, first launches
+        if args.warmup > 0:
+            run_gpu(ctx, prob, args.warmup, args.precision, args.exec_mode, rho=GAP_RHO)
+            run_gpu(ctx, prob, args.warmup, "fp64", "perop", rho=GAP_RHO)
+        progress("warm-up done")
+        mixed = gap_leg(ctx, prob, args.precision, args.exec_mode, GAP_TARGET, args.gap_max_iters,
+                        args.gap_max_time)
+        progress("mixed to gap %g: %s s, %d iterations" % (GAP_TARGET, mixed["seconds"],
+                                                           mixed["iterations"]))
+        fp64 = gap_leg(ctx, prob, "fp64", "perop", GAP_TARGET, args.gap_max_iters,
+                       args.gap_max_time)
+        progress("fp64 to gap %g: %s s, %d iterations" % (GAP_TARGET, fp64["seconds"],
+                                                          fp64["iterations"]))
+        # throughput at the reference's rho = 0.01, Lanczos step kernels timed
+        el, log, stats = run_gpu(ctx, prob, args.steps, args.precision, args.exec_mode,
+                                 timing=(args.exec_mode != "perop"))
+        el64, log64, _ = run_gpu(ctx, prob, args.steps, "fp64", "perop")
+        progress("throughput legs done")
+    finally:
+        ctx.close()
+    K, K64 = int(log[-1]["t"]), int(log64[-1]["t"])
+    thr = {"rho": SOLVER["rho"], "steps": K, "value": round(K / el, 3), "unit": "USBS iterations/s",
+           "ms_per_step": round(1e3 * el / max(K, 1), 4),
+           "plain_fp64_ms_per_step": round(1e3 * el64 / max(K64, 1), 4),
+           "obj_gap_end": log[-1]["obj_gap"], "plain_fp64_obj_gap_end": log64[-1]["obj_gap"],
+           "marks": gap_marks(log), "plain_fp64_marks": gap_marks(log64)}
+    batch = gap_batch_leg(args.gap_batch, BATCH_GAP_TARGET, args.gap_max_iters,
+                          args.gap_max_time) if args.gap_batch > 0 else None
+    progress("batched leg done")
+    serve = serving_leg(args.requests, args.rate, args.slots, args.solve_iters) \
+        if args.requests > 0 else None
+    progress("serving leg done")
+    t_m, t_f = mixed["seconds"], fp64["seconds"]
+    out = {"metric": METRIC, "value": t_m, "unit": "s to 1e-5 relative objective gap",
+           "n_gpus": 1, "steps": mixed["iterations"], "warmup": args.warmup,
+           "ms_per_step": mixed["ms_per_iteration"], "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 storage / f64 accumulate" if args.precision == "mixed" else "f64",
+           "data": "synthetic (seeded graph, seeded Omega and q0)",
+           "config": {"workload": WORKLOADS["torus"], "n": n, "m": m, "precision": args.precision,
+                      "exec_mode": args.exec_mode, "step": "one USBS iteration; value = wall time "
+                      "of the usbs() call until obj_gap <= 1e-5 (solver/usbs.py:646-647)",
+                      **dict(SOLVER, rho=GAP_RHO), "target_gap": GAP_TARGET},
+           "time_to_gap": {"mixed": mixed, "plain_fp64": fp64,
+                           "speedup": (round(t_f / t_m, 3) if t_m and t_f else None)},
+           "batched": batch, "serving": serve, "throughput": thr}
+    out["roofline"] = roofline(stats) if stats else None
+    if args.cpu_iters > 0:
+        cb = cpu_baseline(args.cpu_iters, "torus", args.cpu_iters_single)
+        progress("cpu baseline done")
+        # extrapolated: the mixed run's iteration count at the CPU's iteration rate
+        cb["time_to_gap_estimate_s"] = round(mixed["iterations"] / cb["value"], 1)
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,11 +605,20 @@ def main():
                          "randomly numbered RGG only)")
     ap.add_argument("--kron-vectors", type=int, default=8192,
                     help="qap-kron: vectors per launch")
-    ap.add_argument("--requests", type=int, default=24, help="serving: number of requests")
-    ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
+    ap.add_argument("--requests", type=int, default=12,
+                    help="serving: number of requests (0 skips the serving leg of the default run)")
+    ap.add_argument("--rate", type=float, default=4.0, help="serving: arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent solver slots")
-    ap.add_argument("--solve-iters", type=int, default=20, help="serving: USBS iterations per solve")
+    ap.add_argument("--solve-iters", type=int, default=10, help="serving: USBS iterations per solve")
     ap.add_argument("--cpu-iters", type=int, default=15, help="oracle iterations (0 = skip)")
+    ap.add_argument("--cpu-iters-single", type=int, default=4,
+                    help="oracle iterations on one core (torus default run)")
+    ap.add_argument("--gap-max-iters", type=int, default=12000,
+                    help="time-to-gap legs: iteration cap")
+    ap.add_argument("--gap-max-time", type=float, default=60.0,
+                    help="time-to-gap legs: seconds cap per solve")
+    ap.add_argument("--gap-batch", type=int, default=4,
+                    help="torus default run: problems in the batched time-to-gap leg (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 20 device-to-device copies of MIB MiB (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -430,6 +648,11 @@ def main():
         return
     if args.workload == "qap-kron":
         kron_bench(args)
+        return
+    if args.workload == "torus":
+        # HIP's default of 4 hardware queues: with 8, the solves of the batched
+        # leg were starved unevenly (one solve 2.4x slower than with 4)
+        gap_bench(args)
         return
 
     from usbs_amd import _lib

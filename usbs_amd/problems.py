@@ -80,14 +80,6 @@ def rgg_radius(n, mean_degree):
     return 0.5 * (lo + hi)
 
 
-def random_geometric(n, radius, seed):
-    """n points uniform in [0,1]^2, unit edges between points closer than radius."""
-    from scipy.spatial import cKDTree
-    pts = _rng(seed).random((n, 2))
-    pairs = cKDTree(pts).query_pairs(radius, output_type="ndarray")
-    return _sym_from_edges(n, pairs[:, 0], pairs[:, 1], np.ones(pairs.shape[0]))
-
-
 def qap_instance(l, seed):
     """QAP data: flow F symmetric integers 0-100 with ~70% zeros; distance D
     from l random points on the 100x100 integer grid (Euclidean, rounded,

@@ -55,7 +55,13 @@ _lib.EXTRA_SIGNATURES["usbs_solve"] = [
 
 
 def _finite(x, default):
-    return float(x) if np.isfinite(x) else default
+    """x as a finite double for the C ABI: +-inf become +-1e300 (so an eps of
+    +inf places no requirement and -inf can never be met, as in the
+    reference's comparisons); NaN becomes `default`."""
+    x = float(x)
+    if x != x:
+        return default
+    return max(-1e300, min(1e300, x))
 
 
 def cut_scale_of(sdp_state, callback_nonstatic_args):
