@@ -250,6 +250,23 @@ int usbs_sketch_update(usbs_ctx* ctx, int64_t n, int32_t k, int32_t kc, int32_t 
                        int32_t precision, double* P_out, double* Pbar_out);
 
 /*
+ * The sketch update of usbs_sketch_update on caller-owned DEVICE arrays,
+ * allocating nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid allocation).  F (n x k), Omega, Pbar, P_out, Pbar_out
+ * (n x r; one column when r == 0, the diag-only form) are column-major with
+ * leading dimension ld >= n, doubles for USBS_PREC_FP64 and floats for
+ * USBS_PREC_MIXED; Pbar is left unchanged.  Intermediates live in a fixed
+ * 2 MiB device array per GPU: sizes whose row-tile partials of F^T Omega do
+ * not fit (ntiles * k * r + 1024 doubles) are refused with USBS_ERR_INVALID.
+ * Runs on the context's stream and returns when done; calls that overlap on
+ * one device share that array and race.
+ */
+int usbs_sketch_update_noalloc(usbs_ctx* ctx, int64_t n, int32_t k, int32_t kc, int32_t r,
+                               int64_t ld, const void* F, const void* Omega, const void* Pbar,
+                               double eta, int32_t precision, void* P_out, void* Pbar_out);
+
+/*
  * Primal objective, constraint map, dual candidate and infeasibility sums of
  * one USBS iteration (solver/usbs.py:571-585, 637-652) for the uploaded
  * problem: z = eta zbar + A(F F^T), zbar' = eta zbar + A(F_c F_c^T),

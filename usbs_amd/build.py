@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libusbs_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["engine.hip", "lanczos.hip", "ops.hip", "dense_kernels.hip", "solve.hip", "cgal.hip",
-           "api_ops.hip", "qap_kron.hip", "boundary_ops.hip"]
+           "api_ops.hip", "qap_kron.hip", "boundary_ops.hip", "noalloc.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

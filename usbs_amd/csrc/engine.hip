@@ -442,6 +442,7 @@ int usbs_create(usbs_ctx** out, int device) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return USBS_ERR_HIP;
   if (device < 0 || device >= ndev) return USBS_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return USBS_ERR_HIP;
+  if (noalloc_init_device(device) != USBS_OK) return USBS_ERR_HIP;
   usbs_ctx* h = new usbs_ctx();
   h->c.device = device;
   if (hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking) != hipSuccess) {

@@ -115,6 +115,10 @@ template <typename T>
 int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int nd,
                  int max_restarts, double tol, const T* opval, LzInst<T>& L);
 
+// noalloc.hip: look up this device's copy of the fixed scratch array of
+// usbs_sketch_update_noalloc (once per device, with the device current).
+int noalloc_init_device(int device);
+
 // kernels in ops.hip
 int launch_assemble(const Problem& P, double coefC, const double* z1, double a1, const double* z2,
                     double a2, const double* z3, double a3, double* out64, float* out32,
