@@ -154,7 +154,9 @@ typedef struct usbs_params {
   int32_t precision;   /* USBS_PREC_*                                               */
   int32_t exec_mode;   /* USBS_EXEC_* for the Lanczos calls                          */
   int32_t warm;        /* 1: start from y, z and P the context kept from its last    */
-                       /*    usbs_solve (device-resident; y0, z0, P0 are ignored)     */
+                       /*    usbs_solve (device-resident; y0, z0, P0 are ignored);    */
+                       /*    z and P are taken from that solve's SCALE_X to this one's */
+                       /*    (SCALE_A must be the same), y carries over               */
   int32_t reserved;
   double trace_ub, trace_factor, rho, beta;
   double max_time, obj_gap_eps, infeas_gap_eps, max_infeas_eps;

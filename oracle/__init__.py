@@ -15,7 +15,7 @@ Module map (reference file:line each function follows is in its docstring):
 * ``usbs``       -- solver/usbs.py:436-789 (the USBS outer loop)
 * ``cgal``       -- solver/cgal.py:21-243 (the CGAL loop)
 * ``common``     -- utils/common.py:11-121 (SDPState, scaling, Nystrom)
-* ``maxcut``     -- utils/maxcut_helpers.py:26-92, 292-302
+* ``maxcut``     -- utils/maxcut_helpers.py:26-92, 95-289 (warm starts), 292-302
 
 Pinning status of each function (what checks it besides its own formula) is
 listed in ``PINS`` below; tests/test_oracle_*.py implement those checks.
@@ -41,4 +41,9 @@ PINS = {
     "cgal.cgal": "SDP closed forms: bipartite |E| (exact), K_n n^2/4 (1e-3, first-order method)",
     "common.reconstruct_from_sketch": "exact recovery of a rank<=r PSD matrix (Tropp et al. 2017)",
     "maxcut.compute_max_cut": "brute force over all 2^n cuts on n<=14; integer edge-sum identity",
+    "maxcut.implicit_warm_start_state": "unscaling round trip on the same graph; diagonal "
+                                        "correction and scales on a grown graph",
+    "maxcut.explicit_warm_start_state": "sketch of an exactly rank-r PSD X: trace kept, "
+                                        "z = diag(X + delta E E^T) to the Nystrom shift",
+    "maxcut.dual_only_warm_start_state": "y padded, primal quantities zero",
 }

@@ -69,3 +69,111 @@ def cut_values_bruteforce(W):
         iu, ju = np.triu_indices(n, 1)
         out[mask] = float(np.sum(W[iu, ju] * (s[iu] != s[ju])))
     return out
+
+
+def _laplacian_cost_and_data(W):
+    C = laplacian_cost(W)
+    n = C.shape[0]
+    A_data, A_indices, b, mask = get_all_problem_data(n)
+    return C, n, A_data, A_indices, b, mask
+
+
+def _diag(C):
+    return np.asarray(C.diagonal(), dtype=np.float64)
+
+
+def _warm_scales(C, n):
+    """SCALE_X = 1/n, SCALE_C = 1/|C.data| (utils/maxcut_helpers.py:134-136)."""
+    return 1.0 / float(n), 1.0 / np.linalg.norm(C.data)
+
+
+def implicit_warm_start_state(old, W, sketch_dim, Omega_pad):
+    """utils/maxcut_helpers.py:95-161.  The new graph W has n >= n_old
+    vertices; P and Omega are padded (new rows of Omega: Omega_pad, drawn by
+    the reference from jax.random with PRNGKey(n)); y, z padded with zeros;
+    the primal objective corrected for the change of C's diagonal; the state
+    rescaled by SCALE_X = 1/n, SCALE_C = 1/|C|_F."""
+    from .common import unscale_sdp_state
+    old = unscale_sdp_state(old)
+    C, n, A_data, A_indices, b, mask = _laplacian_cost_and_data(W)
+    n_old = old.C.shape[0]
+    X, P, Omega = old.X, old.P, old.Omega
+    if old.X is not None:
+        X = np.zeros((n, n))
+        X[:n_old, :n_old] = old.X
+    if old.P is not None:
+        P = np.zeros((n, sketch_dim))
+        P[:n_old] = old.P
+        Omega = np.concatenate([old.Omega, Omega_pad], axis=0)
+    y = np.zeros(n)
+    y[:old.b.shape[0]] = old.y
+    z = np.zeros(n)
+    z[:old.b.shape[0]] = old.z
+    primal_obj = old.primal_obj - _diag(old.C) @ old.z + _diag(C) @ z
+    SX, SC = _warm_scales(C, n)
+    s = SDPState(C=C, A_indices=A_indices, A_data=A_data, b=b, b_ineq_mask=mask, X=X, P=P,
+                 Omega=Omega, y=y, z=z, tr_X=old.tr_X, primal_obj=primal_obj, SCALE_C=SC,
+                 SCALE_X=SX, SCALE_A=np.ones(n))
+    return scale_sdp_state(s)
+
+
+def explicit_warm_start_state(old, W, sketch_dim, Omega_new):
+    """utils/maxcut_helpers.py:164-229.  The old sketch is reconstructed
+    (Nystrom, utils/common.py:99-116), its eigenvalues shifted so that their
+    sum is the old trace, the factor padded to the new size, and P, z, tr_X
+    and the primal objective recomputed from it with a fresh test matrix
+    (Omega_new, PRNGKey(n) in the reference)."""
+    from .common import unscale_sdp_state
+    from .operators import apply_A_operator_batched
+    old = unscale_sdp_state(old)
+    C, n, A_data, A_indices, b, mask = _laplacian_cost_and_data(W)
+    n_old = old.C.shape[0]
+    X, P, Omega = old.X, old.P, old.Omega
+    if old.X is not None:
+        X = np.zeros((n, n))
+        X[:n_old, :n_old] = old.X
+        z = np.bincount(A_indices[:, 0], weights=A_data * X[A_indices[:, 1], A_indices[:, 2]],
+                        minlength=n)
+        tr_X = np.trace(X)
+        primal_obj = float(np.sum(C.multiply(X)))
+    if old.P is not None:
+        Omega = Omega_new
+        E, Lam = reconstruct_from_sketch(old.Omega, old.P)
+        tr_offset = (old.tr_X - np.sum(Lam)) / Lam.shape[0]
+        Lam = Lam + tr_offset
+        Ep = np.zeros((n, E.shape[1]))
+        Ep[:n_old] = E
+        S = Ep * np.sqrt(Lam)[None, :]
+        P = S @ (S.T @ Omega)
+        z = apply_A_operator_batched(n, A_data, A_indices, S)
+        tr_X = np.sum(Lam)
+        primal_obj = float(np.trace(S.T @ (C @ S)))
+    y = np.zeros(n)
+    y[:old.b.shape[0]] = old.y
+    SX, SC = _warm_scales(C, n)
+    s = SDPState(C=C, A_indices=A_indices, A_data=A_data, b=b, b_ineq_mask=mask, X=X, P=P,
+                 Omega=Omega, y=y, z=z, tr_X=tr_X, primal_obj=primal_obj, SCALE_C=SC, SCALE_X=SX,
+                 SCALE_A=np.ones(n))
+    return scale_sdp_state(s)
+
+
+def dual_only_warm_start_state(old, W, sketch_dim, Omega_new):
+    """utils/maxcut_helpers.py:232-289: only y carries over (padded); the
+    primal starts at zero with a fresh test matrix (PRNGKey(0) in the
+    reference), rescaled as the other warm starts."""
+    from .common import unscale_sdp_state
+    old = unscale_sdp_state(old)
+    C, n, A_data, A_indices, b, mask = _laplacian_cost_and_data(W)
+    if sketch_dim == -1:
+        X, Omega, P = np.zeros((n, n)), None, None
+    elif sketch_dim > 0:
+        X, Omega, P = None, Omega_new, np.zeros_like(Omega_new)
+    else:
+        raise ValueError("Invalid value for sketch_dim")
+    y = np.zeros(n)
+    y[:old.b.shape[0]] = old.y
+    SX, SC = _warm_scales(C, n)
+    s = SDPState(C=C, A_indices=A_indices, A_data=A_data, b=b, b_ineq_mask=mask, X=X, P=P,
+                 Omega=Omega, y=y, z=np.zeros(n), tr_X=0.0, primal_obj=0.0, SCALE_C=SC,
+                 SCALE_X=SX, SCALE_A=np.ones(n))
+    return scale_sdp_state(s)

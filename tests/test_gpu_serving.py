@@ -34,10 +34,12 @@ def test_serving_mix_equals_alone():
 
 
 def test_warm_start_continues_the_parent():
-    """A follow-up on the same graph warm-started from the device-resident
-    state begins where its parent ended: its first record (t = 0) carries the
-    parent's final primal objective and, rounded from the same sketch, the
-    parent's final cut value."""
+    """A follow-up on the same graph warm-started (the reference's implicit
+    transform, which rescales to SCALE_X = 1/n, SCALE_C = 1/|C|_F) from the
+    device-resident state begins where its parent ended: its first record
+    (t = 0) carries the parent's final primal objective (to the rounding of
+    the rescaling) and, rounded from the same sketch, the parent's final cut
+    value."""
     from usbs_amd import maxcut_helpers
     from usbs_amd.usbs import usbs
     reqs = _mix()
@@ -45,11 +47,12 @@ def test_warm_start_continues_the_parent():
     parent = srv.solve(reqs[1], 0, None, 0.0)
     W = reqs[1].W
     n = W.shape[0]
-    st, _ = srv._state(serving.Request(99, W, 0.0, 1), parent)
+    st = srv._state(serving.Request(99, W, 0.0, 1), parent)
+    assert st.SCALE_X == 1.0 / n
     log = []
-    usbs(st, n, n, 2.0 * n, 2.0, 0.01, 0.25, 4, 1, 2, float("inf"), -float("inf"), -float("inf"),
-         -float("inf"), 32, 10, 1e-7, 100, 1.0, maxcut_helpers.compute_max_cut, None, st.C,
-         precision="mixed", exec_mode="perop", q0=np.ones(n) / np.sqrt(n), log=log,
-         context=srv.ctxs[0], warm_start=True)
-    assert log[0]["primal_obj"] == parent.primal_obj
+    usbs(st, n, n, 2.0 * n * st.SCALE_X, 2.0, 0.01, 0.25, 4, 1, 2, float("inf"), -float("inf"),
+         -float("inf"), -float("inf"), 32, 10, 1e-7, 100, 1.0, maxcut_helpers.compute_max_cut, None,
+         st.C / st.SCALE_C, precision="mixed", exec_mode="perop", q0=np.ones(n) / np.sqrt(n),
+         log=log, context=srv.ctxs[0], warm_start=True)
+    assert abs(log[0]["primal_obj"] - parent.primal_obj) <= 1e-12 * abs(parent.primal_obj)
     assert log[0]["callback_val"] == parent.cut

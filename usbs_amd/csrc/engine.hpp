@@ -96,6 +96,7 @@ struct Ctx {
   double *keep_y = nullptr, *keep_z = nullptr, *keep_P = nullptr;
   int keep_n = 0, keep_m = 0, keep_r = -1;
   long long keep_ld = 0;
+  double keep_sx = 1.0;           // SCALE_X of the solve that left them
   usbs_stats stats{};
 };
 

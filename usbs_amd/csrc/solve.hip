@@ -482,10 +482,11 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
   if (warm) {
     if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld)
       return set_err(c, USBS_ERR_STATE, "warm start needs a previous solve of the same size on this context");
-    if (hipMemcpyAsync(S.y, c->keep_y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(S.z, c->keep_z, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(S.zbar, c->keep_z, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    // y carries over; z and P are taken from the kept solve's SCALE_X to this one's
+    if (hipMemcpyAsync(S.y, c->keep_y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "copy of the kept start point");
+    k_rescale<double><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(m, c->keep_z, S.z, c->keep_sx, p.SCALE_X);
+    k_rescale<double><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(m, c->keep_z, S.zbar, c->keep_sx, p.SCALE_X);
   } else if (hipMemcpy(S.y, y0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
              hipMemcpy(S.z, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
              hipMemcpy(S.zbar, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess) {
@@ -499,8 +500,8 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
   S.fill_T(S.PO, ld * 2 * (r > 0 ? r : 1));
   if (warm) {
     const long long cnt = ld * rk;
-    k_cast<double, T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.P_());
-    k_cast<double, T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.Pbar);
+    k_rescale<T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.P_(), c->keep_sx, p.SCALE_X);
+    k_rescale<T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.Pbar, c->keep_sx, p.SCALE_X);
   }
   if (r > 0) {
     double* d_om = up(Omega, (size_t)n * r);
@@ -637,6 +638,7 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
     c->keep_r = rk;
     c->keep_ld = ld;
   }
+  c->keep_sx = p.SCALE_X;
   {
     const long long cnt = ld * rk;
     k_cast<T, double><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, S.P_(), c->keep_P);

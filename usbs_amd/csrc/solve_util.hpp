@@ -29,6 +29,17 @@ __global__ void k_fill(T* p, long long count, T v) {
   if (i < count) p[i] = v;
 }
 
+// dst = (T)((src / sx_old) * sx_new): a state kept in the units of SCALE_X =
+// sx_old taken to sx_new, with the operations of unscaling then scaling it
+// (utils/common.py:29-96 with SCALE_A = 1), so that the result equals that of
+// the same transform on the host bit for bit.
+template <typename TD>
+__global__ void k_rescale(long long count, const double* __restrict__ src, TD* __restrict__ dst,
+                          double sx_old, double sx_new) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = (TD)((src[i] / sx_old) * sx_new);
+}
+
 template <typename TS, typename TD>
 __global__ void k_cast(long long count, const TS* __restrict__ src, TD* __restrict__ dst) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -13,10 +13,14 @@ parent's final y, z and sketch P:
 * otherwise it starts from the host copy of the same arrays (the same values,
   so the result is identical).
 
-The warm start follows utils/maxcut_helpers.py:95-161 (implicit warm start):
-y, z and P carry over and the primal objective is corrected by the change of
-C's diagonal, p' = p - <diag C_old, z> + <diag C_new, z>; both problems keep
-SCALE_C = SCALE_X = 1, as the cold start of utils/maxcut_helpers.py:36-92.
+The warm start is the reference's implicit warm start
+(utils/maxcut_helpers.py:95-161, ``maxcut_helpers.get_implicit_warm_start_state``):
+y, z and P carry over, the primal objective is corrected by the change of C's
+diagonal, p' = p - <diag C_old, z> + <diag C_new, z>, and the follow-up is
+solved in the scaling SCALE_X = 1/n, SCALE_C = 1/|C|_F (a cold request keeps
+SCALE_C = SCALE_X = 1, as utils/maxcut_helpers.py:36-92).  On the device
+path the library rescales the kept z and P from the parent's SCALE_X with the
+same operations, so both paths give the same start bit for bit.
 
 Every solve runs the same fixed work (``max_iters`` USBS iterations), so a
 result does not depend on what else was on the device: the library's
@@ -58,7 +62,7 @@ class Result:
     z: np.ndarray = field(repr=False, default=None)
     P: np.ndarray = field(repr=False, default=None)
     tr_X: float = 0.0
-    C_diag: np.ndarray = field(repr=False, default=None)
+    state: object = field(repr=False, default=None)   # the solve's final SDPState
 
     @property
     def latency(self):
@@ -113,39 +117,37 @@ class SolveServer:
     def _state(self, req, parent_res):
         W = req.W
         n = W.shape[0]
-        Om = problems.gaussian_matrix(n, self.cfg["r"], n)
-        st = maxcut_helpers.initialize_state(W, self.cfg["r"], Omega=Om)
-        C_diag = np.asarray(maxcut_helpers.laplacian_cost(W).diagonal(), dtype=float)
         if parent_res is None:
-            return st, C_diag
-        p = parent_res.primal_obj - float(parent_res.C_diag @ parent_res.z) + float(C_diag @ parent_res.z)
-        st = st._replace(y=parent_res.y, z=parent_res.z, P=parent_res.P, tr_X=parent_res.tr_X,
-                         primal_obj=p)
-        return st, C_diag
+            Om = problems.gaussian_matrix(n, self.cfg["r"], n)
+            return maxcut_helpers.initialize_state(W, self.cfg["r"], Omega=Om)
+        # a follow-up on a graph of the parent's size: no new rows of Omega
+        return maxcut_helpers.get_implicit_warm_start_state(
+            parent_res.state, W, self.cfg["r"], Omega_pad=np.zeros((0, self.cfg["r"])))
 
     def solve(self, req, slot, parent_res, t0):
         """Run one request on ``slot`` (blocking); returns its Result."""
         ctx = self.ctxs[slot]
         start = time.perf_counter() - t0
-        st, C_diag = self._state(req, parent_res)
+        st = self._state(req, parent_res)
         n = req.W.shape[0]
         warm = "cold"
         if parent_res is not None:
             warm = "device" if self.held[slot] == parent_res.rid else "host"
         c = self.cfg
         log = []
-        out = usbs(st, n, n, c["tf"] * n, c["tf"], c["rho"], c["beta"], c["k_curr"], c["k_past"],
-                   c["iters"], float("inf"), -float("inf"), -float("inf"), -float("inf"), c["mL"],
-                   c["restarts"], c["eps"], c["sub_iters"], 1.0, maxcut_helpers.compute_max_cut,
-                   None, st.C, precision=self.precision, exec_mode=self.exec_mode,
+        out = usbs(st, n, n, c["tf"] * n * st.SCALE_X, c["tf"], c["rho"], c["beta"], c["k_curr"],
+                   c["k_past"], c["iters"], float("inf"), -float("inf"), -float("inf"),
+                   -float("inf"), c["mL"], c["restarts"], c["eps"], c["sub_iters"], 1.0,
+                   maxcut_helpers.compute_max_cut, None, st.C / st.SCALE_C,
+                   precision=self.precision, exec_mode=self.exec_mode,
                    q0=problems.unit_start_vector(n, c["q0_seed"]), log=log, context=ctx,
                    warm_start=(warm == "device"))
         self.held[slot] = req.rid
         end = time.perf_counter() - t0
         last = log[-1]
-        return Result(req.rid, n, slot, warm, req.arrival, start, end, out.primal_obj,
+        return Result(req.rid, n, slot, warm, req.arrival, start, end, last["primal_obj"],
                       last["obj_gap"], last["infeas_gap"], last["callback_val"], y=out.y, z=out.z,
-                      P=out.P, tr_X=out.tr_X, C_diag=C_diag)
+                      P=out.P, tr_X=out.tr_X, state=out)
 
     # -- the request loop -------------------------------------------------
     def run(self, requests, realtime=True):
