@@ -150,7 +150,8 @@ typedef struct usbs_params {
   int32_t lanczos_inner_iterations, lanczos_max_restarts;
   int32_t subprob_max_iters;
   int32_t sketch_dim;  /* r > 0: Nystrom sketch P (n x r); 0: diag-only P (n)       */
-  int32_t rounding;    /* 1: max-cut rounding at every condition check (needs r > 0) */
+  int32_t rounding;    /* 1: max-cut rounding at every condition check (needs r > 0); */
+                       /* 2: QAP rounding (needs usbs_set_qap_rounding, n = l^2 + 1) */
   int32_t precision;   /* USBS_PREC_*                                               */
   int32_t exec_mode;   /* USBS_EXEC_* for the Lanczos calls                          */
   int32_t warm;        /* 1: start from y, z and P the context kept from its last    */
@@ -278,6 +279,35 @@ int usbs_sketch_update_noalloc(usbs_ctx* ctx, int64_t n, int32_t k, int32_t kc, 
 int usbs_objective_infeas(usbs_ctx* ctx, int32_t k, int32_t kc, const double* F, double eta,
                           const double* zbar, const double* y, const double* upsilon, double rho,
                           double* z_out, double* zbar_next_out, double* ycand_out, double* out6);
+
+/*
+ * Minimum-cost assignment of an n x n cost matrix (row-major) by the
+ * Hungarian algorithm with the reference's step order, tie-breaking and
+ * n^2-step bound (munkres, utils/munkres.py:14-120): mask (n x n row-major)
+ * receives the reference's result matrix, 1 at the assigned entries (a
+ * mask left incomplete by the step bound is returned as the reference
+ * returns it).  Host-only: needs no context and no GPU.
+ */
+int usbs_munkres(int32_t n, const double* cost, int32_t* mask);
+
+/*
+ * QAP rounding (qap_round, utils/qap_helpers.py:625-653) of a sketch of the
+ * lifted QAP's X (N = l^2 + 1 rows, r columns, row-major): the Nystrom factor
+ * E is formed on the device; for each column i < l (clamped to r - 1) and its
+ * negative, the assignment minimising max(c) - c for c = E[1:, i] as an l x l
+ * matrix is priced by trace(W Pi D Pi^T).  values (may be NULL): 2 l
+ * candidate objectives (positive columns, then negative); perm (may be NULL):
+ * the best assignment (-1 for a row the assignment left open); best: the
+ * smaller minimum.
+ */
+int usbs_qap_round(usbs_ctx* ctx, int32_t l, int32_t r, const double* P, const double* Omega,
+                   const double* W, const double* D, double* values, int32_t* perm,
+                   double* best);
+
+/* W (flow) and D (distance), l x l row-major, for usbs_solve's QAP rounding
+ * (usbs_params.rounding = 2): the callback_nonstatic_args {"W", "D"} of the
+ * reference's QAP scripts (scripts/warm_start_qap_usbs.py). */
+int usbs_set_qap_rounding(usbs_ctx* ctx, int32_t l, const double* W, const double* D);
 
 /*
  * Live kernel timing (measurement support; no counterpart in the reference).

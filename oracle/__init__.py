@@ -14,6 +14,7 @@ Module map (reference file:line each function follows is in its docstring):
 * ``ipm``        -- solver/usbs.py:25-433 (prox-bundle IPM subproblems)
 * ``usbs``       -- solver/usbs.py:436-789 (the USBS outer loop)
 * ``cgal``       -- solver/cgal.py:21-243 (the CGAL loop)
+* ``qap``        -- utils/qap_helpers.py:157-381, 625-653; utils/munkres.py:14-120
 * ``common``     -- utils/common.py:11-121 (SDPState, scaling, Nystrom)
 * ``maxcut``     -- utils/maxcut_helpers.py:26-92, 95-289 (warm starts), 292-302
 
@@ -46,4 +47,6 @@ PINS = {
     "maxcut.explicit_warm_start_state": "sketch of an exactly rank-r PSD X: trace kept, "
                                         "z = diag(X + delta E E^T) to the Nystrom shift",
     "maxcut.dual_only_warm_start_state": "y padded, primal quantities zero",
+    "qap.munkres": "brute force over all n! assignments (the reference's 7 matrices, random n<=7)",
+    "qap.qap_round": "sketch of the lifted brute-force optimal assignment rounds to the optimal value",
 }

@@ -128,3 +128,83 @@ def lifted_assignment(perm):
     for a, i in enumerate(perm):
         x[1 + l * a + i] = 1.0
     return x
+
+
+def munkres(n, cost):
+    """Minimum-cost assignment mask (n x n, 1 at the chosen entries) by the
+    steps of utils/munkres.py:14-120 in the same order: row reduction;
+    zeros starred in row-major order; then, until n stars, the first
+    uncovered zero (row-major) is primed and either covers its row (a star
+    in the row: the star's column is uncovered) or starts an augmenting
+    path; with no uncovered zero, the smallest uncovered value (covered
+    entries read as the matrix maximum) is subtracted from the uncovered
+    columns and added to the covered rows."""
+    M = np.array(cost, dtype=np.float64)
+    M = M - M.min(axis=1).reshape(-1, 1)
+    mask = np.zeros((n, n), dtype=np.int64)
+    rc = np.zeros(n, dtype=bool)
+    cc = np.zeros(n, dtype=bool)
+    for idx in range(n * n):
+        r, c = divmod(idx, n)
+        if M[r, c] == 0 and not rc[r] and not cc[c]:
+            mask[r, c] = 1
+            rc[r] = True
+            cc[c] = True
+    rc[:] = False
+    cc = (mask == 1).any(axis=0)
+    for _ in range(n * n):
+        if (mask == 1).sum() >= n:
+            break
+        cover = rc[:, None] | cc[None, :]
+        zeros = (M == 0) & ~cover
+        if not zeros.any():
+            v = np.where(~cover, M, M.max()).min()
+            M = np.where(~cc[None, :], M - v, M)
+            M = np.where(rc[:, None], M + v, M)
+            continue
+        r, c = divmod(int(np.argmax(zeros)), n)
+        mask[r, c] = 2
+        if (mask[r] == 1).any():
+            rc[r] = True
+            cc[int(np.argmax(mask[r] == 1))] = False
+            continue
+        path = np.zeros((n, n), dtype=np.int64)
+        path[r, c] = 1
+        col = c
+        for _ in range(n):
+            if not (mask[:, col] == 1).any():
+                break
+            row = int(np.argmax(mask[:, col] == 1))
+            pcol = int(np.argmax(mask[row] == 2))
+            path[row, col] = 1
+            path[row, pcol] = 1
+            col = pcol
+        mask = mask - path
+        mask[mask == 2] = 0
+        rc[:] = False
+        cc = (mask == 1).any(axis=0)
+    return mask
+
+
+def qap_round(P, Omega, l, D, W):
+    """utils/qap_helpers.py:625-653: for each of the first l Nystrom columns
+    (indices past the last column read the last, as JAX's clamped gather
+    does) and its negative, the assignment minimising max(c) - c of the
+    column as an l x l matrix, priced by trace(W Pi D Pi^T); the smaller
+    minimum (the positive one unless the negative is strictly smaller).
+    Returns (best, values of the 2 l candidates)."""
+    from .common import reconstruct_from_sketch
+    E, _ = reconstruct_from_sketch(Omega, P)
+    r = E.shape[1]
+    D = np.asarray(D, dtype=np.float64)
+    W = np.asarray(W, dtype=np.float64)
+    vals = []
+    for sign in (1.0, -1.0):
+        for i in range(l):
+            cm = sign * E[1:, min(i, r - 1)].reshape(l, l)
+            cm = cm.max() - cm
+            perm = munkres(l, cm)
+            vals.append(float(np.trace(W @ perm @ D @ perm.T)))
+    vals = np.array(vals)
+    mp, mn = vals[:l].min(), vals[l:].min()
+    return (mp if mp < mn else mn), vals

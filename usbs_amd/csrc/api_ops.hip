@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/usbs_hip.h"
+#include "munkres.hpp"
 #include "solve.hpp"
 
 namespace usbs {
@@ -41,7 +42,8 @@ __global__ void k_cm_to_rm(long long n, int r, const T* src, long long ld, doubl
 }
 
 // E = (P + rho Omega) G, row-major n x r output (utils/common.py:110-112).
-__global__ void k_nystrom_E(int n, int r, const double* P, const double* Om, long long ld,
+template <typename T>
+__global__ void k_nystrom_E(int n, int r, const T* P, const T* Om, long long ld,
                             const double* G, const double* rho_ptr, double* E) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -95,11 +97,101 @@ static int nystrom_prepare(Ctx* c, Scratch& S, int n, int r, const double* P, co
   return USBS_OK;
 }
 
+template <typename T>
+int launch_nystrom_E(int n, int r, const T* P, const T* Om, long long ld, const double* G,
+                     const double* rho, double* E, hipStream_t s) {
+  k_nystrom_E<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, r, P, Om, ld, G, rho, E);
+  return (int)hipGetLastError();
+}
+template int launch_nystrom_E<double>(int, int, const double*, const double*, long long,
+                                      const double*, const double*, double*, hipStream_t);
+template int launch_nystrom_E<float>(int, int, const float*, const float*, long long,
+                                     const double*, const double*, double*, hipStream_t);
+
+// QAP rounding on the host from the Nystrom factor E (N x r row-major,
+// N = l^2 + 1), utils/qap_helpers.py:625-653: for columns i = 0..l-1 (the
+// reference gathers them with clamped indices, so i >= r reads column r-1),
+// the assignment minimising max(c) - c for c = E[1:, i] and for c = -E[1:, i]
+// (an l x l matrix, c[a][j] = E[1 + l a + j]), each priced by
+// trace(W Pi D Pi^T); the smaller of the two minima (positive unless the
+// negative one is strictly smaller).  values: 2 l (positive candidates, then
+// negative ones); perm: the best assignment.
+int qap_round_host(int l, int r, const double* E, const double* W, const double* D,
+                   double* values, int* perm, double* best) {
+  std::vector<double> cost((size_t)l * l);
+  std::vector<int> mask, asg(l), best_pos(l), best_neg(l);
+  double min_pos = 0.0, min_neg = 0.0;
+  for (int sign = 0; sign < 2; ++sign) {
+    for (int i = 0; i < l; ++i) {
+      const int col = i < r ? i : r - 1;
+      double mx = 0.0;
+      for (int e = 0; e < l * l; ++e) {
+        const double v = E[(size_t)(1 + e) * r + col];
+        cost[e] = sign ? -v : v;
+        mx = (e == 0 || cost[e] > mx) ? cost[e] : mx;
+      }
+      for (int e = 0; e < l * l; ++e) cost[e] = mx - cost[e];
+      munkres_mask(l, cost.data(), mask);
+      mask_to_assign(l, mask, asg.data());
+      const double v = qap_objective_mask(l, W, D, mask);
+      if (values) values[sign * l + i] = v;
+      double& mn = sign ? min_neg : min_pos;
+      std::vector<int>& bp = sign ? best_neg : best_pos;
+      if (i == 0 || v < mn) { mn = v; bp = asg; }
+    }
+  }
+  const bool pos = min_pos < min_neg;
+  *best = pos ? min_pos : min_neg;
+  if (perm)
+    for (int a = 0; a < l; ++a) perm[a] = pos ? best_pos[a] : best_neg[a];
+  return USBS_OK;
+}
+
 }  // namespace usbs
 
 using namespace usbs;
 
 extern "C" {
+
+int usbs_munkres(int32_t n, const double* cost, int32_t* mask_out) {
+  if (n < 1 || n > 4096 || !cost || !mask_out) return USBS_ERR_INVALID;
+  for (long long e = 0; e < (long long)n * n; ++e)
+    if (!std::isfinite(cost[e])) return USBS_ERR_NONFINITE;
+  std::vector<int> mask;
+  munkres_mask(n, cost, mask);
+  for (size_t e = 0; e < mask.size(); ++e) mask_out[e] = mask[e];
+  return USBS_OK;
+}
+
+int usbs_qap_round(usbs_ctx* h, int32_t l, int32_t r, const double* P, const double* Omega,
+                   const double* W, const double* D, double* values, int32_t* perm,
+                   double* best) {
+  if (!h || !P || !Omega || !W || !D || !best) return USBS_ERR_INVALID;
+  Ctx* c = &h->c;
+  if (l < 1 || l > 64 || r < 1 || r > 48) return set_err(c, USBS_ERR_INVALID, "need 1 <= l <= 64, 1 <= r <= 48");
+  const int n = l * l + 1;
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(c, USBS_ERR_HIP, "hipSetDevice");
+  Scratch S(c);
+  double *PO, *G, *lam, *sc;
+  long long ld;
+  int rc = nystrom_prepare(c, S, n, r, P, Omega, &PO, ld, &G, &lam, &sc);
+  if (rc) return rc;
+  double* dE = S.alloc<double>((size_t)n * r);
+  if (!dE) return USBS_ERR_OOM;
+  hipStream_t s = c->stream;
+  if (launch_nystrom_E(n, r, PO, PO + r * ld, ld, G, sc + SC_NYS_RHO, dE, s))
+    return set_err(c, USBS_ERR_HIP, "Nystrom factor");
+  std::vector<double> E((size_t)n * r);
+  if (hipMemcpyAsync(E.data(), dE, sizeof(double) * E.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return set_err(c, USBS_ERR_HIP, "reading the Nystrom factor");
+  std::vector<int> pm(l);
+  rc = qap_round_host(l, r, E.data(), W, D, values, pm.data(), best);
+  if (rc) return set_err(c, rc, "QAP rounding failed");
+  if (perm)
+    for (int a = 0; a < l; ++a) perm[a] = pm[a];
+  return USBS_OK;
+}
 
 int usbs_reconstruct_from_sketch(usbs_ctx* h, int64_t n, int32_t r, const double* Omega,
                                  const double* P, double* E, double* Lambda) {
@@ -115,7 +207,7 @@ int usbs_reconstruct_from_sketch(usbs_ctx* h, int64_t n, int32_t r, const double
   double* dE = S.alloc<double>((size_t)n * r);
   if (!dE) return USBS_ERR_OOM;
   hipStream_t s = c->stream;
-  k_nystrom_E<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((int)n, r, PO, PO + r * ld, ld, G,
+  k_nystrom_E<double><<<(unsigned)((n + 255) / 256), 256, 0, s>>>((int)n, r, PO, PO + r * ld, ld, G,
                                                           sc + SC_NYS_RHO, dE);
   if (hipMemcpyAsync(E, dE, sizeof(double) * n * r, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipMemcpyAsync(Lambda, lam, sizeof(double) * r, hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -97,6 +97,9 @@ struct Ctx {
   int keep_n = 0, keep_m = 0, keep_r = -1;
   long long keep_ld = 0;
   double keep_sx = 1.0;           // SCALE_X of the solve that left them
+  // QAP rounding data (usbs_set_qap_rounding): flow W and distance D, l x l
+  int qap_l = 0;
+  std::vector<double> qap_W, qap_D;
   usbs_stats stats{};
 };
 

@@ -62,6 +62,9 @@ struct Solver {
          *Gn = nullptr, *lam = nullptr, *cut = nullptr, *sc = nullptr, *q0 = nullptr;
   signed char* sg = nullptr;
   CutWeights cutw{};       // weights of the rounding's cut values (C / cut_scale)
+  double* dE = nullptr;    // QAP rounding: the Nystrom factor (n x r row-major)
+  std::vector<double> hE;  // ... its host copy
+  double qap_cb = NAN;     // ... the last rounded QAP objective
   LzInst<T> lz{}, qr{};
   LzInst<T>* d_lz_k = nullptr;   // device copies for the persistent form (nd = k, nd = kc)
   LzInst<T>* d_lz_kc = nullptr;
@@ -166,9 +169,16 @@ struct Solver {
         !(cut = alloc<double>(rr + 1)) || !(sc = alloc<double>(SC_COUNT)) || !(q0 = alloc<double>(n)) ||
         !(sg = alloc<signed char>((size_t)n * rr)))
       return USBS_ERR_OOM;
-    if (p.rounding && r > 0) {
+    if (p.rounding == 1 && r > 0) {
       const int rcw = make_cut_weights(c, pool, p.cut_scale > 0.0 ? p.cut_scale : p.SCALE_C, &cutw);
       if (rcw) return rcw;
+    }
+    if (p.rounding == 2) {
+      if (c->qap_l < 1 || (long long)c->qap_l * c->qap_l + 1 != n || r < 1)
+        return set_err(c, USBS_ERR_STATE, "QAP rounding needs usbs_set_qap_rounding with l^2 + 1 = n "
+                                          "and a sketch");
+      if (!(dE = alloc<double>((size_t)n * r))) return USBS_ERR_OOM;
+      hE.resize((size_t)n * r);
     }
     int rc = make_lanczos<T>(c, pool, *P, mL, k, p.lanczos_max_restarts, p.subprob_eps,
                              sizeof(T) == 4 ? (const T*)op32 : (const T*)op64, lz);
@@ -337,6 +347,17 @@ struct Solver {
     if (r <= 0 || !p.rounding) return USBS_OK;
     CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, P_(), ld, r, PO, ld, 2 * r, partial, PtPO, s)));
     CK(launch_nystrom_core(n, r, PtPO, OtO, Gn, lam, sc, s));
+    if (p.rounding == 2) {
+      // QAP: the factor on the device, the assignments on the host
+      CK(launch_nystrom_E<T>(n, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, dE, s));
+      if (hipMemcpyAsync(hE.data(), dE, sizeof(double) * hE.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return set_err(c, USBS_ERR_HIP, "reading the Nystrom factor");
+      const int rc = qap_round_host(c->qap_l, r, hE.data(), c->qap_W.data(), c->qap_D.data(), nullptr,
+                                    nullptr, &qap_cb);
+      if (rc) return set_err(c, rc, "QAP rounding failed");
+      return USBS_OK;
+    }
     CK(launch_round<T>(*P, cutw, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, sg, ipart, dpart, cut, s));
     k_copy1<<<1, 64, 0, s>>>(sc + SC_CB, cut + r);
     CK((int)hipGetLastError());
@@ -449,7 +470,7 @@ struct Solver {
     rec.obj_gap = h[SC_GAP];
     rec.infeas_gap = h[SC_INFGAP];
     rec.max_infeas = h[SC_MAXINF];
-    rec.callback_val = (r > 0 && p.rounding) ? h[SC_CB] : NAN;
+    rec.callback_val = (r > 0 && p.rounding == 1) ? h[SC_CB] : (r > 0 && p.rounding == 2) ? qap_cb : NAN;
     rec.ipm_iters = h[SC_IPM_IT];
     rec.lb_iters = h[SC_LB_IT];
     rec.lanczos_restarts = lz_restarts;
@@ -662,6 +683,18 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
 
 using namespace usbs;
 
+extern "C" int usbs_set_qap_rounding(usbs_ctx* h, int32_t l, const double* W, const double* D) {
+  if (!h) return USBS_ERR_INVALID;
+  Ctx* c = &h->c;
+  if (l < 1 || l > 64 || !W || !D) return set_err(c, USBS_ERR_INVALID, "need 1 <= l <= 64 and W, D");
+  for (int e = 0; e < l * l; ++e)
+    if (!std::isfinite(W[e]) || !std::isfinite(D[e])) return set_err(c, USBS_ERR_NONFINITE, "non-finite W / D");
+  c->qap_l = l;
+  c->qap_W.assign(W, W + (size_t)l * l);
+  c->qap_D.assign(D, D + (size_t)l * l);
+  return USBS_OK;
+}
+
 extern "C" int usbs_set_timing(usbs_ctx* h, int enable) {
   if (!h) return USBS_ERR_INVALID;
   h->c.timing = enable != 0;
@@ -700,6 +733,7 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   if (p.sketch_dim < 0 || p.sketch_dim > 48) return set_err(c, USBS_ERR_INVALID, "sketch_dim in [0, 48]");
   if (p.sketch_dim > 0 && !Omega) return set_err(c, USBS_ERR_INVALID, "Omega required when sketch_dim > 0");
   if (p.rounding && p.sketch_dim == 0) return set_err(c, USBS_ERR_INVALID, "rounding needs a sketch");
+  if (p.rounding < 0 || p.rounding > 2) return set_err(c, USBS_ERR_INVALID, "rounding must be 0, 1 or 2");
   if (!(p.cond_exp_base >= 1.0 && p.cond_exp_base <= 2.0))
     return set_err(c, USBS_ERR_INVALID, "cond_exp_base must be in [1, 2]");
   if (!(p.rho > 0.0) || !(p.trace_ub > 0.0) || !(p.trace_factor > 0.0))

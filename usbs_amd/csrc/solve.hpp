@@ -107,6 +107,13 @@ inline CutWeights cut_weights_of(const Problem& P) {
 // unscaled cost of a state whose C was multiplied by SCALE_C = 1 / s), else
 // the FP64 path with inv_scale.  Returns 0 or a USBS_ERR_* code.
 int make_cut_weights(Ctx* c, std::vector<DevBuf>& pool, double s, CutWeights* out);
+// api_ops.hip: E = (P + rho Omega) G (row-major n x r); the QAP rounding on
+// the host from E (utils/qap_helpers.py:625-653).
+template <typename T>
+int launch_nystrom_E(int n, int r, const T* P, const T* Om, long long ld, const double* G,
+                     const double* rho, double* E, hipStream_t s);
+int qap_round_host(int l, int r, const double* E, const double* W, const double* D, double* values,
+                   int* perm, double* best);
 template <typename T>
 int launch_round(const Problem& P, const CutWeights& cw, int r, const T* Pm, const T* Om, long long ld, const double* G,
                  const double* rho, signed char* sg, long long* ipart, double* dpart, double* out,

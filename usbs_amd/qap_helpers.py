@@ -187,3 +187,59 @@ def kron_matvec(D, W, Y, precision="bf16", reps=1, context=None, device=0, retur
                                               _lib.dptr(Y), KRON_PRECISION[precision], int(reps),
                                               _lib.dptr(U), ctypes.byref(ms)))
     return (U, ms.value) if return_time else U
+
+
+_lib.EXTRA_SIGNATURES.update({
+    "usbs_munkres": [ctypes.c_int32, _lib._dp, _lib._ip],
+    "usbs_qap_round": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _lib._dp, _lib._dp, _lib._dp,
+                       _lib._dp, _lib._dp, _lib._ip, _lib._dp],
+    "usbs_set_qap_rounding": [ctypes.c_void_p, ctypes.c_int32, _lib._dp, _lib._dp],
+})
+
+
+def munkres(n, cost_mx):
+    """Minimum-cost assignment of an n x n cost matrix as the reference's
+    mask (1 at the assigned entries; utils/munkres.py:14-120), by the
+    library's host-side Hungarian algorithm with the reference's step order,
+    tie-breaking and n^2-step bound."""
+    C = np.ascontiguousarray(cost_mx, dtype=np.float64)
+    if C.shape != (n, n):
+        raise ValueError("cost_mx must be n x n")
+    out = np.zeros((n, n), dtype=np.int32)
+    rc = _lib.lib().usbs_munkres(int(n), _lib.dptr(C), _lib.iptr(out))
+    if rc != 0:
+        raise _lib.USBSError(rc, "usbs_munkres failed")
+    return out.astype(np.float64)
+
+
+def _W_D(callback_nonstatic_args):
+    W = np.ascontiguousarray(callback_nonstatic_args["W"], dtype=np.float64)
+    D = np.ascontiguousarray(callback_nonstatic_args["D"], dtype=np.float64)
+    l = W.shape[0]
+    if W.shape != (l, l) or D.shape != (l, l):
+        raise ValueError("W and D must be l x l")
+    return l, W, D
+
+
+def qap_round(P, Omega, callback_static_args, callback_nonstatic_args, context=None, device=0):
+    """QAP rounding callback (utils/qap_helpers.py:625-653): the smallest
+    QAP objective trace(W Pi D Pi^T) over the assignments rounded from the
+    Nystrom columns of the sketch and their negatives.  The reference passes
+    l pickled in callback_static_args; here l is W's order (nothing is
+    unpickled).  callback_nonstatic_args: {"W": flow, "D": distance}.
+    Passed as callback_fn to usbs(), it runs natively at every condition
+    check; called directly, it runs the same code on the given arrays."""
+    l, W, D = _W_D(callback_nonstatic_args)
+    P, Omega = _lib.as_f64(P), _lib.as_f64(Omega)
+    r = P.shape[1]
+    ctx = context if context is not None else _lib.Context(device)
+    best = ctypes.c_double(0.0)
+    ctx.check(_lib.lib().usbs_qap_round(ctx.h, l, r, _lib.dptr(P), _lib.dptr(Omega), _lib.dptr(W),
+                                        _lib.dptr(D), None, None, ctypes.byref(best)))
+    return best.value
+
+
+def set_rounding(ctx, callback_nonstatic_args):
+    """Hand W and D to a context for usbs()'s native QAP rounding."""
+    l, W, D = _W_D(callback_nonstatic_args)
+    ctx.check(_lib.lib().usbs_set_qap_rounding(ctx.h, l, _lib.dptr(W), _lib.dptr(D)))

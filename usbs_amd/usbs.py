@@ -113,14 +113,20 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
     from .maxcut_helpers import compute_max_cut
     if sdp_state.X is not None:
         raise NotImplementedError("dense-X storage is not provided by the HIP backend")
+    from . import qap_helpers
     rounding = 0
     if callback_fn is not None and sdp_state.Omega is not None:
-        if callback_fn is not compute_max_cut:
-            raise NotImplementedError("only the max-cut rounding callback runs natively")
-        rounding = 1
+        if callback_fn is compute_max_cut:
+            rounding = 1
+        elif callback_fn is qap_helpers.qap_round:
+            rounding = 2
+        else:
+            raise NotImplementedError("only the max-cut and QAP rounding callbacks run natively")
     ctx = context if context is not None else _lib.Context(device)
     if upload:
         upload_problem(ctx, sdp_state, n, m)
+    if rounding == 2:
+        qap_helpers.set_rounding(ctx, callback_nonstatic_args)
     r = 0 if sdp_state.Omega is None else int(np.asarray(sdp_state.Omega).shape[1])
     prm = usbs_params(k_curr=k_curr, k_past=k_past, max_iters=int(max_iters),
                       lanczos_inner_iterations=lanczos_inner_iterations,
@@ -134,7 +140,8 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
                       max_infeas_eps=_finite(max_infeas_eps, -1e300), subprob_eps=subprob_eps,
                       cond_exp_base=cond_exp_base, SCALE_C=float(sdp_state.SCALE_C),
                       SCALE_X=float(sdp_state.SCALE_X),
-                      cut_scale=cut_scale_of(sdp_state, callback_nonstatic_args) if rounding else 0.0)
+                      cut_scale=(cut_scale_of(sdp_state, callback_nonstatic_args) if rounding == 1
+                                 else 0.0))
     y0 = _lib.as_f64(sdp_state.y)
     z0 = _lib.as_f64(sdp_state.z)
     if q0 is None:
