@@ -210,3 +210,24 @@ def test_usbs_geometric_condition_schedule(gpu_ctx):
         np.testing.assert_allclose(b["primal_obj"], a["primal_obj"], rtol=1e-5)
         assert b["callback_val"] == a["callback_val"]
     assert np.isfinite(out.primal_obj)
+
+
+def test_usbs_prints_the_reference_log_line(gpu_ctx, capsys):
+    """print_log=True prints one line per condition check in the format of
+    solver/usbs.py:514-530, carrying the record's values."""
+    import re
+    W = problems.erdos_renyi(100, 0.08, seed=3)
+    n = W.shape[0]
+    sg = maxcut_helpers.initialize_state(W, 4, omega_seed=0)
+    log = []
+    usbs(sg, n, n, 2.0 * n, 2.0, 1.0, 0.25, 4, 1, 3, 1e9, -1.0, -1.0, -1.0, 32, 10, 1e-7, 100, 1.0,
+         maxcut_helpers.compute_max_cut, None, sg.C, q0=problems.unit_start_vector(n, 1), log=log,
+         context=gpu_ctx, print_log=True)
+    lines = [l for l in capsys.readouterr().out.splitlines() if l.startswith("t: ")]
+    assert len(lines) == len(log) == 4
+    for line, rec in zip(lines, log):
+        m = re.search(r"t:\s(\d+) - curr_time:\s([\.0-9]+).*primal_obj:\s([-e\.0-9]+).*"
+                      r"callback_val:\s(.+)$", line)
+        assert int(m.group(1)) == rec["t"] and float(m.group(3)) == rec["primal_obj"]
+        assert float(m.group(4)) == rec["callback_val"]
+        assert abs(float(m.group(2)) - rec["curr_time"]) < 1e-6

@@ -16,9 +16,13 @@ after the reference's select the execution:
             faster; the results are the same either way)
   q0        Lanczos start vector (default: seeded normal, normalised; the
             reference draws it from jax.random with PRNGKey(0))
-  log       a list that receives one dict per condition check
+  log       a list that receives one dict per condition check (with
+            curr_time, the wall clock of the check, as the reference prints)
+  print_log print each condition check's line in the reference's format
+            (solver/usbs.py:514-530; usbs_amd.loaders.format_log_line)
 """
 import ctypes
+import time
 
 import numpy as np
 import scipy.sparse
@@ -101,7 +105,7 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
          lanczos_max_restarts, subprob_eps, subprob_max_iters, cond_exp_base, callback_fn,
          callback_static_args, callback_nonstatic_args, backend="hip", precision="fp64",
          exec_mode="perop", q0=None, log=None, device=0, context=None, upload=True,
-         warm_start=False):
+         warm_start=False, print_log=False):
     """Run USBS on the GPU and return the final SDPState (scaled units).
 
     warm_start=True starts from the y, z and sketch P that ``context`` kept on
@@ -155,14 +159,23 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
     cap = 1 << 16
     recs = (usbs_record * cap)()
     nlog = ctypes.c_int(0)
+    start_epoch = time.time()
     ctx.check(_lib.lib().usbs_solve(ctx.h, ctypes.byref(prm), _lib.dptr(y0), _lib.dptr(z0),
                                     float(sdp_state.tr_X), float(sdp_state.primal_obj),
                                     _lib.dptr(Om), _lib.dptr(P0), _lib.dptr(q0), _lib.dptr(y_out),
                                     _lib.dptr(z_out), _lib.dptr(P_out), _lib.dptr(scal), recs, cap,
                                     ctypes.byref(nlog)))
+    records = []
+    for i in range(min(nlog.value, cap)):
+        rec = {f: getattr(recs[i], f) for f in RECORD_FIELDS}
+        rec["curr_time"] = start_epoch + rec["time"]
+        records.append(rec)
     if log is not None:
-        for i in range(min(nlog.value, cap)):
-            log.append({f: getattr(recs[i], f) for f in RECORD_FIELDS})
+        log.extend(records)
+    if print_log:
+        from .loaders import format_log_line
+        for rec in records:
+            print(format_log_line(rec, start_epoch), flush=True)
     P_ret = P_out if sdp_state.P is not None or r > 0 else None
     return SDPState(C=sdp_state.C, A_indices=sdp_state.A_indices, A_data=sdp_state.A_data,
                     b=sdp_state.b, b_ineq_mask=sdp_state.b_ineq_mask, X=None, P=P_ret,
