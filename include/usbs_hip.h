@@ -149,7 +149,8 @@ typedef struct usbs_params {
   int32_t max_iters;
   int32_t lanczos_inner_iterations, lanczos_max_restarts;
   int32_t subprob_max_iters;
-  int32_t sketch_dim;  /* r > 0: Nystrom sketch P (n x r); 0: diag-only P (n)       */
+  int32_t sketch_dim;  /* r > 0: Nystrom sketch P (n x r); 0: diag-only P (n);       */
+                       /* -1: dense X (n x n, FP64; P0 / P_out carry X0 / X)        */
   int32_t rounding;    /* 1: max-cut rounding at every condition check (needs r > 0); */
                        /* 2: QAP rounding (needs usbs_set_qap_rounding, n = l^2 + 1) */
   int32_t precision;   /* USBS_PREC_*                                               */

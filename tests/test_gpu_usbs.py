@@ -231,3 +231,44 @@ def test_usbs_prints_the_reference_log_line(gpu_ctx, capsys):
         assert int(m.group(1)) == rec["t"] and float(m.group(3)) == rec["primal_obj"]
         assert float(m.group(4)) == rec["callback_val"]
         assert abs(float(m.group(2)) - rec["curr_time"]) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_usbs_dense_X_matches_oracle(gpu_ctx, precision):
+    """Dense-X storage (sketch_dim = -1, solver/usbs.py:575-576, 631-632): the
+    records and the final X (n x n) follow the oracle's, with the trajectory
+    rule of the other tests (FP64; mixed within 1e-5 of it), and diag(X) is
+    the constraint value z of the same iterate (max-cut: A(X) = diag X, up to
+    the clipping of S's rounding-level negative eigenvalues in z)."""
+    from oracle import common as ocommon
+    W = problems.erdos_renyi(150, 0.06, seed=14, signed=True)
+    n = W.shape[0]
+    q0 = problems.unit_start_vector(n, 2)
+    q1 = q0 + 1e-15 * np.random.default_rng(5).standard_normal(n)
+    so = omc.initialize_state(W, -1)
+    sg = maxcut_helpers.initialize_state(W, -1)
+    assert sg.X.shape == (n, n) and sg.Omega is None
+
+    def run_o(q):
+        log = []
+        out, _ = ousbs.usbs(so, n, n, 2.0 * n, 2.0, 1.0, 0.25, 4, 1, 4, 1e9, -1.0, -1.0, -1.0, 32,
+                            10, 1e-7, 100, 1.0, None, None, None, q, log=log)
+        return out, log
+
+    (out_o, log_o), (out_p, _) = run_o(q0), run_o(q1 / np.linalg.norm(q1))
+    log_g = []
+    print("case: dense X, ER(150), 4 iterations, %s" % precision, flush=True)
+    out_g = usbs(sg, n, n, 2.0 * n, 2.0, 1.0, 0.25, 4, 1, 4, 1e9, -1.0, -1.0, -1.0, 32, 10, 1e-7,
+                 100, 1.0, None, None, None, precision=precision, q0=q0, log=log_g, context=gpu_ctx)
+    assert out_g.P is None and out_g.X.shape == (n, n)
+    rel = 1e-5 if precision == "mixed" else 1e-6
+    for a, b in zip(log_o[1:], log_g[1:]):
+        np.testing.assert_allclose(b["primal_obj"], a["primal_obj"], rtol=rel)
+    scale = np.abs(out_o.X).max()
+    # FP32 storage perturbs the bundle at 1e-7, which four prox steps amplify:
+    # the mixed X is held to 1e-4 of the scale, as the other mixed-vs-FP64 checks
+    tol = max((1e-4 if precision == "mixed" else 1e-6) * scale,
+              20.0 * np.abs(out_o.X - out_p.X).max())
+    assert np.abs(out_g.X - out_o.X).max() <= tol, (np.abs(out_g.X - out_o.X).max(), tol)
+    assert abs(np.trace(out_g.X) - out_g.tr_X) <= 1e-9 * abs(out_g.tr_X)
+    np.testing.assert_allclose(np.diag(out_g.X), out_g.z, rtol=1e-6, atol=1e-9)

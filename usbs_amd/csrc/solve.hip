@@ -40,6 +40,39 @@ __global__ void k_copy1(double* dst, const double* src) {
   if (threadIdx.x == 0) *dst = *src;
 }
 
+// Dense-X storage (solver/usbs.py:575-576, 631-632):
+// X[i, j] = eta Xbar[i, j] + sum_{a < k} A[i, a] B[j, a]  (X, Xbar: n x n
+// column-major with leading dimension ldx, FP64; A, B: n x k column-major
+// with leading dimension ld).  X may be Xbar (each entry is read and then
+// written by one thread).  A block covers 256 rows and 32 columns; the 32
+// rows of B it needs sit in LDS.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dense_rankk(int n, int k, const double* __restrict__ eta_ptr, const double* Xbar, double* X,
+              long long ldx, const T* __restrict__ A, const T* __restrict__ B, long long ld) {
+  __shared__ double Bs[16][32];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j0 = blockIdx.y * 32;
+  for (int idx = threadIdx.x; idx < 16 * 32; idx += 256) {
+    const int a = idx / 32, jj = idx % 32;
+    Bs[a][jj] = (a < k && j0 + jj < n) ? (double)B[(j0 + jj) + (long long)a * ld] : 0.0;
+  }
+  __syncthreads();
+  if (i >= n) return;
+  double ai[16];
+#pragma unroll
+  for (int a = 0; a < 16; ++a) ai[a] = a < k ? (double)A[i + (long long)a * ld] : 0.0;
+  const double eta = *eta_ptr;
+  const int jn = min(32, n - j0);
+  for (int jj = 0; jj < jn; ++jj) {
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) s += ai[a] * Bs[a][jj];
+    const long long e = i + (long long)(j0 + jj) * ldx;
+    X[e] = eta * Xbar[e] + s;
+  }
+}
+
 template <typename T>
 struct Solver {
   Ctx* c = nullptr;
@@ -62,6 +95,9 @@ struct Solver {
          *Gn = nullptr, *lam = nullptr, *cut = nullptr, *sc = nullptr, *q0 = nullptr;
   signed char* sg = nullptr;
   CutWeights cutw{};       // weights of the rounding's cut values (C / cut_scale)
+  bool dense = false;      // dense-X storage (sketch_dim = -1): X, Xbar n x n FP64
+  double *X = nullptr, *Xbar = nullptr;
+  T* VS = nullptr;         // V S (n x k), the factor of the X update
   double* dE = nullptr;    // QAP rounding: the Nystrom factor (n x r row-major)
   std::vector<double> hE;  // ... its host copy
   double qap_cb = NAN;     // ... the last rounded QAP objective
@@ -137,7 +173,8 @@ struct Solver {
     kc = p.k_curr;
     kp = p.k_past;
     k = kc + kp;
-    r = p.sketch_dim;
+    dense = p.sketch_dim == -1;
+    r = dense ? 0 : p.sketch_dim;
     d = k * (k + 1) / 2;
     mL = p.lanczos_inner_iterations;
     ld = ((long long)n + 63) / 64 * 64;
@@ -152,6 +189,9 @@ struct Solver {
     const int rr = r > 0 ? r : 1;
     if (!(PO = alloc<T>(ld * 2 * rr)) || !(Pbar = alloc<T>(ld * rr))) return USBS_ERR_OOM;
     if (!(op64 = alloc<double>(P->nnz))) return USBS_ERR_OOM;
+    if (dense && (!(X = alloc<double>((size_t)ld * n)) || !(Xbar = alloc<double>((size_t)ld * n)) ||
+                  !(VS = alloc<T>((size_t)ld * k))))
+      return USBS_ERR_OOM;
     if (sizeof(T) == 4 && !(op32 = alloc<float>(P->nnz))) return USBS_ERR_OOM;
     const int qt = qbase_tile_rows(m, d);
     const size_t pw = std::max(
@@ -400,6 +440,15 @@ struct Solver {
       }
     }
     // --- primal / sketch / dual candidate (solver/usbs.py:568-585)
+    if (dense) {
+      // X = eta Xbar + V S V^T, then Xbar = eta Xbar + Fc Fc^T (usbs.py:576, 632)
+      const dim3 g((unsigned)((n + 255) / 256), (unsigned)((n + 31) / 32));
+      CK((launch_tsmm<T, T>(n, V(), ld, k, S_io, k, VS, ld, 0.0, nullptr, 0, s)));
+      k_dense_rankk<T><<<g, 256, 0, s>>>(n, k, sc + SC_ETA, Xbar, X, ld, VS, V(), ld);
+      CK((int)hipGetLastError());
+      k_dense_rankk<T><<<g, 256, 0, s>>>(n, kc, sc + SC_ETA, Xbar, Xbar, ld, F, F, ld);
+      CK((int)hipGetLastError());
+    }
     CK(launch_constraint_update<T>(*P, k, kc, F, ld, sc + SC_ETA, zbar, y,
                                    P->has_ineq ? ups : nullptr, rho, z, zbn, ycand, partial,
                                    sc + SC_BYC, s));
@@ -539,6 +588,19 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
     }
     if (launch_tsgram<T, T>(n, c->P.tile, c->P.ntiles, S.Om(), ld, r, S.Om(), ld, r, S.partial, S.OtO, s))
       return set_err(c, USBS_ERR_HIP, "Omega Gram");
+  } else if (S.dense) {
+    // dense X: P0 carries X0 (n x n row-major); X and Xbar start from it
+    S.fill_T(S.Pbar, ld);
+    const long long nn = (long long)n * n;
+    if (P0) {
+      double* d_x = up(P0, (size_t)nn);
+      if (!d_x) return USBS_ERR_OOM;
+      k_load_colmajor<double><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(n, n, d_x, S.X, ld);
+      k_load_colmajor<double><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(n, n, d_x, S.Xbar, ld);
+    } else {
+      S.fill(S.X, ld * n, 0.0);
+      S.fill(S.Xbar, ld * n, 0.0);
+    }
   } else {
     if (warm) {
     } else if (P0) {
@@ -631,10 +693,14 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
   }
   if (log_len) *log_len = nlog;
   // outputs
-  double* d_out = (double*)ctx_alloc(c, tmp, sizeof(double) * (size_t)n * (r > 0 ? r : 1));
+  // P (n x r, or n for diag-only), or X (n x n) in the dense mode
+  const int rr = S.dense ? n : (r > 0 ? r : 1);
+  double* d_out = (double*)ctx_alloc(c, tmp, sizeof(double) * (size_t)n * rr);
   if (!d_out) return USBS_ERR_OOM;
-  const int rr = r > 0 ? r : 1;
-  k_store_rowmajor<T><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.P_(), ld, d_out);
+  if (S.dense)
+    k_store_rowmajor<double><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.X, ld, d_out);
+  else
+    k_store_rowmajor<T><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.P_(), ld, d_out);
   std::vector<double> hs(SC_COUNT);
   if ((y_out && hipMemcpyAsync(y_out, S.y, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
       (z_out && hipMemcpyAsync(z_out, S.z, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
@@ -642,8 +708,14 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
       hipMemcpyAsync(hs.data(), S.sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return set_err(c, USBS_ERR_HIP, "reading results");
-  // keep y, z, P on the device for a warm start of the next solve
-  if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld) {
+  // keep y, z, P on the device for a warm start of the next solve (not
+  // in the dense mode, which has no sketch to keep)
+  if (S.dense) {
+    ctx_free_pool(c->keep_bufs);
+    c->keep_y = c->keep_z = c->keep_P = nullptr;
+    c->keep_n = c->keep_m = 0;
+    c->keep_r = -1;
+  } else if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld) {
     ctx_free_pool(c->keep_bufs);
     c->keep_y = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
     c->keep_z = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
@@ -660,7 +732,7 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
     c->keep_ld = ld;
   }
   c->keep_sx = p.SCALE_X;
-  {
+  if (!S.dense) {
     const long long cnt = ld * rk;
     k_cast<T, double><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, S.P_(), c->keep_P);
     if (hipMemcpyAsync(c->keep_y, S.y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
@@ -730,7 +802,12 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
     return set_err(c, USBS_ERR_INVALID, "lanczos_inner_iterations must be in [k, min(n, 32)]");
   if (p.lanczos_max_restarts < 1 || p.subprob_max_iters < 1 || p.max_iters < 0)
     return set_err(c, USBS_ERR_INVALID, "iteration limits must be positive");
-  if (p.sketch_dim < 0 || p.sketch_dim > 48) return set_err(c, USBS_ERR_INVALID, "sketch_dim in [0, 48]");
+  if (p.sketch_dim < -1 || p.sketch_dim > 48)
+    return set_err(c, USBS_ERR_INVALID, "sketch_dim in [-1, 48] (-1: dense X)");
+  if (p.sketch_dim == -1 && (Omega || p.rounding || p.warm))
+    return set_err(c, USBS_ERR_INVALID, "the dense-X mode takes no Omega, rounding or warm start");
+  if (p.sketch_dim == -1 && (long long)c->P.n * c->P.n > (1ll << 31))
+    return set_err(c, USBS_ERR_INVALID, "the dense-X mode holds n x n doubles: n too large");
   if (p.sketch_dim > 0 && !Omega) return set_err(c, USBS_ERR_INVALID, "Omega required when sketch_dim > 0");
   if (p.rounding && p.sketch_dim == 0) return set_err(c, USBS_ERR_INVALID, "rounding needs a sketch");
   if (p.rounding < 0 || p.rounding > 2) return set_err(c, USBS_ERR_INVALID, "rounding must be 0, 1 or 2");

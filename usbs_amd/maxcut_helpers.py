@@ -42,7 +42,7 @@ def initialize_state(C, sketch_dim, omega_seed=0, Omega=None):
     A_data, A_indices, b, b_ineq_mask = get_all_problem_data(L)
     m = b.shape[0]
     if sketch_dim == -1:
-        raise NotImplementedError("dense-X storage (sketch_dim=-1) is not provided by the HIP backend")
+        X, P, Omega = np.zeros((n, n)), None, None
     elif sketch_dim > 0:
         X = None
         if Omega is None:

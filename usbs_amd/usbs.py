@@ -115,8 +115,7 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
     if backend != "hip":
         raise ValueError("backend=%r is not available in usbs_amd (only 'hip')" % (backend,))
     from .maxcut_helpers import compute_max_cut
-    if sdp_state.X is not None:
-        raise NotImplementedError("dense-X storage is not provided by the HIP backend")
+    dense = sdp_state.X is not None
     from . import qap_helpers
     rounding = 0
     if callback_fn is not None and sdp_state.Omega is not None:
@@ -131,7 +130,7 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
         upload_problem(ctx, sdp_state, n, m)
     if rounding == 2:
         qap_helpers.set_rounding(ctx, callback_nonstatic_args)
-    r = 0 if sdp_state.Omega is None else int(np.asarray(sdp_state.Omega).shape[1])
+    r = -1 if dense else 0 if sdp_state.Omega is None else int(np.asarray(sdp_state.Omega).shape[1])
     prm = usbs_params(k_curr=k_curr, k_past=k_past, max_iters=int(max_iters),
                       lanczos_inner_iterations=lanczos_inner_iterations,
                       lanczos_max_restarts=lanczos_max_restarts, subprob_max_iters=subprob_max_iters,
@@ -151,10 +150,15 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
     if q0 is None:
         q0 = problems.unit_start_vector(n, 0)
     q0 = _lib.as_f64(q0)
-    Om = None if r == 0 else _lib.as_f64(sdp_state.Omega)
-    P0 = None if sdp_state.P is None else _lib.as_f64(sdp_state.P)
+    Om = None if r <= 0 else _lib.as_f64(sdp_state.Omega)
+    if dense:  # X0 in, X out (n x n) through the P arguments
+        P0 = _lib.as_f64(sdp_state.X)
+        if P0.shape != (n, n):
+            raise ValueError("X must be n x n")
+    else:
+        P0 = None if sdp_state.P is None else _lib.as_f64(sdp_state.P)
     y_out, z_out = np.zeros(m), np.zeros(m)
-    P_out = np.zeros((n, r)) if r > 0 else np.zeros(n)
+    P_out = np.zeros((n, n)) if dense else np.zeros((n, r)) if r > 0 else np.zeros(n)
     scal = np.zeros(6)
     cap = 1 << 16
     recs = (usbs_record * cap)()
@@ -176,8 +180,9 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
         from .loaders import format_log_line
         for rec in records:
             print(format_log_line(rec, start_epoch), flush=True)
-    P_ret = P_out if sdp_state.P is not None or r > 0 else None
+    P_ret = None if dense else P_out if sdp_state.P is not None or r > 0 else None
     return SDPState(C=sdp_state.C, A_indices=sdp_state.A_indices, A_data=sdp_state.A_data,
-                    b=sdp_state.b, b_ineq_mask=sdp_state.b_ineq_mask, X=None, P=P_ret,
+                    b=sdp_state.b, b_ineq_mask=sdp_state.b_ineq_mask,
+                    X=P_out if dense else None, P=P_ret,
                     Omega=sdp_state.Omega, y=y_out, z=z_out, tr_X=scal[0], primal_obj=scal[1],
                     SCALE_C=sdp_state.SCALE_C, SCALE_X=sdp_state.SCALE_X, SCALE_A=sdp_state.SCALE_A)
