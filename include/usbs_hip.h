@@ -168,6 +168,9 @@ typedef struct usbs_params {
                        /* (scaled) C; the reference's callers pass C / SCALE_C      */
                        /* (scripts/warm_start_maxcut_usbs.py:121), i.e. SCALE_C;    */
                        /* 0 means SCALE_C                                           */
+  double lanczos_rtol; /* Ritz pairs count as converged below                       */
+                       /* max(subprob_eps, lanczos_rtol * max|theta|); 0: the       */
+                       /* reference's absolute test (solver/lanczos.py:205-208)     */
 } usbs_params;
 
 /* One condition-check record: the fields of the reference's log line

@@ -213,7 +213,7 @@ __device__ void lz_build_T(int m, int k, const double* alpha, const double* beta
 // ----------------------------------------------------------------------------
 // Ritz step (one block): arrowhead T, eigh, convergence count, saved set,
 // reorder, new alpha/beta and the rotation Yr (lanczos.py:128-168, 202-221).
-__device__ __forceinline__ void lz_ritz(int m, int nd, double tol, int max_restarts, double* alpha, double* beta,
+__device__ __forceinline__ void lz_ritz(int m, int nd, double tol, double rtol, int max_restarts, double* alpha, double* beta,
                         double* Yr, int* ctrl, double* lds) {
   const int t = threadIdx.x, nt = blockDim.x;
   const int M = m + (m & 1), lda = M + 1;
@@ -239,10 +239,14 @@ __device__ __forceinline__ void lz_ritz(int m, int nd, double tol, int max_resta
   if (jacobi) block_eigh(Tw, lda, m, w, Y, Vw, cs, sched, red);
   const double bm = beta[m - 1];
   if (t == 0) {
+    // convergence test (lanczos.py:205-208); with rtol > 0 the threshold has
+    // a floor relative to the spectrum's scale (the FP32 basis policy)
+    const double scale = fmax(fabs(w[0]), fabs(w[m - 1]));
+    const double thr = (rtol > 0.0 && rtol * scale > tol) ? rtol * scale : tol;
     int nc = m;
     for (int j = 0; j < m; ++j) {
       const double res = bm * fabs(Y[(m - 1) * m + j]);
-      if (!(res < tol)) { nc = j; break; }
+      if (!(res < thr)) { nc = j; break; }
     }
     int delta = m - nd;
     const int d2 = 2 * (m - nc) / 5;
@@ -612,7 +616,7 @@ template <typename T>
 __global__ void __launch_bounds__(LZ_RITZ_NT) k_lz_ritz(LzInst<T> L) {
   extern __shared__ double lds[];
   if (L.ctrl[LZ_PENDING]) return;
-  lz_ritz(L.m, L.nd, L.tol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
+  lz_ritz(L.m, L.nd, L.tol, L.rtol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
 }
 
 // Ritz step of restart r for every instance still running (block = instance).
@@ -622,7 +626,7 @@ __global__ void __launch_bounds__(LZ_RITZ_NT) k_lz_ritz_multi(const LzInst<T>* _
   extern __shared__ double lds[];
   const LzInst<T>& L = insts[blockIdx.x];
   if (L.ctrl[LZ_DONE] || L.ctrl[LZ_RESTART] != r || L.ctrl[LZ_ERROR]) return;
-  lz_ritz(L.m, L.nd, L.tol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
+  lz_ritz(L.m, L.nd, L.tol, L.rtol, L.max_restarts, L.alpha, L.beta, L.Yr, L.ctrl, lds);
 }
 
 template <typename T, int MAXC>

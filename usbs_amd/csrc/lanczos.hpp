@@ -28,6 +28,9 @@ template <typename T>
 struct LzInst {
   int n, tile, ntiles, m, nd, max_restarts, pstride, nblocks;
   double tol;
+  double rtol;   // Ritz residual floor relative to the largest |Ritz value|:
+                 // a pair counts as converged when its residual estimate is
+                 // below max(tol, rtol * max|theta|) (0: tol alone)
   const int* row_ptr;
   const int* col;
   const T* val;
@@ -53,7 +56,7 @@ struct LzInst {
   int reserved;      // keeps the struct free of padding bytes (it is compared
                      // bytewise as the key of the captured per-op graphs)
 };
-static_assert(sizeof(LzInst<double>) == 8 * 4 + 8 + 8 * 17 + 4 * 4, "LzInst has padding");
+static_assert(sizeof(LzInst<double>) == 8 * 4 + 8 * 2 + 8 * 17 + 4 * 4, "LzInst has padding");
 
 // Largest LDS block of a persistent workgroup that caches its tile.
 #define LZ_CACHE_MAX_BYTES (64 * 1024)

@@ -223,6 +223,7 @@ struct Solver {
     int rc = make_lanczos<T>(c, pool, *P, mL, k, p.lanczos_max_restarts, p.subprob_eps,
                              sizeof(T) == 4 ? (const T*)op32 : (const T*)op64, lz);
     if (rc) return rc;
+    lz.rtol = p.lanczos_rtol;
     rc = make_lanczos<T>(c, pool, *P, k, k, 1, 0.0, (const T*)nullptr, qr);
     if (rc) return rc;
     if (lz.ldq != ld || qr.ldq != ld) return set_err(c, USBS_ERR_INVALID, "leading dimension mismatch");
@@ -800,6 +801,8 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   if (p.lanczos_inner_iterations < k || p.lanczos_inner_iterations > 32 ||
       p.lanczos_inner_iterations > c->P.n)
     return set_err(c, USBS_ERR_INVALID, "lanczos_inner_iterations must be in [k, min(n, 32)]");
+  if (!(p.lanczos_rtol >= 0.0) || !(p.lanczos_rtol < 1.0))
+    return set_err(c, USBS_ERR_INVALID, "lanczos_rtol must be in [0, 1)");
   if (p.lanczos_max_restarts < 1 || p.subprob_max_iters < 1 || p.max_iters < 0)
     return set_err(c, USBS_ERR_INVALID, "iteration limits must be positive");
   if (p.sketch_dim < -1 || p.sketch_dim > 48)

@@ -40,7 +40,7 @@ class usbs_params(ctypes.Structure):
                [(name, ctypes.c_double) for name in
                 ("trace_ub", "trace_factor", "rho", "beta", "max_time", "obj_gap_eps",
                  "infeas_gap_eps", "max_infeas_eps", "subprob_eps", "cond_exp_base", "SCALE_C",
-                 "SCALE_X", "cut_scale")]
+                 "SCALE_X", "cut_scale", "lanczos_rtol")]
 
 
 RECORD_FIELDS = ("t", "time", "pen_dual_obj", "cand_pen_dual_obj", "lb_spec_est", "primal_obj",
@@ -105,7 +105,7 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
          lanczos_max_restarts, subprob_eps, subprob_max_iters, cond_exp_base, callback_fn,
          callback_static_args, callback_nonstatic_args, backend="hip", precision="fp64",
          exec_mode="perop", q0=None, log=None, device=0, context=None, upload=True,
-         warm_start=False, print_log=False):
+         warm_start=False, print_log=False, lanczos_rtol=None):
     """Run USBS on the GPU and return the final SDPState (scaled units).
 
     warm_start=True starts from the y, z and sketch P that ``context`` kept on
@@ -144,7 +144,8 @@ def usbs(sdp_state, n, m, trace_ub, trace_factor, rho, beta, k_curr, k_past, max
                       cond_exp_base=cond_exp_base, SCALE_C=float(sdp_state.SCALE_C),
                       SCALE_X=float(sdp_state.SCALE_X),
                       cut_scale=(cut_scale_of(sdp_state, callback_nonstatic_args) if rounding == 1
-                                 else 0.0))
+                                 else 0.0),
+                      lanczos_rtol=0.0 if lanczos_rtol is None else float(lanczos_rtol))
     y0 = _lib.as_f64(sdp_state.y)
     z0 = _lib.as_f64(sdp_state.z)
     if q0 is None:
