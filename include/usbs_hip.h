@@ -199,6 +199,36 @@ int usbs_solve(usbs_ctx* ctx, const usbs_params* params, const double* y0, const
                const double* q0, double* y_out, double* z_out, double* P_out, double* scal_out,
                usbs_record* log, int log_cap, int* log_len);
 
+/* One solve of a usbs_solve_batch call: its context (with its problem
+ * uploaded), its parameters, and the arguments and outputs of usbs_solve. */
+typedef struct usbs_batch_item {
+  usbs_ctx* ctx;
+  const usbs_params* params;
+  const double *y0, *z0;
+  double tr_X0, primal_obj0;
+  const double *Omega, *P0, *q0;
+  double *y_out, *z_out, *P_out, *scal_out;
+  usbs_record* log;
+  int32_t log_cap;
+  int32_t* log_len;
+} usbs_batch_item;
+
+/*
+ * count usbs_solve calls on one device in lockstep, for throughput when many
+ * problems are solved at once (the reference solves one problem per call,
+ * solver/usbs.py:436-789; each item here gives that call's results).  Each
+ * solve runs on its own context and stream; per iteration one persistent
+ * (cooperative) Lanczos launch per restart serves every solve still running,
+ * and each solve's outputs and records are bitwise those of usbs_solve with
+ * exec_mode USBS_EXEC_PERSISTENT (record times aside).  The items must use
+ * distinct contexts on one device and share precision, k_curr, k_past,
+ * lanczos_inner_iterations and lanczos_max_restarts; exec_mode must be
+ * USBS_EXEC_AUTO or USBS_EXEC_PERSISTENT.  A solve leaves the batch when its
+ * stop test holds.  On an error every item's context holds the message,
+ * which names the failing item.
+ */
+int usbs_solve_batch(const usbs_batch_item* items, int32_t count);
+
 /* Parameters of cgal() (solver/cgal.py:21-39), plus the execution choices. */
 typedef struct usbs_cgal_params {
   int32_t max_iters;

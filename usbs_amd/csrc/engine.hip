@@ -320,6 +320,70 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
   return USBS_OK;
 }
 
+template <typename T>
+int LzBatch<T>::build(Ctx* c0, std::vector<LzInst<T>>& L, int m_) {
+  ctx_free_pool(pool);
+  m = m_;
+  ninst = (int)L.size();
+  total_tiles = 0;
+  for (auto& x : L) total_tiles += x.ntiles;
+  // workgroups: one per tile with the LDS cache when every instance can
+  // cache and all fit together, else proportional shares without it
+  size_t cb = 0;
+  bool all_cache = true;
+  for (int q = 0; q < ninst; ++q) {
+    L[q].nblocks = L[q].ntiles;
+    const size_t b = lanczos_cache_bytes(L[q]);
+    if (b == 0) all_cache = false;
+    cb = std::max(cb, b);
+  }
+  if (!all_cache) cb = 0;
+  int cap = lanczos_max_coresident_blocks(cb ? cb : lanczos_tile_lds_bytes(m), c0->device);
+  if (cb && cap < total_tiles) {
+    cb = 0;
+    cap = lanczos_max_coresident_blocks(lanczos_tile_lds_bytes(m), c0->device);
+  }
+  if (cap < ninst) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
+  total_wgs = 0;
+  for (int q = 0; q < ninst; ++q) {
+    int nb = L[q].ntiles;
+    if (total_tiles > cap) nb = std::max(1, (int)((long long)L[q].ntiles * cap / total_tiles));
+    L[q].nblocks = nb;
+    total_wgs += nb;
+  }
+  if (total_wgs > cap) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
+  cache = cb;
+  std::vector<int2> wgmap, tilemap;
+  for (int q = 0; q < ninst; ++q) {
+    for (int i = 0; i < L[q].nblocks; ++i) wgmap.push_back(make_int2(q, i));
+    for (int i = 0; i < L[q].ntiles; ++i) tilemap.push_back(make_int2(q, i));
+  }
+  d_insts = (LzInst<T>*)ctx_alloc(c0, pool, sizeof(LzInst<T>) * ninst);
+  d_wg = (int*)ctx_alloc(c0, pool, sizeof(int2) * wgmap.size());
+  d_tm = (int*)ctx_alloc(c0, pool, sizeof(int2) * tilemap.size());
+  if (!d_insts || !d_wg || !d_tm) return USBS_ERR_OOM;
+  USBS_HIP(c0, hipMemcpy(d_insts, L.data(), sizeof(LzInst<T>) * ninst, hipMemcpyHostToDevice));
+  USBS_HIP(c0, hipMemcpy(d_wg, wgmap.data(), sizeof(int2) * wgmap.size(), hipMemcpyHostToDevice));
+  USBS_HIP(c0, hipMemcpy(d_tm, tilemap.data(), sizeof(int2) * tilemap.size(), hipMemcpyHostToDevice));
+  rowwise = 0;
+  ctrls.assign(ninst, nullptr);
+  for (int q = 0; q < ninst; ++q) {
+    rowwise |= (L[q].tile > 256) ? 1 : 0;
+    ctrls[q] = L[q].ctrl;
+  }
+  return USBS_OK;
+}
+
+template <typename T>
+int LzBatch<T>::run(hipStream_t s, int max_restarts, int first_check, int* launched) {
+  const LzStop stop{ctrls.data(), first_check, launched};
+  return lanczos_persistent<T>(d_insts, ninst, total_wgs, m, max_restarts, s, d_wg, d_tm,
+                               total_tiles, 1ull << 26, cache, nullptr, rowwise, &stop);
+}
+
+template struct LzBatch<double>;
+template struct LzBatch<float>;
+
 // Several instances (one per context, each with its own uploaded problem) in
 // one persistent launch per restart.  All contexts are on one device; the
 // launch runs on the first context's stream.  Workgroups: one per tile of
@@ -358,51 +422,11 @@ static int run_lanczos_batch(Ctx** cs, int ninst, double C_coef, const double* c
     if (reset_lanczos(L[q], d_q0, s)) return set_err(c0, USBS_ERR_HIP, "reset launch failed");
     total_tiles += L[q].ntiles;
   }
-  // workgroups: try one per tile with the LDS cache, else shares without it
-  size_t cache = 0;
-  bool all_cache = true;
-  for (int q = 0; q < ninst; ++q) {
-    L[q].nblocks = L[q].ntiles;
-    const size_t b = lanczos_cache_bytes(L[q]);
-    if (b == 0) all_cache = false;
-    cache = std::max(cache, b);
-  }
-  if (!all_cache) cache = 0;
-  int cap = lanczos_max_coresident_blocks(cache ? cache : lanczos_tile_lds_bytes(m), c0->device);
-  if (cache && cap < total_tiles) {
-    cache = 0;
-    cap = lanczos_max_coresident_blocks(lanczos_tile_lds_bytes(m), c0->device);
-  }
-  if (cap < ninst) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
-  int total_wgs = 0;
-  for (int q = 0; q < ninst; ++q) {
-    int nb = L[q].ntiles;
-    if (total_tiles > cap) nb = std::max(1, (int)((long long)L[q].ntiles * cap / total_tiles));
-    L[q].nblocks = nb;
-    total_wgs += nb;
-  }
-  if (total_wgs > cap) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
-  std::vector<int2> wgmap, tilemap;
-  for (int q = 0; q < ninst; ++q) {
-    for (int i = 0; i < L[q].nblocks; ++i) wgmap.push_back(make_int2(q, i));
-    for (int i = 0; i < L[q].ntiles; ++i) tilemap.push_back(make_int2(q, i));
-  }
-  LzInst<T>* d_inst = (LzInst<T>*)ctx_alloc(c0, pool0, sizeof(LzInst<T>) * ninst);
-  int* d_wg = (int*)ctx_alloc(c0, pool0, sizeof(int2) * wgmap.size());
-  int* d_tm = (int*)ctx_alloc(c0, pool0, sizeof(int2) * tilemap.size());
-  if (!d_inst || !d_wg || !d_tm) return USBS_ERR_OOM;
-  USBS_HIP(c0, hipMemcpyAsync(d_inst, L.data(), sizeof(LzInst<T>) * ninst, hipMemcpyHostToDevice, s));
-  USBS_HIP(c0, hipMemcpyAsync(d_wg, wgmap.data(), sizeof(int2) * wgmap.size(), hipMemcpyHostToDevice, s));
-  USBS_HIP(c0, hipMemcpyAsync(d_tm, tilemap.data(), sizeof(int2) * tilemap.size(), hipMemcpyHostToDevice, s));
-  int rowwise = 0;
-  std::vector<int*> ctrls(ninst);
-  for (int q = 0; q < ninst; ++q) {
-    rowwise |= (L[q].tile > 256) ? 1 : 0;
-    ctrls[q] = L[q].ctrl;
-  }
-  const LzStop stop{ctrls.data(), 2, nullptr};
-  int rc = lanczos_persistent<T>(d_inst, ninst, total_wgs, m, max_restarts, s, d_wg, d_tm,
-                                 total_tiles, 1ull << 26, cache, nullptr, rowwise, &stop);
+  LzBatch<T> lb;
+  int rc = lb.build(c0, L, m);
+  if (rc) return rc;
+  if (hipStreamSynchronize(s) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "synchronise");
+  rc = lb.run(s, max_restarts);
   if (rc) return set_err(c0, USBS_ERR_HIP, std::string("lanczos batch: ") + hipGetErrorString((hipError_t)rc));
   for (int q = 0; q < ninst; ++q) {
     const Problem& P = cs[q]->P;

@@ -123,6 +123,27 @@ int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int
 // usbs_sketch_update_noalloc (once per device, with the device current).
 int noalloc_init_device(int device);
 
+// Several Lanczos instances (possibly of different sizes, one device) advanced
+// together by one persistent (cooperative) launch per restart: build() plans
+// the workgroups -- one per row tile with the tile cached in LDS when all fit
+// on the device at once, else proportional shares -- and uploads the
+// descriptors (the instances' nblocks are set); run() issues the restarts on
+// stream s, reading the instances' control words after first_check restarts
+// and at each doubling to stop once all are done.
+template <typename T>
+struct LzBatch {
+  std::vector<DevBuf> pool;
+  LzInst<T>* d_insts = nullptr;
+  int* d_wg = nullptr;
+  int* d_tm = nullptr;
+  int ninst = 0, m = 0, total_wgs = 0, total_tiles = 0, rowwise = 0;
+  size_t cache = 0;
+  std::vector<int*> ctrls;
+  ~LzBatch() { ctx_free_pool(pool); }
+  int build(Ctx* c0, std::vector<LzInst<T>>& L, int m);
+  int run(hipStream_t s, int max_restarts, int first_check = 2, int* launched = nullptr);
+};
+
 // kernels in ops.hip
 int launch_assemble(const Problem& P, double coefC, const double* z1, double a1, const double* z2,
                     double a2, const double* z3, double a3, double* out64, float* out32,

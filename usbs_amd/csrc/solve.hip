@@ -12,7 +12,9 @@
 //   SpMM+Gram (g_1) -> lower-bound IPM -> scalar bookkeeping -> y select
 //   V_next = orth([V E_S[:, kc:], cand_vecs])
 //   condition: Nystrom rounding (max-cut) + gaps
+#include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <vector>
 
@@ -334,13 +336,21 @@ struct Solver {
     return USBS_OK;
   }
 
+  // the start of a Lanczos call: column 0 = q0, control words reset
+  int lanczos_start(int nd) {
+    LzInst<T> L = lz;
+    L.nd = nd;
+    k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
+    CK((int)hipGetLastError());
+    return USBS_OK;
+  }
+
   // Lanczos on the assembled operator (op64/op32 already hold -C + A*(z)).
   int lanczos(int nd) {
     if (tuning) return tune_lanczos(nd);
     LzInst<T> L = lz;
     L.nd = nd;
-    k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
-    CK((int)hipGetLastError());
+    if (lanczos_start(nd)) return USBS_ERR_HIP;
     if (p.exec_mode == USBS_EXEC_PEROP) {
       if (use_ptm) {
         ptm.ms = 0.0;
@@ -405,7 +415,9 @@ struct Solver {
     return USBS_OK;
   }
 
-  int body() {
+  // one USBS iteration up to the candidate Lanczos call (its operator
+  // -C + A*(ycand) assembled; the lower-bound IPM forked onto the side stream)
+  int body_pre() {
     const double tu = p.trace_ub, rho = p.rho;
     // --- prox subproblem inputs (solver/usbs.py:53-61)
     CK(launch_dots3(*P, zbar, y, P->has_ineq ? ups : nullptr, partial, sc + SC_ZZ, s));
@@ -465,7 +477,19 @@ struct Solver {
     if (project(Glb, s2)) return USBS_ERR_HIP;
     CK(launch_ipm_lb(k, p.subprob_max_iters, tu, p.subprob_eps, Glb, sc, s2));
     CK((int)hipEventRecord(ev_join, s2));
+    return USBS_OK;
+  }
+
+  // the candidate Lanczos call happens between body_pre and body_post
+  int body() {
+    int rc = body_pre();
+    if (rc) return rc;
     if (lanczos(kc)) return USBS_ERR_HIP;
+    return body_post();
+  }
+
+  int body_post() {
+    const double tu = p.trace_ub;
     CK((int)hipStreamWaitEvent(s, ev_join, 0));
     ScalarArgs sa{};
     sa.tu = tu; sa.tf = p.trace_factor; sa.beta = p.beta; sa.SC = p.SCALE_C; sa.SX = p.SCALE_X;
@@ -530,130 +554,165 @@ struct Solver {
 #undef CK
 };
 
+// One usbs_solve, in the stages the batched driver (usbs_solve_batch) also
+// runs: prepare (buffers, start point, scalar block), the initial bundle
+// (operator assembly, a Lanczos call the caller issues, bundle and
+// penalised dual), condition checks, and the outputs.  The statements are
+// those of the single-solve loop, in its order.
 template <typename T>
-static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const double* z0, double trX0,
-                     double pobj0, const double* Omega, const double* P0, const double* q0h,
-                     double* y_out, double* z_out, double* P_out, double* scal_out,
-                     usbs_record* log, int log_cap, int* log_len) {
+struct SolveRun {
+  Ctx* c = nullptr;
+  usbs_params p{};
   Solver<T> S;
-  int rc = S.setup(c, p);
-  if (rc) return rc;
-  hipStream_t s = c->stream;
-  const int n = S.n, m = S.m, r = S.r;
-  const long long ld = S.ld;
-  std::vector<DevBuf> tmp;
-  struct G { std::vector<DevBuf>& t; ~G() { ctx_free_pool(t); } } guard{tmp};
-  auto up = [&](const double* h, size_t cnt) -> double* {
+  std::vector<DevBuf> tmp;  // per-call uploads and the output staging
+  double t_origin = 0.0;
+  int nlog = 0;
+  long long iters = 0;
+  bool go = true;
+  usbs_record* log = nullptr;
+  int log_cap = 0;
+
+  ~SolveRun() { ctx_free_pool(tmp); }
+
+  double* up(const double* h, size_t cnt) {
     double* dptr = (double*)ctx_alloc(c, tmp, sizeof(double) * cnt);
     if (dptr && hipMemcpy(dptr, h, sizeof(double) * cnt, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     return dptr;
-  };
-  const int rk = r > 0 ? r : 1;  // columns of P (the diag-only P is one column)
-  const bool warm = p.warm != 0;
-  if (warm) {
-    if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld)
-      return set_err(c, USBS_ERR_STATE, "warm start needs a previous solve of the same size on this context");
-    // y carries over; z and P are taken from the kept solve's SCALE_X to this one's
-    if (hipMemcpyAsync(S.y, c->keep_y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess)
-      return set_err(c, USBS_ERR_HIP, "copy of the kept start point");
-    k_rescale<double><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(m, c->keep_z, S.z, c->keep_sx, p.SCALE_X);
-    k_rescale<double><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(m, c->keep_z, S.zbar, c->keep_sx, p.SCALE_X);
-  } else if (hipMemcpy(S.y, y0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
-             hipMemcpy(S.z, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
-             hipMemcpy(S.zbar, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess) {
-    return set_err(c, USBS_ERR_HIP, "upload of the start point");
   }
-  if (hipMemcpy(S.q0, q0h, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
-    return set_err(c, USBS_ERR_HIP, "upload of q0");
-  S.fill(S.ups, m, 0.0);
-  S.fill_T(S.Qa, ld * (S.k + 1));
-  S.fill_T(S.Qb, ld * (S.k + 1));
-  S.fill_T(S.PO, ld * 2 * (r > 0 ? r : 1));
-  if (warm) {
-    const long long cnt = ld * rk;
-    k_rescale<T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.P_(), c->keep_sx, p.SCALE_X);
-    k_rescale<T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.Pbar, c->keep_sx, p.SCALE_X);
-  }
-  if (r > 0) {
-    double* d_om = up(Omega, (size_t)n * r);
-    if (!d_om) return USBS_ERR_OOM;
-    k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_om, S.Om(), ld);
+
+  int prepare(Ctx* cc, const usbs_params& pp, const double* y0, const double* z0, double trX0,
+              double pobj0, const double* Omega, const double* P0, const double* q0h,
+              usbs_record* lg, int cap) {
+    c = cc;
+    p = pp;
+    log = lg;
+    log_cap = cap;
+    int rc = S.setup(c, p);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    const int n = S.n, m = S.m, r = S.r;
+    const long long ld = S.ld;
+    const int rk = r > 0 ? r : 1;  // columns of P (the diag-only P is one column)
+    const bool warm = p.warm != 0;
     if (warm) {
-    } else if (P0) {
-      double* d_p = up(P0, (size_t)n * r);
-      if (!d_p) return USBS_ERR_OOM;
-      k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_p, S.P_(), ld);
-      k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_p, S.Pbar, ld);
-    } else {
-      S.fill_T(S.Pbar, ld * r);
+      if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld)
+        return set_err(c, USBS_ERR_STATE, "warm start needs a previous solve of the same size on this context");
+      // y carries over; z and P are taken from the kept solve's SCALE_X to this one's
+      if (hipMemcpyAsync(S.y, c->keep_y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return set_err(c, USBS_ERR_HIP, "copy of the kept start point");
+      k_rescale<double><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(m, c->keep_z, S.z, c->keep_sx, p.SCALE_X);
+      k_rescale<double><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(m, c->keep_z, S.zbar, c->keep_sx, p.SCALE_X);
+    } else if (hipMemcpy(S.y, y0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(S.z, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(S.zbar, z0, sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess) {
+      return set_err(c, USBS_ERR_HIP, "upload of the start point");
     }
-    if (launch_tsgram<T, T>(n, c->P.tile, c->P.ntiles, S.Om(), ld, r, S.Om(), ld, r, S.partial, S.OtO, s))
-      return set_err(c, USBS_ERR_HIP, "Omega Gram");
-  } else if (S.dense) {
-    // dense X: P0 carries X0 (n x n row-major); X and Xbar start from it
-    S.fill_T(S.Pbar, ld);
-    const long long nn = (long long)n * n;
-    if (P0) {
-      double* d_x = up(P0, (size_t)nn);
-      if (!d_x) return USBS_ERR_OOM;
-      k_load_colmajor<double><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(n, n, d_x, S.X, ld);
-      k_load_colmajor<double><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(n, n, d_x, S.Xbar, ld);
-    } else {
-      S.fill(S.X, ld * n, 0.0);
-      S.fill(S.Xbar, ld * n, 0.0);
-    }
-  } else {
+    if (hipMemcpy(S.q0, q0h, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "upload of q0");
+    S.fill(S.ups, m, 0.0);
+    S.fill_T(S.Qa, ld * (S.k + 1));
+    S.fill_T(S.Qb, ld * (S.k + 1));
+    S.fill_T(S.PO, ld * 2 * (r > 0 ? r : 1));
     if (warm) {
-    } else if (P0) {
-      double* d_p = up(P0, (size_t)n);
-      if (!d_p) return USBS_ERR_OOM;
-      k_load_colmajor<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, 1, d_p, S.P_(), ld);
-      k_load_colmajor<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, 1, d_p, S.Pbar, ld);
-    } else {
+      const long long cnt = ld * rk;
+      k_rescale<T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.P_(), c->keep_sx, p.SCALE_X);
+      k_rescale<T><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, c->keep_P, S.Pbar, c->keep_sx, p.SCALE_X);
+    }
+    if (r > 0) {
+      double* d_om = up(Omega, (size_t)n * r);
+      if (!d_om) return USBS_ERR_OOM;
+      k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_om, S.Om(), ld);
+      if (warm) {
+      } else if (P0) {
+        double* d_p = up(P0, (size_t)n * r);
+        if (!d_p) return USBS_ERR_OOM;
+        k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_p, S.P_(), ld);
+        k_load_colmajor<T><<<(unsigned)(((long long)n * r + 255) / 256), 256, 0, s>>>(n, r, d_p, S.Pbar, ld);
+      } else {
+        S.fill_T(S.Pbar, ld * r);
+      }
+      if (launch_tsgram<T, T>(n, c->P.tile, c->P.ntiles, S.Om(), ld, r, S.Om(), ld, r, S.partial, S.OtO, s))
+        return set_err(c, USBS_ERR_HIP, "Omega Gram");
+    } else if (S.dense) {
+      // dense X: P0 carries X0 (n x n row-major); X and Xbar start from it
       S.fill_T(S.Pbar, ld);
+      const long long nn = (long long)n * n;
+      if (P0) {
+        double* d_x = up(P0, (size_t)nn);
+        if (!d_x) return USBS_ERR_OOM;
+        k_load_colmajor<double><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(n, n, d_x, S.X, ld);
+        k_load_colmajor<double><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(n, n, d_x, S.Xbar, ld);
+      } else {
+        S.fill(S.X, ld * n, 0.0);
+        S.fill(S.Xbar, ld * n, 0.0);
+      }
+    } else {
+      if (warm) {
+      } else if (P0) {
+        double* d_p = up(P0, (size_t)n);
+        if (!d_p) return USBS_ERR_OOM;
+        k_load_colmajor<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, 1, d_p, S.P_(), ld);
+        k_load_colmajor<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, 1, d_p, S.Pbar, ld);
+      } else {
+        S.fill_T(S.Pbar, ld);
+      }
     }
-  }
-  // scalar block
-  std::vector<double> h(SC_COUNT, 0.0);
-  h[SC_TRX] = trX0;
-  h[SC_TRXB] = trX0;
-  h[SC_P] = pobj0;
-  h[SC_PB] = pobj0;
-  h[SC_OBJUB] = INFINITY;
-  h[SC_GAP] = 1.1 * p.obj_gap_eps;
-  h[SC_INFGAP] = 1.1 * p.infeas_gap_eps;
-  h[SC_MAXINF] = 1.1 * p.max_infeas_eps;
-  h[SC_CB] = NAN;
-  {
-    std::vector<double> bh(m), sah(m);
-    if (hipMemcpy(bh.data(), c->P.b, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(sah.data(), c->P.scale_a, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)
-      return set_err(c, USBS_ERR_HIP, "reading b");
-    double nb2 = 0.0;
-    for (int i = 0; i < m; ++i) {
-      const double v = (bh[i] / sah[i]) / p.SCALE_X;
-      nb2 += v * v;
+    // scalar block
+    std::vector<double> h(SC_COUNT, 0.0);
+    h[SC_TRX] = trX0;
+    h[SC_TRXB] = trX0;
+    h[SC_P] = pobj0;
+    h[SC_PB] = pobj0;
+    h[SC_OBJUB] = INFINITY;
+    h[SC_GAP] = 1.1 * p.obj_gap_eps;
+    h[SC_INFGAP] = 1.1 * p.infeas_gap_eps;
+    h[SC_MAXINF] = 1.1 * p.max_infeas_eps;
+    h[SC_CB] = NAN;
+    {
+      std::vector<double> bh(m), sah(m);
+      if (hipMemcpy(bh.data(), c->P.b, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(sah.data(), c->P.scale_a, sizeof(double) * m, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_err(c, USBS_ERR_HIP, "reading b");
+      double nb2 = 0.0;
+      for (int i = 0; i < m; ++i) {
+        const double v = (bh[i] / sah[i]) / p.SCALE_X;
+        nb2 += v * v;
+      }
+      h[SC_BNORM] = std::sqrt(nb2);
     }
-    h[SC_BNORM] = std::sqrt(nb2);
+    if (hipMemcpy(S.sc, h.data(), sizeof(double) * SC_COUNT, hipMemcpyHostToDevice) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "upload of the scalar block");
+
+    t_origin =
+        std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    return USBS_OK;
   }
-  if (hipMemcpy(S.sc, h.data(), sizeof(double) * SC_COUNT, hipMemcpyHostToDevice) != hipSuccess)
-    return set_err(c, USBS_ERR_HIP, "upload of the scalar block");
 
-  const double t_origin =
-      std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-  // initial bundle: k smallest eigenpairs of -C + A*(y0) (solver/usbs.py:693-706)
-  if (launch_assemble(c->P, -1.0, S.y, 1.0, nullptr, 0.0, nullptr, 0.0, S.op64,
-                      sizeof(T) == 4 ? S.op32 : nullptr, s))
-    return set_err(c, USBS_ERR_HIP, "assemble");
-  rc = S.lanczos(S.k);
-  if (rc) return rc;
-  if (hipMemcpyAsync(S.Qa + ld, S.lz.Q, sizeof(T) * ld * S.k, hipMemcpyDeviceToDevice, s) != hipSuccess)
-    return set_err(c, USBS_ERR_HIP, "copy of the initial bundle");
-  if (launch_dots3(c->P, S.y, c->P.b, nullptr, S.partial, S.tmp4, s)) return set_err(c, USBS_ERR_HIP, "dots");
-  k_init_pen<<<1, 64, 0, s>>>(S.sc, S.tmp4, S.lz.alpha, p.trace_ub);
+  // the operator of the initial Lanczos call, -C + A*(y0)
+  int assemble_initial() {
+    hipStream_t s = c->stream;
+    // initial bundle: k smallest eigenpairs of -C + A*(y0) (solver/usbs.py:693-706)
+    if (launch_assemble(c->P, -1.0, S.y, 1.0, nullptr, 0.0, nullptr, 0.0, S.op64,
+                        sizeof(T) == 4 ? S.op32 : nullptr, s))
+      return set_err(c, USBS_ERR_HIP, "assemble");
+    return USBS_OK;
+  }
 
-  int nlog = 0;
-  auto cond = [&](bool& go) -> int {
+  // after the initial Lanczos call: the bundle and the penalised dual
+  int after_initial() {
+    hipStream_t s = c->stream;
+    const long long ld = S.ld;
+    if (hipMemcpyAsync(S.Qa + ld, S.lz.Q, sizeof(T) * ld * S.k, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "copy of the initial bundle");
+    if (launch_dots3(c->P, S.y, c->P.b, nullptr, S.partial, S.tmp4, s)) return set_err(c, USBS_ERR_HIP, "dots");
+    k_init_pen<<<1, 64, 0, s>>>(S.sc, S.tmp4, S.lz.alpha, p.trace_ub);
+
+    return USBS_OK;
+  }
+
+  // condition check (solver/usbs.py:503-540): rounding, record, stop test
+  int cond() {
+    hipStream_t s = c->stream;
     int rc2 = S.rounding();
     if (rc2) return rc2;
     if (launch_dots3(c->P, S.y, c->P.b, nullptr, S.partial, S.tmp4, s)) return set_err(c, USBS_ERR_HIP, "dots");
@@ -667,88 +726,275 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
          (rec.time < p.max_time && (rec.obj_gap > p.obj_gap_eps || rec.infeas_gap > p.infeas_gap_eps ||
                                     rec.max_infeas > p.max_infeas_eps));
     return USBS_OK;
-  };
-  bool go = true;
-  rc = cond(go);
+  }
+
+  int finish(double* y_out, double* z_out, double* P_out, double* scal_out, int* log_len) {
+    hipStream_t s = c->stream;
+    const int n = S.n, m = S.m, r = S.r;
+    const long long ld = S.ld;
+    const int rk = r > 0 ? r : 1;
+    if (log_len) *log_len = nlog;
+    // outputs
+    // P (n x r, or n for diag-only), or X (n x n) in the dense mode
+    const int rr = S.dense ? n : (r > 0 ? r : 1);
+    double* d_out = (double*)ctx_alloc(c, tmp, sizeof(double) * (size_t)n * rr);
+    if (!d_out) return USBS_ERR_OOM;
+    if (S.dense)
+      k_store_rowmajor<double><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.X, ld, d_out);
+    else
+      k_store_rowmajor<T><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.P_(), ld, d_out);
+    std::vector<double> hs(SC_COUNT);
+    if ((y_out && hipMemcpyAsync(y_out, S.y, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        (z_out && hipMemcpyAsync(z_out, S.z, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        (P_out && hipMemcpyAsync(P_out, d_out, sizeof(double) * (size_t)n * rr, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipMemcpyAsync(hs.data(), S.sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "reading results");
+    // keep y, z, P on the device for a warm start of the next solve (not
+    // in the dense mode, which has no sketch to keep)
+    if (S.dense) {
+      ctx_free_pool(c->keep_bufs);
+      c->keep_y = c->keep_z = c->keep_P = nullptr;
+      c->keep_n = c->keep_m = 0;
+      c->keep_r = -1;
+    } else if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld) {
+      ctx_free_pool(c->keep_bufs);
+      c->keep_y = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
+      c->keep_z = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
+      c->keep_P = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * ld * rk);
+      if (!c->keep_y || !c->keep_z || !c->keep_P) {
+        ctx_free_pool(c->keep_bufs);
+        c->keep_n = c->keep_m = 0;
+        c->keep_r = -1;
+        return USBS_ERR_OOM;
+      }
+      c->keep_n = n;
+      c->keep_m = m;
+      c->keep_r = rk;
+      c->keep_ld = ld;
+    }
+    c->keep_sx = p.SCALE_X;
+    if (!S.dense) {
+      const long long cnt = ld * rk;
+      k_cast<T, double><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, S.P_(), c->keep_P);
+      if (hipMemcpyAsync(c->keep_y, S.y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(c->keep_z, S.z, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return set_err(c, USBS_ERR_HIP, "keeping the final state");
+    }
+    if (scal_out) {
+      scal_out[0] = hs[SC_TRX];
+      scal_out[1] = hs[SC_P];
+      scal_out[2] = (double)iters;
+      scal_out[3] = hs[SC_GAP];
+      scal_out[4] = hs[SC_INFGAP];
+      scal_out[5] = hs[SC_MAXINF];
+    }
+    return USBS_OK;
+  }
+};
+
+template <typename T>
+static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const double* z0, double trX0,
+                     double pobj0, const double* Omega, const double* P0, const double* q0h,
+                     double* y_out, double* z_out, double* P_out, double* scal_out,
+                     usbs_record* log, int log_cap, int* log_len) {
+  SolveRun<T> R;
+  int rc = R.prepare(c, p, y0, z0, trX0, pobj0, Omega, P0, q0h, log, log_cap);
   if (rc) return rc;
-  long long iters = 0;
+  if ((rc = R.assemble_initial())) return rc;
+  if ((rc = R.S.lanczos(R.S.k))) return rc;
+  if ((rc = R.after_initial())) return rc;
+  if ((rc = R.cond())) return rc;
+  Solver<T>& S = R.S;
   if (p.cond_exp_base > 1.0) {
     for (long long steps : geometric_schedule(p.max_iters, p.cond_exp_base)) {
-      if (!go) break;
+      if (!R.go) break;
       for (long long i = 0; i < steps; ++i) {
         rc = S.body();
         if (rc) return rc;
-        ++iters;
+        ++R.iters;
       }
-      rc = cond(go);
+      rc = R.cond();
       if (rc) return rc;
     }
   } else {
-    for (long long it = 0; it < p.max_iters && go; ++it) {
+    for (long long it = 0; it < p.max_iters && R.go; ++it) {
       rc = S.body();
       if (rc) return rc;
-      ++iters;
-      rc = cond(go);
+      ++R.iters;
+      rc = R.cond();
       if (rc) return rc;
     }
   }
-  if (log_len) *log_len = nlog;
-  // outputs
-  // P (n x r, or n for diag-only), or X (n x n) in the dense mode
-  const int rr = S.dense ? n : (r > 0 ? r : 1);
-  double* d_out = (double*)ctx_alloc(c, tmp, sizeof(double) * (size_t)n * rr);
-  if (!d_out) return USBS_ERR_OOM;
-  if (S.dense)
-    k_store_rowmajor<double><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.X, ld, d_out);
-  else
-    k_store_rowmajor<T><<<(unsigned)(((long long)n * rr + 255) / 256), 256, 0, s>>>(n, rr, S.P_(), ld, d_out);
-  std::vector<double> hs(SC_COUNT);
-  if ((y_out && hipMemcpyAsync(y_out, S.y, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-      (z_out && hipMemcpyAsync(z_out, S.z, sizeof(double) * m, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-      (P_out && hipMemcpyAsync(P_out, d_out, sizeof(double) * (size_t)n * rr, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-      hipMemcpyAsync(hs.data(), S.sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return set_err(c, USBS_ERR_HIP, "reading results");
-  // keep y, z, P on the device for a warm start of the next solve (not
-  // in the dense mode, which has no sketch to keep)
-  if (S.dense) {
-    ctx_free_pool(c->keep_bufs);
-    c->keep_y = c->keep_z = c->keep_P = nullptr;
-    c->keep_n = c->keep_m = 0;
-    c->keep_r = -1;
-  } else if (c->keep_n != n || c->keep_m != m || c->keep_r != rk || c->keep_ld != ld) {
-    ctx_free_pool(c->keep_bufs);
-    c->keep_y = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
-    c->keep_z = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * m);
-    c->keep_P = (double*)ctx_alloc(c, c->keep_bufs, sizeof(double) * ld * rk);
-    if (!c->keep_y || !c->keep_z || !c->keep_P) {
-      ctx_free_pool(c->keep_bufs);
-      c->keep_n = c->keep_m = 0;
-      c->keep_r = -1;
-      return USBS_ERR_OOM;
+  return R.finish(y_out, z_out, P_out, scal_out, log_len);
+}
+
+// The condition-check schedule of one solve: entry i is the number of
+// iterations before the i-th check after the initial one -- 1 each without a
+// geometric base, else utils/loop.py:85-98's steps.
+struct CondSchedule {
+  std::vector<long long> steps;
+  long long count = 0;
+  bool unit = true;
+  explicit CondSchedule(const usbs_params& p) {
+    unit = !(p.cond_exp_base > 1.0);
+    if (unit) {
+      count = p.max_iters;
+    } else {
+      steps = geometric_schedule(p.max_iters, p.cond_exp_base);
+      count = (long long)steps.size();
     }
-    c->keep_n = n;
-    c->keep_m = m;
-    c->keep_r = rk;
-    c->keep_ld = ld;
   }
-  c->keep_sx = p.SCALE_X;
-  if (!S.dense) {
-    const long long cnt = ld * rk;
-    k_cast<T, double><<<(unsigned)((cnt + 255) / 256), 256, 0, s>>>(cnt, S.P_(), c->keep_P);
-    if (hipMemcpyAsync(c->keep_y, S.y, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(c->keep_z, S.z, sizeof(double) * m, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-      return set_err(c, USBS_ERR_HIP, "keeping the final state");
+  long long at(long long i) const { return unit ? 1 : steps[(size_t)i]; }
+};
+
+struct StreamEvent {
+  hipEvent_t e = nullptr;
+  ~StreamEvent() { if (e) (void)hipEventDestroy(e); }
+};
+
+// B solves in lockstep on one device.  Each runs the statements of run_solve
+// on its own context and stream; per iteration every active solve enqueues its
+// work up to the candidate Lanczos call, then one persistent cooperative
+// launch per restart advances all their Lanczos calls together on a batch
+// stream (the same step, Ritz and rotation kernels as a single persistent
+// call, whose results do not depend on the workgroup plan), and each solve
+// continues on its own stream.  A solve whose stop test holds, or whose
+// schedule ends, leaves the batch; the launch plan is rebuilt for the rest.
+template <typename T>
+static int run_batch(const std::vector<Ctx*>& cs, const std::vector<usbs_params>& ps,
+                     const usbs_batch_item* items, int& failed) {
+  const int B = (int)cs.size();
+  Ctx* c0 = cs[0];
+  std::vector<std::unique_ptr<SolveRun<T>>> R(B);
+  int rc = USBS_OK;
+  for (int q = 0; q < B; ++q) {
+    failed = q;
+    const usbs_batch_item& it = items[q];
+    R[q].reset(new SolveRun<T>());
+    rc = R[q]->prepare(cs[q], ps[q], it.y0, it.z0, it.tr_X0, it.primal_obj0, it.Omega, it.P0,
+                       it.q0, it.log, it.log_cap);
+    if (rc) return rc;
+    if ((rc = R[q]->assemble_initial())) return rc;
   }
-  if (scal_out) {
-    scal_out[0] = hs[SC_TRX];
-    scal_out[1] = hs[SC_P];
-    scal_out[2] = (double)iters;
-    scal_out[3] = hs[SC_GAP];
-    scal_out[4] = hs[SC_INFGAP];
-    scal_out[5] = hs[SC_MAXINF];
+  failed = -1;
+  hipStream_t bs = nullptr;
+  if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess)
+    return set_err(c0, USBS_ERR_HIP, "batch stream creation failed");
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+  } bs_guard{bs};
+  std::vector<StreamEvent> ready(B);
+  StreamEvent lz_done;
+  for (auto& e : ready)
+    if (hipEventCreateWithFlags(&e.e, hipEventDisableTiming) != hipSuccess)
+      return set_err(c0, USBS_ERR_HIP, "event creation failed");
+  if (hipEventCreateWithFlags(&lz_done.e, hipEventDisableTiming) != hipSuccess)
+    return set_err(c0, USBS_ERR_HIP, "event creation failed");
+  const int m = ps[0].lanczos_inner_iterations, max_restarts = ps[0].lanczos_max_restarts;
+
+  // one batched Lanczos call over the solves in `act` (descriptors in lb)
+  auto batched_lanczos = [&](const std::vector<int>& act, LzBatch<T>& lb, int nd) -> int {
+    int hint = 1;
+    for (int q : act) {
+      Solver<T>& S = R[q]->S;
+      failed = q;
+      if (S.lanczos_start(nd)) return USBS_ERR_HIP;
+      if (hipEventRecord(ready[q].e, S.s) != hipSuccess || hipStreamWaitEvent(bs, ready[q].e, 0) != hipSuccess)
+        return set_err(cs[q], USBS_ERR_HIP, "batch fork");
+      hint = std::max(hint, S.lz_hint);
+    }
+    failed = 0;
+    const int e = lb.run(bs, max_restarts, hint);
+    if (e) return set_err(c0, USBS_ERR_HIP, std::string("batched Lanczos: ") + hipGetErrorString((hipError_t)e));
+    if (hipEventRecord(lz_done.e, bs) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "batch join");
+    for (int q : act)
+      if (hipStreamWaitEvent(R[q]->S.s, lz_done.e, 0) != hipSuccess) {
+        failed = q;
+        return set_err(cs[q], USBS_ERR_HIP, "batch join");
+      }
+    failed = -1;
+    return USBS_OK;
+  };
+  auto plan = [&](const std::vector<int>& act, LzBatch<T>& lb, int nd) -> int {
+    if (hipStreamSynchronize(bs) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "synchronise");
+    std::vector<LzInst<T>> L;
+    for (int q : act) {
+      L.push_back(R[q]->S.lz);
+      L.back().nd = nd;
+    }
+    failed = 0;
+    const int e = lb.build(c0, L, m);
+    if (!e) failed = -1;
+    return e;
+  };
+
+  // initial bundles (nd = k), then the first condition check
+  std::vector<int> act(B);
+  for (int q = 0; q < B; ++q) act[q] = q;
+  {
+    LzBatch<T> lb_init;
+    if ((rc = plan(act, lb_init, R[0]->S.k))) return rc;
+    if ((rc = batched_lanczos(act, lb_init, R[0]->S.k))) return rc;
+    for (int q = 0; q < B; ++q) {
+      failed = q;
+      if ((rc = R[q]->after_initial()) || (rc = R[q]->cond())) return rc;
+    }
+    if (hipStreamSynchronize(bs) != hipSuccess) return set_err(c0, USBS_ERR_HIP, "synchronise");
   }
+  // per solve: schedule position, iterations left before its next check
+  std::vector<CondSchedule> sched;
+  std::vector<long long> pos(B, 0), left(B, 0);
+  std::vector<char> on(B, 0);
+  auto checks = [&](int q) -> int {  // run the checks that are due; leave on[q] set while the solve goes on
+    while (on[q] && left[q] == 0) {
+      failed = q;
+      int e = R[q]->cond();
+      if (e) return e;
+      ++pos[q];
+      on[q] = R[q]->go && pos[q] < sched[q].count;
+      if (on[q]) left[q] = sched[q].at(pos[q]);
+    }
+    return USBS_OK;
+  };
+  for (int q = 0; q < B; ++q) {
+    sched.emplace_back(ps[q]);
+    on[q] = R[q]->go && sched[q].count > 0;
+    if (on[q]) left[q] = sched[q].at(0);
+    if ((rc = checks(q))) return rc;
+  }
+  LzBatch<T> lb;
+  std::vector<int> planned;
+  while (true) {
+    act.clear();
+    for (int q = 0; q < B; ++q)
+      if (on[q]) act.push_back(q);
+    if (act.empty()) break;
+    if (act != planned) {
+      if ((rc = plan(act, lb, R[act[0]]->S.kc))) return rc;
+      planned = act;
+    }
+    for (int q : act) {
+      failed = q;
+      if ((rc = R[q]->S.body_pre())) return rc;
+    }
+    if ((rc = batched_lanczos(act, lb, R[act[0]]->S.kc))) return rc;
+    for (int q : act) {
+      failed = q;
+      if ((rc = R[q]->S.body_post())) return rc;
+      ++R[q]->iters;
+      --left[q];
+      if ((rc = checks(q))) return rc;
+    }
+  }
+  for (int q = 0; q < B; ++q) {
+    failed = q;
+    const usbs_batch_item& it = items[q];
+    if ((rc = R[q]->finish(it.y_out, it.z_out, it.P_out, it.scal_out, it.log_len))) return rc;
+  }
+  failed = -1;
   return USBS_OK;
 }
 
@@ -781,19 +1027,10 @@ extern "C" int usbs_get_stats(const usbs_ctx* h, usbs_stats* out) {
   return USBS_OK;
 }
 
-extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0, const double* z0,
-                          double tr_X0, double primal_obj0, const double* Omega, const double* P0,
-                          const double* q0, double* y_out, double* z_out, double* P_out,
-                          double* scal_out, usbs_record* log, int log_cap, int* log_len) {
-  if (!h || !prm) return USBS_ERR_INVALID;
-  Ctx* c = &h->c;
+// The argument checks of usbs_solve (and of each usbs_solve_batch item).
+static int check_solve_args(Ctx* c, const usbs_params& p, const double* y0, const double* z0,
+                            const double* Omega, const double* q0) {
   if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
-  usbs_params p = *prm;
-  if (p.exec_mode == USBS_EXEC_AUTO)
-    p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision, c->num_cus);
-  if (p.exec_mode == USBS_EXEC_TUNE && c->tuned_mode >= 0 &&
-      c->tuned_m == p.lanczos_inner_iterations && c->tuned_prec == p.precision)
-    p.exec_mode = c->tuned_mode;
   if (!q0 || (!p.warm && (!y0 || !z0))) return set_err(c, USBS_ERR_INVALID, "null y0 / z0 / q0");
   const int k = p.k_curr + p.k_past;
   if (p.k_curr < 1 || p.k_past < 0 || k > 11)
@@ -826,6 +1063,24 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
   if (!p.warm)
     for (int i = 0; i < c->P.m; ++i)
       if (!std::isfinite(y0[i]) || !std::isfinite(z0[i])) return set_err(c, USBS_ERR_NONFINITE, "non-finite y0/z0");
+  return USBS_OK;
+}
+
+extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0, const double* z0,
+                          double tr_X0, double primal_obj0, const double* Omega, const double* P0,
+                          const double* q0, double* y_out, double* z_out, double* P_out,
+                          double* scal_out, usbs_record* log, int log_cap, int* log_len) {
+  if (!h || !prm) return USBS_ERR_INVALID;
+  Ctx* c = &h->c;
+  if (!c->has_problem) return set_err(c, USBS_ERR_STATE, "no problem uploaded");
+  usbs_params p = *prm;
+  if (p.exec_mode == USBS_EXEC_AUTO)
+    p.exec_mode = resolve_exec_mode(c->P, p.lanczos_inner_iterations, p.precision, c->num_cus);
+  if (p.exec_mode == USBS_EXEC_TUNE && c->tuned_mode >= 0 &&
+      c->tuned_m == p.lanczos_inner_iterations && c->tuned_prec == p.precision)
+    p.exec_mode = c->tuned_mode;
+  int rc = check_solve_args(c, p, y0, z0, Omega, q0);
+  if (rc) return rc;
   const bool coop_form = p.exec_mode == USBS_EXEC_PERSISTENT || p.exec_mode == USBS_EXEC_TUNE;
   CoopGuard coop(coop_form ? (const void*)c : nullptr);
   if (coop_form && !coop.ok()) return set_err(c, USBS_ERR_STATE, COOP_BUSY_MSG);
@@ -835,4 +1090,50 @@ extern "C" int usbs_solve(usbs_ctx* h, const usbs_params* prm, const double* y0,
                              scal_out, log, log_cap, log_len);
   return run_solve<float>(c, p, y0, z0, tr_X0, primal_obj0, Omega, P0, q0, y_out, z_out, P_out,
                           scal_out, log, log_cap, log_len);
+}
+
+extern "C" int usbs_solve_batch(const usbs_batch_item* items, int32_t count) {
+  if (!items || count < 1) return USBS_ERR_INVALID;
+  std::vector<Ctx*> cs(count);
+  std::vector<usbs_params> ps(count);
+  for (int q = 0; q < count; ++q) {
+    if (!items[q].ctx || !items[q].params) return USBS_ERR_INVALID;
+    cs[q] = &items[q].ctx->c;
+  }
+  Ctx* c0 = cs[0];
+  auto fail_all = [&](int rc, const std::string& msg) {
+    for (Ctx* c : cs) set_err(c, rc, msg);
+    return rc;
+  };
+  for (int q = 0; q < count; ++q) {
+    Ctx* c = cs[q];
+    for (int j = 0; j < q; ++j)
+      if (cs[j] == c) return fail_all(USBS_ERR_INVALID, "a context appears twice in the batch");
+    if (c->device != c0->device) return fail_all(USBS_ERR_INVALID, "batch items on different devices");
+    usbs_params p = *items[q].params;
+    if (p.exec_mode != USBS_EXEC_AUTO && p.exec_mode != USBS_EXEC_PERSISTENT)
+      return fail_all(USBS_ERR_INVALID, "a batch runs the persistent form: exec_mode must be auto or persistent");
+    p.exec_mode = USBS_EXEC_PERSISTENT;
+    const usbs_params& p0 = *items[0].params;
+    if (p.precision != p0.precision || p.k_curr != p0.k_curr || p.k_past != p0.k_past ||
+        p.lanczos_inner_iterations != p0.lanczos_inner_iterations ||
+        p.lanczos_max_restarts != p0.lanczos_max_restarts)
+      return fail_all(USBS_ERR_INVALID, "batch items must share precision, k_curr, k_past, "
+                                        "lanczos_inner_iterations and lanczos_max_restarts");
+    const usbs_batch_item& it = items[q];
+    const int rc = check_solve_args(c, p, it.y0, it.z0, it.Omega, it.q0);
+    if (rc) return fail_all(rc, "batch item " + std::to_string(q) + ": " + c->err);
+    ps[q] = p;
+  }
+  CoopGuard coop(c0);
+  if (!coop.ok()) return fail_all(USBS_ERR_STATE, COOP_BUSY_MSG);
+  if (hipSetDevice(c0->device) != hipSuccess) return fail_all(USBS_ERR_HIP, "hipSetDevice");
+  int failed = -1;
+  const int rc = ps[0].precision == USBS_PREC_FP64 ? run_batch<double>(cs, ps, items, failed)
+                                                   : run_batch<float>(cs, ps, items, failed);
+  if (rc) {
+    const Ctx* src = failed >= 0 ? cs[failed] : c0;
+    return fail_all(rc, "batch item " + std::to_string(failed >= 0 ? failed : 0) + ": " + src->err);
+  }
+  return USBS_OK;
 }
