@@ -338,10 +338,13 @@ int LzBatch<T>::build(Ctx* c0, std::vector<LzInst<T>>& L, int m_) {
     cb = std::max(cb, b);
   }
   if (!all_cache) cb = 0;
-  int cap = lanczos_max_coresident_blocks(cb ? cb : lanczos_tile_lds_bytes(m), c0->device);
+  int rw = 0;
+  for (auto& x : L) rw |= (x.tile > 256) ? 1 : 0;
+  int cap = cb ? lanczos_max_coresident_blocks<T>(m, true, false, cb, c0->device)
+               : lanczos_max_coresident_blocks<T>(m, false, rw != 0, lanczos_tile_lds_bytes(m), c0->device);
   if (cb && cap < total_tiles) {
     cb = 0;
-    cap = lanczos_max_coresident_blocks(lanczos_tile_lds_bytes(m), c0->device);
+    cap = lanczos_max_coresident_blocks<T>(m, false, rw != 0, lanczos_tile_lds_bytes(m), c0->device);
   }
   if (cap < ninst) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
   total_wgs = 0;

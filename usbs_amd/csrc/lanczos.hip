@@ -1013,24 +1013,31 @@ int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int
 
 size_t lanczos_tile_lds_bytes(int m) { return tile_lds_bytes(m + 1); }
 
-int lanczos_max_coresident_blocks(size_t lds_bytes, int device) {
-  const void* kerns[] = {(const void*)k_lz_steps_coop<float, false, 49, false>,
-                         (const void*)k_lz_steps_coop<float, false, 49, true>,
-                         (const void*)k_lz_steps_coop<float, true, 49, false>,
-                         (const void*)k_lz_steps_coop<double, false, 49, false>,
-                         (const void*)k_lz_steps_coop<double, false, 49, true>,
-                         (const void*)k_lz_steps_coop<double, true, 49, false>};
-  int pc = 1 << 30;
-  for (const void* kfn : kerns) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, USBS_NT, lds_bytes) != hipSuccess)
-      return 0;
-    if (per_cu < pc) pc = per_cu;
-  }
+// The step kernel persistent_impl launches for (T, m, cache, rowwise).
+template <typename T>
+static const void* coop_kernel(int m, bool cache, bool rowwise) {
+#define USBS_COOP_K(MC)                                                          \
+  return cache ? (const void*)k_lz_steps_coop<T, true, MC, false>                \
+               : rowwise ? (const void*)k_lz_steps_coop<T, false, MC, true>      \
+                         : (const void*)k_lz_steps_coop<T, false, MC, false>
+  if (m + 1 <= 17) USBS_COOP_K(17);
+  if (m + 1 <= 33) USBS_COOP_K(33);
+  USBS_COOP_K(49);
+#undef USBS_COOP_K
+}
+
+template <typename T>
+int lanczos_max_coresident_blocks(int m, bool cache, bool rowwise, size_t lds_bytes, int device) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, coop_kernel<T>(m, cache, rowwise), USBS_NT,
+                                                   lds_bytes) != hipSuccess)
+    return 0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
-  return pc * prop.multiProcessorCount;
+  return per_cu * prop.multiProcessorCount;
 }
+template int lanczos_max_coresident_blocks<double>(int, bool, bool, size_t, int);
+template int lanczos_max_coresident_blocks<float>(int, bool, bool, size_t, int);
 
 template <typename T>
 int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cache_bytes) {
@@ -1038,10 +1045,12 @@ int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cach
   if (want_wgs > 0 && want_wgs < nb) nb = want_wgs;
   L.nblocks = nb;
   size_t cb = lanczos_cache_bytes(L);
-  int cores = lanczos_max_coresident_blocks(cb ? cb : tile_lds_bytes(L.m + 1), device);
+  const bool rw = L.tile > 256;
+  int cores = cb ? lanczos_max_coresident_blocks<T>(L.m, true, false, cb, device)
+                 : lanczos_max_coresident_blocks<T>(L.m, false, rw, tile_lds_bytes(L.m + 1), device);
   if (cb && cores < nb) {
     cb = 0;
-    cores = lanczos_max_coresident_blocks(tile_lds_bytes(L.m + 1), device);
+    cores = lanczos_max_coresident_blocks<T>(L.m, false, rw, tile_lds_bytes(L.m + 1), device);
   }
   if (cores < 1) return (int)hipErrorInvalidConfiguration;
   if (nb > cores) nb = cores;

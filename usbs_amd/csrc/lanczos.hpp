@@ -122,7 +122,10 @@ size_t lanczos_cache_bytes(const LzInst<T>& L);
 // case where it beats per-op launches): light rows, tiles of at most 256
 // rows, cache within LZ_CACHE_MAX_BYTES.
 int lanczos_prefers_persistent(int tile, int heavy, int m, int tile_nnz_max, int value_bytes);
-int lanczos_max_coresident_blocks(size_t lds_bytes, int device);
+// Workgroups of the persistent step kernel for (T, m, cache, rowwise) that
+// fit on the device at once with lds_bytes of LDS each.
+template <typename T>
+int lanczos_max_coresident_blocks(int m, bool cache, bool rowwise, size_t lds_bytes, int device);
 // LDS of a persistent workgroup that does not cache its tile (basis size m).
 size_t lanczos_tile_lds_bytes(int m);
 template <typename T>
