@@ -16,21 +16,56 @@ int set_err(Ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+void ctx_release_cache(Ctx* c) {
+  for (auto& kv : c->cache.blocks) (void)hipFree(kv.second);
+  c->cache.blocks.clear();
+  c->cache.bytes = 0;
+}
+
 void* ctx_alloc(Ctx* c, std::vector<DevBuf>& pool, size_t bytes) {
+  bytes = (bytes + 255) / 256 * 256;
+  if (bytes == 0) bytes = 256;
+  BufCache& bc = c->cache;
+  bc.stream = c->stream;
+  // the smallest cached block that fits without wasting more than half of it
+  auto it = bc.blocks.lower_bound(bytes);
+  if (it != bc.blocks.end() && it->first <= 2 * bytes) {
+    DevBuf b{it->second, it->first, &bc};
+    bc.bytes -= it->first;
+    bc.blocks.erase(it);
+    pool.push_back(b);
+    return b.p;
+  }
   void* p = nullptr;
-  if (bytes == 0) bytes = 8;
   if (hipMalloc(&p, bytes) != hipSuccess) {
     (void)hipGetLastError();
-    c->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed";
-    return nullptr;
+    ctx_release_cache(c);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      c->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed";
+      return nullptr;
+    }
   }
-  pool.push_back({p, bytes});
+  pool.push_back({p, bytes, &bc});
   return p;
 }
 
 void ctx_free_pool(std::vector<DevBuf>& pool) {
-  for (auto& b : pool)
-    if (b.p) (void)hipFree(b.p);
+  BufCache* synced = nullptr;
+  for (auto& b : pool) {
+    if (!b.p) continue;
+    if (!b.cache) {
+      (void)hipFree(b.p);
+      continue;
+    }
+    if (b.cache != synced) {
+      // the context's queued work may still use the block
+      (void)hipStreamSynchronize(b.cache->stream);
+      synced = b.cache;
+    }
+    b.cache->blocks.emplace(b.bytes, b.p);
+    b.cache->bytes += b.bytes;
+  }
   pool.clear();
 }
 
@@ -491,6 +526,7 @@ void usbs_destroy(usbs_ctx* h) {
   ctx_free_pool(h->c.work_bufs);
   ctx_free_pool(h->c.prob_bufs);
   ctx_free_pool(h->c.keep_bufs);
+  ctx_release_cache(&h->c);
   (void)hipStreamDestroy(h->c.stream);
   delete h;
 }

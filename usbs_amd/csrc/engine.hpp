@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/usbs_hip.h"
@@ -17,10 +18,26 @@
 
 namespace usbs {
 
-// A device buffer owned by a context.
+// Device buffers a context has finished with, kept for its later
+// allocations: a solve's ~45 workspaces then cost no hipMalloc / hipFree
+// (hipFree waits for the whole device, stalling every other context's work)
+// once the context has seen its sizes.  Blocks return here only after the
+// context's stream has drained; work a block saw on another stream (a
+// solver's side stream, a batch stream) is waited for by its owner before
+// the block is released.  Freed for good by usbs_destroy, or when a
+// hipMalloc fails.
+struct BufCache {
+  hipStream_t stream = nullptr;           // the owning context's stream
+  std::multimap<size_t, void*> blocks;    // size -> block
+  size_t bytes = 0;
+};
+
+// A device buffer owned by a context (cache: where it returns when freed;
+// null: hipFree).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  BufCache* cache = nullptr;
 };
 
 // The operator pattern merges C's pattern with the (i, j) pattern of the
@@ -82,6 +99,7 @@ struct Ctx {
   std::string err;
   bool has_problem = false;
   Problem P;
+  BufCache cache;                 // freed buffers for reuse (see BufCache)
   std::vector<DevBuf> prob_bufs;  // freed on new problem / destroy
   std::vector<DevBuf> work_bufs;  // solve workspaces
   int num_cus = 256;             // compute units of the device
@@ -114,6 +132,8 @@ int set_err(Ctx* c, int code, const std::string& msg);
 
 void* ctx_alloc(Ctx* c, std::vector<DevBuf>& pool, size_t bytes);
 void ctx_free_pool(std::vector<DevBuf>& pool);
+// hipFree every cached block of the context.
+void ctx_release_cache(Ctx* c);
 
 template <typename T>
 int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int nd,

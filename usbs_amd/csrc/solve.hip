@@ -879,6 +879,9 @@ static int run_batch(const std::vector<Ctx*>& cs, const std::vector<usbs_params>
     if ((rc = R[q]->assemble_initial())) return rc;
   }
   failed = -1;
+  // launch plans (initial, loop): declared before the batch stream's guard so
+  // that the stream drains before their descriptors are released
+  LzBatch<T> lb_init, lb;
   hipStream_t bs = nullptr;
   if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess)
     return set_err(c0, USBS_ERR_HIP, "batch stream creation failed");
@@ -935,7 +938,6 @@ static int run_batch(const std::vector<Ctx*>& cs, const std::vector<usbs_params>
   std::vector<int> act(B);
   for (int q = 0; q < B; ++q) act[q] = q;
   {
-    LzBatch<T> lb_init;
     if ((rc = plan(act, lb_init, R[0]->S.k))) return rc;
     if ((rc = batched_lanczos(act, lb_init, R[0]->S.k))) return rc;
     for (int q = 0; q < B; ++q) {
@@ -965,7 +967,6 @@ static int run_batch(const std::vector<Ctx*>& cs, const std::vector<usbs_params>
     if (on[q]) left[q] = sched[q].at(0);
     if ((rc = checks(q))) return rc;
   }
-  LzBatch<T> lb;
   std::vector<int> planned;
   while (true) {
     act.clear();
