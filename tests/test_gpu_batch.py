@@ -90,3 +90,31 @@ def test_batch_rejects_mismatched_items():
     finally:
         for ctx in ctxs:
             ctx.close()
+
+
+def test_batch_torus_scale_equals_single():
+    """Four torus 100 x 200 solves (the benchmark's size: 79 row tiles each,
+    all workgroups of the batched launch co-resident), 150 iterations in
+    lockstep: every record and result equals the single persistent solve
+    bitwise.  The step kernel's device-wide barrier had a rare coherence race
+    (another wavefront's stores missing the L2 writeback, or its loads beating
+    the invalidation) that made such runs differ now and then."""
+    graphs = [problems.torus(100, 200, seed=s) for s in range(4)]
+    ctxs = [_lib.Context(0) for _ in graphs]
+    try:
+        print("case: batch of 4 torus 100x200, 150 iterations, mixed", flush=True)
+        calls = [_kw(W, ctx, 20 + i, 150, "mixed", rho=3.0, r=10)
+                 for i, (W, ctx) in enumerate(zip(graphs, ctxs))]
+        logs_b = [[] for _ in calls]
+        for kw, lg in zip(calls, logs_b):
+            kw["log"] = lg
+        out_b = usbs_batch(calls)
+        for i, (W, ctx) in enumerate(zip(graphs, ctxs)):
+            log_s = []
+            out_s = usbs(**_kw(W, ctx, 20 + i, 150, "mixed", rho=3.0, r=10), exec_mode="persistent",
+                         log=log_s)
+            _same(out_b[i], out_s)
+            _same_logs(logs_b[i], log_s)
+    finally:
+        for ctx in ctxs:
+            ctx.close()

@@ -146,16 +146,23 @@ __device__ __forceinline__ double counter_normal(uint64_t key, uint64_t idx) {
 // Device-wide barrier among `nblocks` workgroups that share `counter`
 // (monotonic, zeroed before the launch).  The caller's workgroups must all be
 // resident (cooperative launch).  `target` is kept by the caller and grows by
-// nblocks each call.  One release fence before the arrival and one acquire
-// fence after the wait: the L2 of an XCD is not coherent with the others, so
-// each agent-scope fence writes back / invalidates it, and the polling loop
-// itself uses relaxed loads so that waiting does not flush the cache that the
-// XCD's other workgroups are still working from.  A bounded wait: after
-// `spin_limit` polls the barrier records an error in *err and every later
-// barrier returns at once, so a wait that can never complete ends instead of
-// hanging the device.
+// nblocks each call.  The L2 of an XCD is not coherent with the others, so
+// thread 0 writes it back (agent-scope release fence) before arriving and
+// invalidates it with the compute unit's L1 (acquire fence) after the wait.
+// __syncthreads() orders only LDS operations, so two waits make those fences
+// cover the whole workgroup: every wavefront waits for its own global stores
+// to reach the L2 before the workgroup barrier in front of the writeback, and
+// thread 0 waits for the invalidation to complete before the barrier that
+// releases the other wavefronts' loads.  Without them another wavefront's
+// stores could miss the writeback, or its first loads be served stale; both
+// happened, rarely, as results that differed between runs.  The polling loop
+// uses relaxed loads so that waiting does not flush the cache the XCD's other
+// workgroups are working from.  A bounded wait: after `spin_limit` polls the
+// barrier records an error in *err and every later barrier returns at once,
+// so a wait that can never complete ends instead of hanging the device.
 __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned nblocks, unsigned& target,
                                              int* err, unsigned long long spin_limit) {
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   target += nblocks;
   __shared__ int s_ok;
@@ -181,6 +188,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned nblocks
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __builtin_amdgcn_s_waitcnt(0);
     }
     s_ok = ok;
   }
