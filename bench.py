@@ -440,8 +440,10 @@ def gap_leg(ctx, prob, precision, exec_mode, target, max_iters, max_time):
 
 def gap_batch_leg(B, target, max_iters, max_time):
     """B torus problems (seeds 0..B-1) to `target`: the mixed form with every
-    solve in flight together (a context and host thread each) against plain
-    FP64 per-op solving them one after another."""
+    solve in flight together (a context and host thread each), the mixed
+    form in lockstep (usbs_solve_batch: one persistent Lanczos launch per
+    restart for all solves), and plain FP64 per-op solving them one after
+    another."""
     import threading
     from usbs_amd import _lib
     from usbs_amd.usbs import upload_problem
@@ -475,6 +477,8 @@ def gap_batch_leg(B, target, max_iters, max_time):
             raise errs[0]
         wall_b = max(r[1] for r in res) - min(r[0] for r in res)
         progress("batched leg: %d mixed solves in flight, %.2f s" % (B, wall_b))
+        lock = lockstep_leg(ctxs, probs, target, max_iters, max_time)
+        progress("batched leg: %d mixed solves in lockstep, %.2f s" % (B, lock["wall_s"]))
         seq = []
         for b in range(B):
             el, log, _ = run_gpu(ctxs[b], probs[b], max_iters, "fp64", "perop", rho=GAP_RHO,
@@ -489,11 +493,44 @@ def gap_batch_leg(B, target, max_iters, max_time):
             "workload": "torus 100x200, w in {-1,+1}, seeds 0..%d" % (B - 1),
             "mixed_in_flight": {"wall_s": round(wall_b, 3), "iterations": [r[2] for r in res],
                                 "all_reached": all(r[3] for r in res), "exec_mode": "perop"},
+            "mixed_lockstep": lock,
             "plain_fp64_one_after_another": {"wall_s": round(wall_s, 3),
                                              "iterations": [x[1] for x in seq],
                                              "all_reached": all(x[2] for x in seq),
                                              "exec_mode": "perop"},
-            "speedup": round(wall_s / wall_b, 3)}
+            "speedup": round(wall_s / wall_b, 3),
+            "speedup_lockstep": round(wall_s / lock["wall_s"], 3)}
+
+
+def lockstep_leg(ctxs, probs, target, max_iters, max_time):
+    """usbs_batch over the resident problems (mixed, persistent form) until
+    each reaches `target`; a short untimed batch first loads the modules."""
+    from usbs_amd.usbs import usbs_batch
+
+    def calls(iters, gap_eps, logs):
+        out = []
+        for ctx, (state, n, m, q0, s, tu, cb), log in zip(ctxs, probs, logs):
+            other = -float("inf") if gap_eps is None else float("inf")
+            out.append(dict(sdp_state=state, n=n, m=m, trace_ub=tu, trace_factor=s["trace_factor"],
+                            rho=GAP_RHO, beta=s["beta"], k_curr=s["k_curr"], k_past=s["k_past"],
+                            max_iters=iters, max_time=max_time,
+                            obj_gap_eps=-float("inf") if gap_eps is None else gap_eps,
+                            infeas_gap_eps=other, max_infeas_eps=other,
+                            lanczos_inner_iterations=s["lanczos_inner"],
+                            lanczos_max_restarts=s["lanczos_restarts"],
+                            subprob_eps=s["subprob_eps"], subprob_max_iters=s["subprob_max_iters"],
+                            cond_exp_base=1.0, callback_fn=cb, callback_static_args=None,
+                            callback_nonstatic_args=state.C, precision="mixed",
+                            exec_mode="persistent", q0=q0, log=log, context=ctx, upload=False))
+        return out
+    usbs_batch(calls(3, None, [[] for _ in probs]))
+    logs = [[] for _ in probs]
+    t0 = time.perf_counter()
+    usbs_batch(calls(max_iters, target, logs))
+    wall = time.perf_counter() - t0
+    return {"wall_s": round(wall, 3), "iterations": [int(lg[-1]["t"]) for lg in logs],
+            "all_reached": all(lg[-1]["obj_gap"] <= target for lg in logs),
+            "exec_mode": "persistent (usbs_solve_batch)"}
 
 
 def serving_leg(requests, rate, slots, iters):

@@ -1,7 +1,11 @@
 """Thick-restart Lanczos for the smallest eigenpairs of C + A*(v), FP64 NumPy.
 
 Restates solver/lanczos.py:59-284 step for step (oracle code: see
-oracle/__init__.py).  Differences that are forced, not chosen:
+oracle/__init__.py).  That reference file is Copyright 2020 Google LLC under
+the Apache License, Version 2.0; this restatement follows its algorithm and
+keeps some of its function names (_iterative_classical_gram_schmidt,
+_build_arrowhead_matrix, _pick_saved_ritz_values) so the two read side by
+side.  Differences that are forced, not chosen:
 
 * the Krylov-breakdown vector (solver/lanczos.py:108-111) comes from
   ``breakdown_normal`` below instead of jax.random.normal(PRNGKey(i+1)); the

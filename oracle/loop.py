@@ -1,5 +1,8 @@
 """Bounded while-loop semantics of utils/loop.py:21-115 (oracle code).
 
+That reference file is Copyright 2021 Google LLC under the Apache License,
+Version 2.0; this restates its loop schedule.
+
 Only *when* cond and body run matters for the solver, so the loops are plain
 Python with the same call pattern as the jitted scan forms.
 """
