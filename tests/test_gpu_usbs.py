@@ -153,7 +153,10 @@ def test_usbs_reordered_nodes_same_sdp(gpu_ctx):
         np.testing.assert_allclose(b["dual_obj"], a["dual_obj"], rtol=1e-8)
     np.testing.assert_allclose(outs[1].y, outs[0].y, rtol=1e-7, atol=1e-9)
     P0, P1 = outs[0].P, unpermute_rows(outs[1].P, perm)
-    np.testing.assert_allclose(P1, P0, rtol=1e-6, atol=1e-8 * np.abs(P0).max())
+    # P accumulates Lanczos vectors converged to an absolute residual of 1e-7
+    # (subprob_eps): a different node order changes their rounding, and with
+    # it P, at that scale relative to |P|
+    np.testing.assert_allclose(P1, P0, rtol=1e-6, atol=1e-7 * np.abs(P0).max())
 
 
 def test_usbs_scaled_state_and_cut_scale(gpu_ctx):

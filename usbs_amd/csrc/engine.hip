@@ -248,7 +248,7 @@ int make_lanczos(Ctx* c, std::vector<DevBuf>& pool, const Problem& P, int m, int
   if (!(L.Yr = (double*)ctx_alloc(c, pool, sizeof(double) * m * m))) return USBS_ERR_OOM;
   if (!(L.ctrl = (int*)ctx_alloc(c, pool, sizeof(int) * LZ_NCTRL))) return USBS_ERR_OOM;
   L.nbar = max_restarts + 1 > 64 ? max_restarts + 1 : 64;
-  if (!(L.bar = (unsigned*)ctx_alloc(c, pool, sizeof(unsigned) * L.nbar))) return USBS_ERR_OOM;
+  if (!(L.bar = (unsigned*)ctx_alloc(c, pool, sizeof(unsigned) * LZ_BAR_STRIDE * L.nbar))) return USBS_ERR_OOM;
   if (!(L.lstat = (int*)ctx_alloc(c, pool, sizeof(int) * 2 * L.nbar))) return USBS_ERR_OOM;
   return USBS_OK;
 }
@@ -269,7 +269,7 @@ __global__ void k_lz_reset(LzInst<T> L, const double* q0) {
   }
   if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
   if (i < LZ_NCTRL) L.ctrl[i] = 0;
-  for (long long x = i; x < L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
+  for (long long x = i; x < (long long)LZ_BAR_STRIDE * L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
   for (long long x = i; x < 2 * L.nbar; x += (long long)gridDim.x * blockDim.x) L.lstat[x] = 0;
 }
 
@@ -330,7 +330,7 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
     int* ctrl_ptr = L.ctrl;
     const LzStop stop{&ctrl_ptr, 2, nullptr};
     rc = lanczos_persistent<T>(d_inst, 1, nb, m, max_restarts, c->stream, d_map, d_map + 2 * nb,
-                               L.ntiles, 1ull << 26, cache_bytes, nullptr, L.tile > 256 ? 1 : 0,
+                               L.ntiles, LZ_SPIN_LIMIT, cache_bytes, nullptr, L.tile > 256 ? 1 : 0,
                                &stop);
     if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos persistent: ") + hipGetErrorString((hipError_t)rc));
     USBS_HIP(c, hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, c->stream));
@@ -416,7 +416,7 @@ template <typename T>
 int LzBatch<T>::run(hipStream_t s, int max_restarts, int first_check, int* launched) {
   const LzStop stop{ctrls.data(), first_check, launched};
   return lanczos_persistent<T>(d_insts, ninst, total_wgs, m, max_restarts, s, d_wg, d_tm,
-                               total_tiles, 1ull << 26, cache, nullptr, rowwise, &stop);
+                               total_tiles, LZ_SPIN_LIMIT, cache, nullptr, rowwise, &stop);
 }
 
 template struct LzBatch<double>;

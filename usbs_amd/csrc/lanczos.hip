@@ -26,6 +26,7 @@
 // rows of Q, the work vector and the CSR slice in LDS for the whole restart
 // when they fit.
 #include <cstring>
+#include <type_traits>
 #include <tuple>
 #include <vector>
 
@@ -92,7 +93,7 @@ static inline size_t rot_lds_bytes(int m) { return sizeof(double) * (size_t)(m *
 // out[j] = sum_rl Q(rl, j) y(rl) for j < c: thread t sums rows g, g+G, ... of
 // column t / G (g = t % G, G = NT / c), then the G sums of a column are added
 // in g order.
-template <typename T>
+template <typename T, typename M = MemPlain>
 __device__ void col_dots(const T* qb, long long qld, const T* yb, int R, int c, double* red,
                          double* out) {
   const int t = threadIdx.x;
@@ -109,13 +110,13 @@ __device__ void col_dots(const T* qb, long long qld, const T* yb, int R, int c, 
   if (t < c) {
     double s = 0.0;
     for (int q = 0; q < G; ++q) s += red[t * G + q];
-    out[t] = s;
+    M::st(out + t, s);
   }
   __syncthreads();
 }
 
 // out[0] = sum_rl y(rl)^2 (thread-strided partial sums, then in thread order).
-template <typename T>
+template <typename T, typename M = MemPlain>
 __device__ void self_dot(const T* yb, int R, double* red, double* out) {
   const int t = threadIdx.x;
   double s = 0.0;
@@ -132,7 +133,7 @@ __device__ void self_dot(const T* yb, int R, double* red, double* out) {
   if (t == 0) {
     double v = 0.0;
     for (int q = 0; q < 64; ++q) v += red[USBS_NT + q];
-    out[0] = v;
+    M::st(out, v);
   }
   __syncthreads();
 }
@@ -149,10 +150,10 @@ __device__ void row_update(const T* qb, long long qld, T* yb, int R, int c, cons
 }
 
 // beta = sqrt(sum of the part3 tile partials), identical in every block.
-template <typename T>
+template <typename T, typename M = MemPlain>
 __device__ double lz_beta(const LzInst<T>& L, double* red) {
   double* s = red + USBS_NT;
-  sum_tile_partials(L.part3, L.ntiles, L.pstride, 1, s, red);
+  sum_tile_partials<M>(L.part3, L.ntiles, L.pstride, 1, s, red);
   return sqrt(s[0]);
 }
 
@@ -315,19 +316,19 @@ struct TileCtx {
 // mode 3: only normalise w into column i).  first: q_i is already in Q.
 // RW: the kernel may meet tiles of more than 256 rows (row-wise reductions);
 // kernels instantiated with RW = false carry no row-wise code.
-template <typename T, bool C, int MAXC, bool RW>
+template <typename T, bool C, int MAXC, bool RW, typename M = MemPlain>
 __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, bool first,
                        int mode, double* red, double beta_known = -1.0) {
   const int t = threadIdx.x;
   const T* gb = L.w + (long long)((i + 1) & 1) * L.ldq;  // w_{i-1}
   T* wc = L.w + (long long)(i & 1) * L.ldq;               // this step's work vector (per-op)
   double beta = 1.0;
-  if (!first) beta = beta_known >= 0.0 ? beta_known : lz_beta(L, red);
+  if (!first) beta = beta_known >= 0.0 ? beta_known : lz_beta<T, M>(L, red);
   T* Qi = L.Q + (long long)i * L.ldq + X.r0;
   if (!first) {
     for (int rl = t; rl < X.R; rl += USBS_NT) {
-      const T q = (T)((double)gb[X.r0 + rl] / beta);
-      Qi[rl] = q;
+      const T q = (T)((double)M::ld(gb + X.r0 + rl) / beta);
+      M::st(Qi + rl, q);
       if constexpr (C) X.qc[(long long)i * X.ldc + rl] = q;
     }
   }
@@ -387,7 +388,7 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
         for (int g = 0; g < RG; ++g)
 #pragma unroll
           for (int u = 0; u < UN; ++u) {
-            const T x = first ? xq[c[g][u]] : (T)((double)gb[c[g][u]] / beta);
+            const T x = first ? M::ld(xq + c[g][u]) : (T)((double)M::ld(gb + c[g][u]) / beta);
             s[g] += (double)v[g][u] * (double)x;
           }
       }
@@ -400,7 +401,7 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
       }
     }
     __syncthreads();
-    col_dots(qb, qld, yb, X.R, i + 1, red, L.part1 + (long long)tl * L.pstride);
+    col_dots<T, M>(qb, qld, yb, X.R, i + 1, red, L.part1 + (long long)tl * L.pstride);
     return;
   }
   for (int rl = t; rl < X.R; rl += USBS_NT) {
@@ -410,7 +411,7 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
         const int p1 = X.rp[rl + 1];
         for (int p = X.rp[rl]; p < p1; ++p) {
           const int c = X.ci[p];
-          const T x = first ? xq[c] : (T)((double)gb[c] / beta);
+          const T x = first ? M::ld(xq + c) : (T)((double)M::ld(gb + c) / beta);
           s += (double)X.cv[p] * (double)x;
         }
       } else {
@@ -444,19 +445,19 @@ __device__ void phaseA(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i
     return;
   }
   __syncthreads();
-  col_dots(qb, qld, yb, X.R, i + 1, red, L.part1 + (long long)tl * L.pstride);
+  col_dots<T, M>(qb, qld, yb, X.R, i + 1, red, L.part1 + (long long)tl * L.pstride);
 }
 
 // Phases B (final = 0) and C (final = 1).
-template <typename T, bool C, int MAXC, bool RW>
+template <typename T, bool C, int MAXC, bool RW, typename M = MemPlain>
 __device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, int final_pass,
                         double* red) {
   const int t = threadIdx.x;
   double* h = lz_hbuf(L, red);
   const double* src = final_pass ? L.part2 : L.part1;
-  sum_tile_partials(src, L.ntiles, L.pstride, i + 1, h, red);
+  sum_tile_partials<M>(src, L.ntiles, L.pstride, i + 1, h, red);
   if (tl == 0)
-    for (int j = t; j <= i; j += USBS_NT) (final_pass ? L.h2 : L.h1)[j] = h[j];
+    for (int j = t; j <= i; j += USBS_NT) M::st((final_pass ? L.h2 : L.h1) + j, h[j]);
   T* yb;
   const T* qb;
   long long qld;
@@ -492,15 +493,15 @@ __device__ void phaseBC(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int 
   } else {
     row_update(qb, qld, yb, X.R, i + 1, h);
     if (!final_pass) {
-      col_dots(qb, qld, yb, X.R, i + 1, red, part_out);
+      col_dots<T, M>(qb, qld, yb, X.R, i + 1, red, part_out);
     } else {
       if constexpr (C)
-        for (int rl = t; rl < X.R; rl += USBS_NT) wc[rl] = yb[rl];
-      self_dot(yb, X.R, red, part_out);
+        for (int rl = t; rl < X.R; rl += USBS_NT) M::st(wc + rl, yb[rl]);
+      self_dot<T, M>(yb, X.R, red, part_out);
     }
   }
   if (final_pass && tl == 0 && t == 0) {
-    L.alpha[i] = L.h1[i] + h[i];
+    L.alpha[i] = M::ld(L.h1 + i) + h[i];
     L.ctrl[LZ_STEPS] += 1;
   }
 }
@@ -535,11 +536,38 @@ __device__ void breakdown_sub(const LzInst<T>& L, int tl, int i, int sub, double
   }
 }
 
+// The same restart vector for a tile cached in LDS (the coherent persistent
+// kernel): built in the LDS work vector with the arithmetic of
+// breakdown_sub, and published to w_{i-1} in global memory in the third pass.
+template <typename T, bool C>
+__device__ void breakdown_sub_lds(const LzInst<T>& L, const TileCtx<T, C>& X, int tl, int i, int sub,
+                                  double* red) {
+  using M = MemCoherent;
+  const int t = threadIdx.x;
+  T* yc = X.yc;
+  if (sub == 0) {
+    for (int rl = t; rl < X.R; rl += USBS_NT) yc[rl] = (T)counter_normal_cold((uint64_t)i, (uint64_t)(X.r0 + rl));
+    __syncthreads();
+    col_dots<T, M>(X.qc, X.ldc, yc, X.R, i, red, L.part1 + (long long)tl * L.pstride);
+  } else {
+    double* h = lz_hbuf(L, red);
+    sum_tile_partials<M>(sub == 1 ? L.part1 : L.part2, L.ntiles, L.pstride, i, h, red);
+    row_update(X.qc, X.ldc, yc, X.R, i, h);
+    if (sub == 1) {
+      col_dots<T, M>(X.qc, X.ldc, yc, X.R, i, red, L.part2 + (long long)tl * L.pstride);
+    } else {
+      T* gb = L.w + (long long)((i + 1) & 1) * L.ldq + X.r0;
+      for (int rl = t; rl < X.R; rl += USBS_NT) M::st(gb + rl, yc[rl]);
+      self_dot<T, M>(yc, X.R, red, L.part3 + (long long)tl * L.pstride);
+    }
+  }
+}
+
 // beta_{i-1} from part3: recorded in beta[i-1]; returns whether it broke down.
-template <typename T>
+template <typename T, typename M = MemPlain>
 __device__ bool lz_record_beta(const LzInst<T>& L, int i, bool writer, double* red,
                                double* beta_out = nullptr) {
-  const double b = lz_beta(L, red);
+  const double b = lz_beta<T, M>(L, red);
   if (beta_out) *beta_out = b;
   const bool brk = b < 1e-15;
   if (writer && threadIdx.x == 0) {
@@ -657,6 +685,496 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_rotate_multi(const LzInst<T>* __
   tile_rotate<T, MAXC>(L, mp.y, knext, lds);
 }
 
+// ----------------------------------------------------------------------------
+// Row-in-registers (RR) tiles: light operator rows (a thread per row in the
+// SpMV) in tiles of at most USBS_NT rows.  Thread t owns row r0 + t of its
+// tile and holds that row of Q in registers (q[MAXC]) and its element of the
+// work vector (y), so the CGS updates y -= Q h are register dot products and
+// only the column sums cross threads: a reduce-scatter butterfly over each
+// wavefront's lanes, then the four wavefronts added in a fixed order.  The
+// per-op kernels load q from Q at the start of each phase; the persistent
+// kernel keeps q and y in registers for the whole restart.  Both use these
+// routines, so they agree bitwise.
+
+// Slots of the butterfly: a power of two >= the most columns a dot product
+// has (m = MAXC - 1).
+template <int MAXC>
+struct RRSlots {
+  static constexpr int C = (MAXC - 1 <= 16) ? 16 : (MAXC - 1 <= 32) ? 32 : 64;
+};
+
+// Reduce-scatter over the 64 lanes: stage s exchanges half of the live slots
+// with lane ^ (1 << s) (lane_xor: the large early stages are DPP moves); a
+// lane keeps the upper half when lane bit s is set.  After log2(C) stages
+// lane l holds the wavefront's sum of slot rr_slot<C>(l), the bit reversal
+// of its low log2(C) bits; the remaining stages add the lanes that share a
+// slot.  Each sum is formed by one fixed lane, so the result is deterministic.
+template <int C>
+__device__ __forceinline__ int rr_slot(int lane) {
+  constexpr int LG = (C >= 64) ? 6 : (C >= 32) ? 5 : (C >= 16) ? 4 : (C >= 8) ? 3 : (C >= 4) ? 2 : (C >= 2) ? 1 : 0;
+  int s = 0;
+  for (int b = 0; b < LG; ++b) s |= ((lane >> b) & 1) << (LG - 1 - b);
+  return s;
+}
+template <int C>
+__device__ __forceinline__ void rr_butterfly(double (&v)[C]) {
+  const int lane = threadIdx.x & 63;
+  static_for<6>([&](auto ST) {
+    constexpr int mask = 1 << ST;
+    constexpr int half = C >> (ST + 1);
+    if constexpr (half >= 1) {
+      const bool up = (lane & mask) != 0;
+      static_for<half>([&](auto S) {
+        const double a = v[S], b = v[S + half];
+        const double send = up ? a : b;
+        const double keep = up ? b : a;
+        v[S] = keep + lane_xor<mask>(send);
+      });
+    } else {
+      v[0] += lane_xor<mask>(v[0]);
+    }
+  });
+}
+
+// out[j] (j < c <= C) = sum over the block's rows of v[j]: the butterfly,
+// then wavefronts 0..3 added as (w0 + w1) + (w2 + w3).  red: 4C doubles of LDS.
+template <int C, typename M>
+__device__ __forceinline__ void rr_reduce_cols(double (&v)[C], int c, double* red, double* out) {
+  rr_butterfly<C>(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane < C) red[wv * C + rr_slot<C>(lane)] = v[0];
+  __syncthreads();
+  if ((int)threadIdx.x < c) {
+    const int j = threadIdx.x;
+    M::st(out + j, (red[j] + red[C + j]) + (red[2 * C + j] + red[3 * C + j]));
+  }
+  __syncthreads();
+}
+
+// h[j] (LDS) = sum over tiles of part[tile * stride + j], j < c <= 64: P = 8
+// lanes per column (4 above 32 columns); lane q of a column adds tiles q,
+// q + P, ... in order, then the P lanes are added by DPP (xor 1, 2, 4).  Every
+// lane of the block takes part; the order depends only on (c, ntiles).
+template <typename M>
+__device__ __forceinline__ void rr_sum_tiles(const double* __restrict__ part, int ntiles, int stride,
+                                             int c, double* out) {
+  const int t = threadIdx.x;
+  const int P = c <= 32 ? 8 : 4;
+  const int j = t / P, q = t & (P - 1);
+  const int jj = j < c ? j : 0;
+  double s = 0.0;
+  for (int tl0 = q; tl0 < ntiles; tl0 += 16 * P) {
+    double v[16];
+    static_for<16>([&](auto U) {
+      const int tl = tl0 + U * P;
+      v[U] = M::ld(part + (size_t)(tl < ntiles ? tl : 0) * stride + jj);
+    });
+    static_for<16>([&](auto U) {
+      if (tl0 + U * P < ntiles) s += v[U];
+    });
+  }
+  s += lane_xor<1>(s);
+  s += lane_xor<2>(s);
+  if (P == 8) s += lane_xor<4>(s);
+  if (q == 0 && j < c) out[j] = s;
+  __syncthreads();
+}
+
+// beta = sqrt(sum of the part3 tile partials), identical in every block.
+template <typename T, typename M>
+__device__ __forceinline__ double rr_beta(const LzInst<T>& L, double* red) {
+  double* s = red + USBS_NT;
+  rr_sum_tiles<M>(L.part3, L.ntiles, L.pstride, 1, s);
+  const double b = sqrt(s[0]);
+  __syncthreads();
+  return b;
+}
+
+// beta_{i-1} recorded in beta[i-1] (writer); returns whether it broke down.
+template <typename T, typename M>
+__device__ __forceinline__ bool rr_record_beta(const LzInst<T>& L, int i, bool writer, double* red,
+                                               double* beta_out = nullptr) {
+  const double b = rr_beta<T, M>(L, red);
+  if (beta_out) *beta_out = b;
+  const bool brk = b < 1e-15;
+  if (writer && threadIdx.x == 0) {
+    L.beta[i - 1] = b;
+    L.ctrl[LZ_BREAKDOWN] = brk ? 1 : 0;
+    if (brk) L.ctrl[LZ_NBREAK] += 1;
+  }
+  __syncthreads();
+  return brk;
+}
+
+// The thread's row of Q[:, 0..c-1] (columns past c-1 and rows past the tile
+// read column 0 / the tile's first row, in bounds, and are zeroed).
+template <typename T, int MAXC, typename M>
+__device__ __forceinline__ void rr_load_q(const LzInst<T>& L, int r, bool valid, int c, T (&q)[MAXC]) {
+  static_for<MAXC>([&](auto J) {
+    constexpr int j = J;
+    const T v = M::ld(L.Q + (long long)(j < c ? j : 0) * L.ldq + r);
+    q[j] = (valid && j < c) ? v : (T)0;
+  });
+}
+
+// v[j] = q[j] y for j <= i (and j < MAXC), else 0.
+template <typename T, int MAXC, int C>
+__device__ __forceinline__ void rr_products(const T (&q)[MAXC], double yd, int i, double (&v)[C]) {
+  static_for<C>([&](auto J) {
+    constexpr int j = J;
+    if constexpr (j < MAXC) v[j] = (j <= i) ? (double)q[j] * yd : 0.0;
+    else v[j] = 0.0;
+  });
+}
+
+// sum_{j < c} q[j] h[j], j ascending.  h: at least MAXC readable doubles
+// (the terms past c are loaded with the others and not added).
+template <typename T, int MAXC>
+__device__ __forceinline__ double rr_qdot(const T (&q)[MAXC], const double* h, int c) {
+  double hv[MAXC];
+  static_for<MAXC>([&](auto J) { hv[J] = h[J]; });
+  double s = 0.0;
+  static_for<MAXC>([&](auto J) {
+    if (J < c) s += (double)q[J] * hv[J];
+  });
+  return s;
+}
+
+// Loads of phase A that do not need beta, issued before it is summed so that
+// all of them (and the beta partials) are in flight together: the thread's
+// element of w_{i-1} and the operand values of its row's first RR_PF
+// operator entries (w_{i-1} raw, or q_i on the first step).  Entries past
+// the row read a valid entry of the row (or column 0 of an empty row).
+#define RR_PF 8
+template <typename T>
+struct RRPre {
+  T w;
+  T x[RR_PF];
+};
+template <typename T, typename M>
+__device__ __forceinline__ void rr_prefetch(const LzInst<T>& L, int tl, int i, bool first, int mode,
+                                            const int* rp, const int* ci, RRPre<T>& pf) {
+  const int t = threadIdx.x;
+  const int r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
+  const bool valid = t < R;
+  const int r = r0 + (valid ? t : 0);
+  const T* gb = L.w + (long long)((i + 1) & 1) * L.ldq;
+  pf.w = first ? (T)0 : M::ld(gb + r);
+  if (mode == 0) {
+    const T* src = first ? L.Q + (long long)i * L.ldq : gb;
+    const int p0 = valid ? rp[t] : 0, p1 = valid ? rp[t + 1] : 0;
+    static_for<RR_PF>([&](auto U) {
+      const int p = p0 + U;
+      const int c = p < p1 ? ci[p] : (p1 > p0 ? ci[p0] : 0);
+      pf.x[U] = M::ld(src + c);
+    });
+  }
+}
+
+// Phase A of step i for the thread's row: mode 0 y = Op q_i (CSR rows rp[t]..
+// rp[t+1] of ci/cv), mode 2 y = Q[:, i+1] (column-by-column QR), mode 3 only
+// normalise w_{i-1} into Q[:, i].  first: q_i is already in Q (and q).
+// part1 = Q^T y.  beta: |w_{i-1}| (unused when first).  pf: rr_prefetch of
+// the same step.  The operator sum runs over the row's entries in order.
+template <typename T, int MAXC, typename M>
+__device__ __forceinline__ void rr_phaseA(const LzInst<T>& L, int tl, int i, bool first, int mode, double beta,
+                          const RRPre<T>& pf, T (&q)[MAXC], T& y, const int* rp, const int* ci,
+                          const T* cv, double* red) {
+  constexpr int C = RRSlots<MAXC>::C;
+  const int t = threadIdx.x;
+  const int r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
+  const bool valid = t < R;
+  const int r = r0 + (valid ? t : 0);
+  const T* gb = L.w + (long long)((i + 1) & 1) * L.ldq;  // w_{i-1}
+  if (!first) {
+    const T qi = (T)((double)pf.w / beta);
+    if (valid) M::st(L.Q + (long long)i * L.ldq + r, qi);
+    const T qv = valid ? qi : (T)0;
+    static_for<MAXC>([&](auto J) { q[J] = (J == i) ? qv : q[J]; });
+  }
+  if (mode == 3) return;
+  double s = 0.0;
+  if (mode == 0) {
+    const T* src = first ? L.Q + (long long)i * L.ldq : gb;
+    if (valid) {
+      const int p0 = rp[t], p1 = rp[t + 1];
+      static_for<RR_PF>([&](auto U) {
+        const int p = p0 + U;
+        if (p < p1) {
+          const T x = first ? pf.x[U] : (T)((double)pf.x[U] / beta);
+          s += (double)cv[p] * (double)x;
+        }
+      });
+      for (int pb = p0 + RR_PF; pb < p1; pb += RR_PF) {
+        T xv[RR_PF];
+        static_for<RR_PF>([&](auto U) {
+          const int p = pb + U;
+          xv[U] = M::ld(src + ci[p < p1 ? p : pb]);
+        });
+        static_for<RR_PF>([&](auto U) {
+          const int p = pb + U;
+          if (p < p1) {
+            const T x = first ? xv[U] : (T)((double)xv[U] / beta);
+            s += (double)cv[p] * (double)x;
+          }
+        });
+      }
+    }
+  } else {
+    s = (double)M::ld(L.Q + (long long)(i + 1) * L.ldq + r);
+  }
+  y = (T)s;
+  double v[C];
+  rr_products<T, MAXC, C>(q, valid ? (double)y : 0.0, i, v);
+  rr_reduce_cols<C, M>(v, i + 1, red, L.part1 + (long long)tl * L.pstride);
+}
+
+// Phases B (final 0: part2 = Q^T y) and C (final 1: part3 = |y|^2) after
+// y -= Q h with h the sum of the previous pass's tile partials.
+template <typename T, int MAXC, typename M>
+__device__ __forceinline__ void rr_phaseBC(const LzInst<T>& L, int tl, int i, int final_pass, T (&q)[MAXC], T& y,
+                           double* red) {
+  constexpr int C = RRSlots<MAXC>::C;
+  const int t = threadIdx.x;
+  const int r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
+  const bool valid = t < R;
+  double* h = lz_hbuf(L, red);
+  rr_sum_tiles<M>(final_pass ? L.part2 : L.part1, L.ntiles, L.pstride, i + 1, h);
+  if (tl == 0)
+    for (int j = t; j <= i; j += USBS_NT) M::st((final_pass ? L.h2 : L.h1) + j, h[j]);
+  y = (T)((double)y - rr_qdot<T, MAXC>(q, h, i + 1));
+  const double yd = valid ? (double)y : 0.0;
+  if (!final_pass) {
+    double v[C];
+    rr_products<T, MAXC, C>(q, yd, i, v);
+    rr_reduce_cols<C, M>(v, i + 1, red, L.part2 + (long long)tl * L.pstride);
+  } else {
+    double v[1] = {yd * yd};
+    rr_reduce_cols<1, M>(v, 1, red, L.part3 + (long long)tl * L.pstride);
+    if (tl == 0 && t == 0) {
+      L.alpha[i] = M::ld(L.h1 + i) + h[i];
+      L.ctrl[LZ_STEPS] += 1;
+    }
+  }
+}
+
+// Breakdown restart vector for tile tl (lanczos.py:107-117): counter normals
+// keyed by step i, orthogonalised twice against Q[:, 0..i-1], left in w_{i-1}
+// with |.|^2 partials in part3; sub = 0, 1, 2 are the three reduction passes.
+// q: the thread's row of Q[:, 0..i-1].
+template <typename T, int MAXC, typename M>
+__device__ __forceinline__ void rr_breakdown_sub(const LzInst<T>& L, int tl, int i, int sub, const T (&q)[MAXC],
+                                 double* red) {
+  constexpr int C = RRSlots<MAXC>::C;
+  const int t = threadIdx.x;
+  const int r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
+  const bool valid = t < R;
+  const int r = r0 + (valid ? t : 0);
+  T* gb = L.w + (long long)((i + 1) & 1) * L.ldq;
+  T y;
+  if (sub == 0) {
+    y = (T)counter_normal_cold((uint64_t)i, (uint64_t)r);
+  } else {
+    double* h = lz_hbuf(L, red);
+    rr_sum_tiles<M>(sub == 1 ? L.part1 : L.part2, L.ntiles, L.pstride, i, h);
+    y = (T)((double)M::ld(gb + r) - rr_qdot<T, MAXC>(q, h, i));
+  }
+  if (valid) M::st(gb + r, y);
+  const double yd = valid ? (double)y : 0.0;
+  if (sub < 2) {
+    double v[C];
+    rr_products<T, MAXC, C>(q, yd, i - 1, v);
+    rr_reduce_cols<C, M>(v, i, red, (sub == 0 ? L.part1 : L.part2) + (long long)tl * L.pstride);
+  } else {
+    double v[1] = {yd * yd};
+    rr_reduce_cols<1, M>(v, 1, red, L.part3 + (long long)tl * L.pstride);
+  }
+}
+
+// per-op RR kernels (one block per tile)
+template <typename T, int MAXC>
+__global__ void __launch_bounds__(USBS_NT) k_lz_A_rr(LzInst<T> L, int i, int first, int mode, int record) {
+  extern __shared__ double lds[];
+  if (L.ctrl[LZ_PENDING]) return;
+  const int tl = blockIdx.x, t = threadIdx.x;
+  RRPre<T> pf;
+  rr_prefetch<T, MemPlain>(L, tl, i, first != 0, mode, L.row_ptr + tl * L.tile, L.col, pf);
+  double beta = -1.0;
+  if (!first) {
+    beta = rr_beta<T, MemPlain>(L, lds);
+    if (record) {
+      __syncthreads();
+      const bool brk = beta < 1e-15;
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (brk) {
+          L.ctrl[LZ_PENDING] = i;
+        } else {
+          L.beta[i - 1] = beta;
+          L.ctrl[LZ_BREAKDOWN] = 0;
+        }
+      }
+      if (brk) return;
+    }
+  }
+  const int r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
+  const bool valid = t < R;
+  const int r = r0 + (valid ? t : 0);
+  T q[MAXC];
+  rr_load_q<T, MAXC, MemPlain>(L, r, valid, i + 1, q);
+  T y = (T)0;
+  rr_phaseA<T, MAXC, MemPlain>(L, tl, i, first != 0, mode, beta, pf, q, y, L.row_ptr + r0, L.col, L.val, lds);
+  if (mode != 3 && valid) L.w[(long long)(i & 1) * L.ldq + r] = y;
+}
+
+template <typename T, int MAXC>
+__global__ void __launch_bounds__(USBS_NT) k_lz_BC_rr(LzInst<T> L, int i, int final_pass) {
+  extern __shared__ double lds[];
+  if (L.ctrl[LZ_PENDING]) return;
+  const int tl = blockIdx.x, t = threadIdx.x;
+  const int r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
+  const bool valid = t < R;
+  const int r = r0 + (valid ? t : 0);
+  T* wc = L.w + (long long)(i & 1) * L.ldq;
+  T q[MAXC];
+  rr_load_q<T, MAXC, MemPlain>(L, r, valid, i + 1, q);
+  T y = wc[r];
+  rr_phaseBC<T, MAXC, MemPlain>(L, tl, i, final_pass, q, y, lds);
+  if (valid) wc[r] = y;
+}
+
+// beta_{i-1} and, on breakdown, the restart vector: one block walking every
+// tile in order (the tile arithmetic of the multi-block phases).
+template <typename T, int MAXC>
+__global__ void __launch_bounds__(USBS_NT) k_lz_pre_rr(LzInst<T> L, int i) {
+  extern __shared__ double lds[];
+  if (threadIdx.x == 0) L.ctrl[LZ_PENDING] = 0;
+  if (!rr_record_beta<T, MemPlain>(L, i, true, lds)) return;
+  for (int sub = 0; sub < 3; ++sub) {
+    for (int tl = 0; tl < L.ntiles; ++tl) {
+      const int t = threadIdx.x, r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
+      T q[MAXC];
+      rr_load_q<T, MAXC, MemPlain>(L, r0 + (t < R ? t : 0), t < R, i, q);
+      rr_breakdown_sub<T, MAXC, MemPlain>(L, tl, i, sub, q, lds);
+    }
+    __syncthreads();
+  }
+}
+
+// All Lanczos steps of restart r of an RR instance inside the persistent
+// launch (k_lz_steps_coop), with coh_barrier between the phases and
+// MemCoherent accesses for every cross-workgroup datum.  RES (every tile has
+// its own workgroup): the workgroup keeps its rows of Q and the work vector
+// in registers and its CSR slice in LDS for the whole restart; otherwise a
+// workgroup walks tiles lw, lw + G, ... and loads them per phase.
+// Basis columns a kernel instantiation holds (MAXC) for basis size m.
+__host__ __device__ constexpr int lz_maxc(int m) { return m + 1 <= 17 ? 17 : m + 1 <= 33 ? 33 : 49; }
+// Reduction scratch and h (rr_qdot reads MAXC entries) before the CSR slice.
+__host__ __device__ constexpr int rr_head_doubles(int m) { return LZ_RED + lz_maxc(m) + 8; }
+template <typename T>
+static inline size_t rr_res_lds_bytes(int tile, int m, int nnz_tile) {
+  size_t b = sizeof(double) * (size_t)rr_head_doubles(m);
+  b += sizeof(T) * (size_t)nnz_tile + sizeof(int) * (size_t)(tile + 1 + nnz_tile);
+  return (b + 255) / 256 * 256;
+}
+
+template <typename T, int MAXC, bool RES>
+__device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsigned long long spin_limit, double* lds) {
+  using M = MemCoherent;
+  const int G = L.nblocks, t = threadIdx.x;
+  int* err = L.ctrl + LZ_ERROR;
+  unsigned* bar = L.bar + (size_t)LZ_BAR_STRIDE * (r < L.nbar ? r : L.nbar - 1);
+  unsigned epoch = 0;
+  const int k = L.ctrl[LZ_K];
+  if (k >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
+  if (lw == 0 && t == 0 && r < L.nbar) {
+    int sumc = 0;
+    for (int i = k; i < L.m; ++i) sumc += i + 1;
+    L.lstat[2 * r] = L.m - k;
+    L.lstat[2 * r + 1] = sumc;
+  }
+  double* red = lds;
+  // RES: the tile's CSR slice in LDS, values first (T), then row pointers
+  // relative to the slice and columns
+  T* cv = (T*)(lds + rr_head_doubles(L.m));
+  int* rp = (int*)(cv + L.tile_nnz_max);
+  int* ci = rp + L.tile + 1;
+  T q[MAXC];
+  T y = (T)0;
+  const int r0 = lw * L.tile;
+  const int R = RES ? min(L.n, r0 + L.tile) - r0 : 0;
+  const bool valid = t < R;
+  const int rr = r0 + (valid ? t : 0);
+  if constexpr (RES) {
+    const int pbase = L.row_ptr[r0];
+    const int nnzt = L.row_ptr[r0 + R] - pbase;
+    for (int x = t; x <= R; x += USBS_NT) rp[x] = L.row_ptr[r0 + x] - pbase;
+    for (int p = t; p < nnzt; p += USBS_NT) { ci[p] = L.col[pbase + p]; cv[p] = L.val[pbase + p]; }
+    rr_load_q<T, MAXC, MemPlain>(L, rr, valid, k + 1, q);
+    __syncthreads();
+  }
+  bool ok = true;
+  for (int i = k; i <= L.m && ok; ++i) {
+    double beta = -1.0;
+    const int mode = i == L.m ? 3 : 0;
+    RRPre<T> pf;
+    if constexpr (RES) rr_prefetch<T, M>(L, lw, i, i == k, mode, rp, ci, pf);
+    if (i > k) {
+      if (rr_record_beta<T, M>(L, i, lw == 0, red, &beta)) {
+        for (int sub = 0; sub < 3 && ok; ++sub) {
+          if constexpr (RES) {
+            rr_breakdown_sub<T, MAXC, M>(L, lw, i, sub, q, red);
+          } else {
+            for (int tl = lw; tl < L.ntiles; tl += G) {
+              const int s0 = tl * L.tile, SR = min(L.n, s0 + L.tile) - s0;
+              T qt[MAXC];
+              rr_load_q<T, MAXC, M>(L, s0 + (t < SR ? t : 0), t < SR, i, qt);
+              rr_breakdown_sub<T, MAXC, M>(L, tl, i, sub, qt, red);
+            }
+          }
+          ok = coh_barrier(bar, lw, G, epoch, err, spin_limit);
+        }
+        if (!ok) break;
+        beta = rr_beta<T, M>(L, red);  // the restart vector's norm
+        if constexpr (RES) rr_prefetch<T, M>(L, lw, i, false, mode, rp, ci, pf);  // w_{i-1} changed
+      }
+    }
+    if constexpr (RES) {
+      rr_phaseA<T, MAXC, M>(L, lw, i, i == k, mode, beta, pf, q, y, rp, ci, cv, red);
+    } else {
+      for (int tl = lw; tl < L.ntiles; tl += G) {
+        const int s0 = tl * L.tile, SR = min(L.n, s0 + L.tile) - s0;
+        const int sr = s0 + (t < SR ? t : 0);
+        T qt[MAXC];
+        rr_load_q<T, MAXC, M>(L, sr, t < SR, i + 1, qt);
+        T yt = (T)0;
+        rr_prefetch<T, M>(L, tl, i, i == k, mode, L.row_ptr + s0, L.col, pf);
+        rr_phaseA<T, MAXC, M>(L, tl, i, i == k, mode, beta, pf, qt, yt, L.row_ptr + s0, L.col, L.val, red);
+        if (mode == 0 && t < SR) M::st(L.w + (long long)(i & 1) * L.ldq + sr, yt);
+      }
+    }
+    if (mode == 3) break;
+    if (!(ok = coh_barrier(bar, lw, G, epoch, err, spin_limit))) break;
+    for (int pass = 0; pass < 2 && ok; ++pass) {
+      if constexpr (RES) {
+        rr_phaseBC<T, MAXC, M>(L, lw, i, pass, q, y, red);
+        if (pass == 1 && valid) M::st(L.w + (long long)(i & 1) * L.ldq + rr, y);
+      } else {
+        for (int tl = lw; tl < L.ntiles; tl += G) {
+          const int s0 = tl * L.tile, SR = min(L.n, s0 + L.tile) - s0;
+          const int sr = s0 + (t < SR ? t : 0);
+          T* wc = L.w + (long long)(i & 1) * L.ldq;
+          T qt[MAXC];
+          rr_load_q<T, MAXC, M>(L, sr, t < SR, i + 1, qt);
+          T yt = M::ld(wc + sr);
+          rr_phaseBC<T, MAXC, M>(L, tl, i, pass, qt, yt, red);
+          if (t < SR) M::st(wc + sr, yt);
+        }
+      }
+      ok = coh_barrier(bar, lw, G, epoch, err, spin_limit);
+    }
+  }
+}
+
 // Destination of the per-op step launches: the stream, or (graph set) a
 // chain of kernel nodes appended to a graph built node by node (no stream
 // capture, so other threads' synchronous HIP calls are unaffected).
@@ -708,18 +1226,22 @@ static void perop_steps_m(const LzInst<T>& L, int i0, int mode, StepSink& k, boo
   const size_t lds = tile_lds_bytes(L.m + 1);
   const dim3 g(L.ntiles), b(USBS_NT), one(1);
   const int rec = optimistic ? 1 : 0;
-  if (resume) k.launch(k_lz_pre<T>, one, b, lds, L, i0);
+  // light rows in tiles of at most USBS_NT rows: the row-in-registers kernels
+  const bool rr = !RW && !L.heavy && L.tile <= USBS_NT;
+  auto kpre = rr ? k_lz_pre_rr<T, MAXC> : k_lz_pre<T>;
+  auto kA = rr ? k_lz_A_rr<T, MAXC> : k_lz_A<T, MAXC, RW>;
+  auto kBC = rr ? k_lz_BC_rr<T, MAXC> : k_lz_BC<T, MAXC, RW>;
+  if (resume) k.launch(kpre, one, b, lds, L, i0);
   for (int i = i0; i < L.m; ++i) {
-    if (i > i0 && !optimistic) k.launch(k_lz_pre<T>, one, b, lds, L, i);
+    if (i > i0 && !optimistic) k.launch(kpre, one, b, lds, L, i);
     // the resumed step normalises the vector k_lz_pre built (beta already recorded)
-    k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, i, (i == i0 && !resume) ? 1 : 0, mode,
-             (resume && i == i0) ? 0 : rec);
-    k.launch(k_lz_BC<T, MAXC, RW>, g, b, lds, L, i, 0);
-    k.launch(k_lz_BC<T, MAXC, RW>, g, b, lds, L, i, 1);
+    k.launch(kA, g, b, lds, L, i, (i == i0 && !resume) ? 1 : 0, mode, (resume && i == i0) ? 0 : rec);
+    k.launch(kBC, g, b, lds, L, i, 0);
+    k.launch(kBC, g, b, lds, L, i, 1);
   }
-  if (!optimistic) k.launch(k_lz_pre<T>, one, b, lds, L, L.m);
+  if (!optimistic) k.launch(kpre, one, b, lds, L, L.m);
   // after a resume at i0 = m the breakdown vector is already built and normalised here
-  k.launch(k_lz_A<T, MAXC, RW>, g, b, lds, L, L.m, 0, 3, (resume && i0 == L.m) ? 0 : rec);
+  k.launch(kA, g, b, lds, L, L.m, 0, 3, (resume && i0 == L.m) ? 0 : rec);
 }
 
 template <typename T, bool RW>
@@ -845,18 +1367,25 @@ static inline size_t lz_cache_bytes(int tile, int m, int nnz_tile) {
   return (b + 255) / 256 * 256;
 }
 
+// Instances with light rows in tiles of at most USBS_NT rows run rr_steps
+// (C: one workgroup per tile, resident in registers).  Others: C (tiles
+// cached in LDS) moves cross-workgroup data (tile partials, the work vector
+// the operator gathers, Q columns) through MemCoherent accesses with
+// coh_barrier, no L2 writeback or invalidate; tiles streamed from global
+// memory (C false) use plain accesses and grid_barrier.  Either way, restart
+// r uses counter region r of L.bar (LZ_BAR_STRIDE each).
 template <typename T, bool C, int MAXC, bool RW>
-__global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __restrict__ insts,
-                                                           const int2* __restrict__ wgmap, int r,
-                                                           unsigned long long spin_limit) {
-  extern __shared__ double lds[];
-  const int2 mp = wgmap[blockIdx.x];
-  const LzInst<T> L = insts[mp.x];
-  if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;  // uniform over the instance's workgroups
-  const int lw = mp.y, G = L.nblocks, t = threadIdx.x;
+__device__ __forceinline__ void tile_steps(const LzInst<T> L, int lw, int r,
+                                           unsigned long long spin_limit, double* lds) {
+  using M = typename std::conditional<C, MemCoherent, MemPlain>::type;
+  const int G = L.nblocks, t = threadIdx.x;
   int* err = L.ctrl + LZ_ERROR;
-  unsigned* bar = L.bar + (r < L.nbar ? r : L.nbar - 1);
+  unsigned* bar = L.bar + (size_t)LZ_BAR_STRIDE * (r < L.nbar ? r : L.nbar - 1);
   unsigned target = 0;
+  auto sync_all = [&]() {
+    if constexpr (C) return coh_barrier(bar, lw, G, target, err, spin_limit);
+    else return grid_barrier(bar, G, target, err, spin_limit);
+  };
   const int k = L.ctrl[LZ_K];
   if (k >= L.m) return;  // every basis column saved: no step, Q[:, m] unchanged
   if (lw == 0 && t == 0 && r < L.nbar) {
@@ -902,28 +1431,44 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
     double beta = -1.0;  // |w_{i-1}| when known here (no breakdown)
     if (i > k) {
       // beta_{i-1} and the breakdown restart vector
-      if (lz_record_beta(L, i, lw == 0, red, &beta)) {
+      if (lz_record_beta<T, M>(L, i, lw == 0, red, &beta)) {
         beta = -1.0;
         for (int sub = 0; sub < 3 && ok; ++sub) {
-          for (int tl = lw; tl < L.ntiles; tl += G) breakdown_sub(L, tl, i, sub, red);
-          ok = grid_barrier(bar, G, target, err, spin_limit);
+          if constexpr (C) breakdown_sub_lds<T, C>(L, X, lw, i, sub, red);
+          else
+            for (int tl = lw; tl < L.ntiles; tl += G) breakdown_sub(L, tl, i, sub, red);
+          ok = sync_all();
         }
         if (!ok) break;
       }
     }
     if (i == L.m) {
       for (int tl = lw; tl < L.ntiles; tl += G)
-        phaseA<T, C, MAXC, RW>(L, tile_of(tl), tl, i, false, 3, red, beta);
+        phaseA<T, C, MAXC, RW, M>(L, tile_of(tl), tl, i, false, 3, red, beta);
       break;
     }
     for (int tl = lw; tl < L.ntiles; tl += G)
-      phaseA<T, C, MAXC, RW>(L, tile_of(tl), tl, i, i == k, 0, red, beta);
-    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC, RW>(L, tile_of(tl), tl, i, 0, red);
-    if (!(ok = grid_barrier(bar, G, target, err, spin_limit))) break;
-    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC, RW>(L, tile_of(tl), tl, i, 1, red);
-    ok = grid_barrier(bar, G, target, err, spin_limit);
+      phaseA<T, C, MAXC, RW, M>(L, tile_of(tl), tl, i, i == k, 0, red, beta);
+    if (!(ok = sync_all())) break;
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC, RW, M>(L, tile_of(tl), tl, i, 0, red);
+    if (!(ok = sync_all())) break;
+    for (int tl = lw; tl < L.ntiles; tl += G) phaseBC<T, C, MAXC, RW, M>(L, tile_of(tl), tl, i, 1, red);
+    ok = sync_all();
   }
+}
+
+template <typename T, bool C, int MAXC, bool RW>
+__global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __restrict__ insts,
+                                                           const int2* __restrict__ wgmap, int r,
+                                                           unsigned long long spin_limit) {
+  extern __shared__ double lds[];
+  const int2 mp = wgmap[blockIdx.x];
+  const LzInst<T>& L = insts[mp.x];
+  if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;  // uniform over the instance's workgroups
+  if (!RW && !L.heavy && L.tile <= USBS_NT)
+    rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds);  // the instance read in place
+  else
+    tile_steps<T, C, MAXC, RW>(L, mp.y, r, spin_limit, lds);  // a copy in registers
 }
 
 template <typename T, int MAXC>
@@ -978,7 +1523,9 @@ template <typename T>
 size_t lanczos_cache_bytes(const LzInst<T>& L) {
   if (L.nblocks < L.ntiles) return 0;  // a workgroup would own several tiles
   if (L.tile > 256) return 0;          // row-wise tiles stream Q from memory
-  const size_t b = lz_cache_bytes<T>(L.tile, L.m, L.tile_nnz_max);
+  // RR instances keep Q in registers: only the CSR slice is cached
+  const size_t b = L.heavy ? lz_cache_bytes<T>(L.tile, L.m, L.tile_nnz_max)
+                           : rr_res_lds_bytes<T>(L.tile, L.m, L.tile_nnz_max);
   return b <= LZ_CACHE_MAX_BYTES ? b : 0;
 }
 template size_t lanczos_cache_bytes<double>(const LzInst<double>&);
@@ -986,8 +1533,8 @@ template size_t lanczos_cache_bytes<float>(const LzInst<float>&);
 
 int lanczos_prefers_persistent(int tile, int heavy, int m, int tile_nnz_max, int value_bytes) {
   if (heavy || tile > 256) return 0;
-  const size_t b = value_bytes == 4 ? lz_cache_bytes<float>(tile, m, tile_nnz_max)
-                                    : lz_cache_bytes<double>(tile, m, tile_nnz_max);
+  const size_t b = value_bytes == 4 ? rr_res_lds_bytes<float>(tile, m, tile_nnz_max)
+                                    : rr_res_lds_bytes<double>(tile, m, tile_nnz_max);
   return b <= LZ_CACHE_MAX_BYTES ? 1 : 0;
 }
 
@@ -1029,8 +1576,11 @@ static const void* coop_kernel(int m, bool cache, bool rowwise) {
 template <typename T>
 int lanczos_max_coresident_blocks(int m, bool cache, bool rowwise, size_t lds_bytes, int device) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, coop_kernel<T>(m, cache, rowwise), USBS_NT,
-                                                   lds_bytes) != hipSuccess)
+  const void* kern = coop_kernel<T>(m, cache, rowwise);
+  if (lds_bytes > 64 * 1024 &&
+      hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, USBS_NT, lds_bytes) != hipSuccess)
     return 0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;

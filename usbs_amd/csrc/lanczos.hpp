@@ -46,10 +46,11 @@ struct LzInst {
   double* beta;  // m
   double* Yr;    // m x m row-major: saved Ritz vectors (columns) of T
   int* ctrl;     // LZ_NCTRL
-  unsigned* bar; // barrier counters, one per restart (persistent launch)
+  unsigned* bar; // barrier counters of the persistent launches: nbar regions
+                 // of LZ_BAR_STRIDE, region r for restart r
   int* lstat;    // 2 x nbar: per persistent launch r, steps done and the sum
                  // over those steps of the valid column count i+1
-  int nbar;      // number of counters in bar (>= max_restarts)
+  int nbar;      // number of counter regions in bar (>= max_restarts)
   int tile_nnz_max;  // largest number of operator entries in one row tile
   int heavy;         // 1: a wavefront per operator row in the SpMV (rows of
                      // 32+ entries on average), else a thread per row
@@ -60,6 +61,14 @@ static_assert(sizeof(LzInst<double>) == 8 * 4 + 8 * 2 + 8 * 17 + 4 * 4, "LzInst 
 
 // Largest LDS block of a persistent workgroup that caches its tile.
 #define LZ_CACHE_MAX_BYTES (64 * 1024)
+// Barrier counters of one persistent launch (restart): a region of
+// LZ_BAR_STRIDE unsigned per restart, which holds grid_barrier's counter at
+// offset 0 or coh_barrier's eight group counters at 0, 64, ..., 448.
+#define LZ_BAR_STRIDE 512
+// Polls before a device-wide barrier wait gives up and records an error
+// (LZ_ERROR, reported as USBS_ERR_TIMEOUT): a few seconds, far above any
+// legitimate wait inside one restart.
+#define LZ_SPIN_LIMIT (1ull << 21)
 
 int tile_rows_for(int n);
 // Row tiles of the Lanczos steps for heavy-row operators (n <= 65536):

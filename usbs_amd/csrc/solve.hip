@@ -307,7 +307,7 @@ struct Solver {
       k_lz_start<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(L, q0);
       return hipGetLastError() != hipSuccess ||
              lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
-                                   d_map + 2 * nb, lz.ntiles, 1ull << 26, cache_bytes, nullptr,
+                                   d_map + 2 * nb, lz.ntiles, LZ_SPIN_LIMIT, cache_bytes, nullptr,
                                    lz.tile > 256 ? 1 : 0, &stop);
     };
     do {
@@ -373,7 +373,7 @@ struct Solver {
       int launched = 0;
       const LzStop stop{&ctrl_ptr, lz_hint, &launched};
       CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
-                               d_map + 2 * nb, lz.ntiles, 1ull << 26, cache_bytes,
+                               d_map + 2 * nb, lz.ntiles, LZ_SPIN_LIMIT, cache_bytes,
                                tev.empty() ? nullptr : tev.data(), lz.tile > 256 ? 1 : 0, &stop));
       tev_launched = launched;
       tev_pending = !tev.empty();

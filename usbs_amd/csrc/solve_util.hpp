@@ -53,7 +53,7 @@ __global__ void k_lz_start(LzInst<T> L, const double* q0) {
   if (i < L.n) L.Q[i] = (T)q0[i];
   if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
   if (i < LZ_NCTRL) L.ctrl[i] = 0;
-  for (long long x = i; x < L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
+  for (long long x = i; x < (long long)LZ_BAR_STRIDE * L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
   for (long long x = i; x < 2 * L.nbar; x += (long long)gridDim.x * blockDim.x) L.lstat[x] = 0;
 }
 
