@@ -654,7 +654,7 @@ def main():
                     help="time-to-gap legs: iteration cap")
     ap.add_argument("--gap-max-time", type=float, default=60.0,
                     help="time-to-gap legs: seconds cap per solve")
-    ap.add_argument("--gap-batch", type=int, default=4,
+    ap.add_argument("--gap-batch", type=int, default=8,
                     help="torus default run: problems in the batched time-to-gap leg (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 20 device-to-device copies of MIB MiB (the reference kernel "
@@ -687,8 +687,11 @@ def main():
         kron_bench(args)
         return
     if args.workload == "torus":
-        # HIP's default of 4 hardware queues: with 8, the solves of the batched
-        # leg were starved unevenly (one solve 2.4x slower than with 4)
+        # 16 hardware queues (HIP's default is 4): the batched legs run two
+        # streams per solve, and streams that share a queue serialise (eight
+        # solves in lockstep: 235 it/s with 4 queues, 375 with 16).  Set
+        # before the first HIP call of the process.
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         gap_bench(args)
         return
 
