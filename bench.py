@@ -631,6 +631,9 @@ def main():
                     help="tune: time the first Lanczos call in both forms and keep the faster; "
                          "auto: persistent where row tiles stay in LDS and fit one per CU, else per-op")
     ap.add_argument("--no-fp64-leg", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--throughput-only", action="store_true",
+                    help="torus: only the throughput leg with its roofline (no time-to-gap, "
+                         "batched or serving legs; the form profiled under rocprofv3)")
     ap.add_argument("--workload", default="torus",
                     choices=["torus", "rgg", "qap", "qap-kron", "serving", "torus-batch"],
                     help="torus 100x200 (default), RGG 2^20, QAP l=48, the QAP Kronecker "
@@ -686,7 +689,7 @@ def main():
     if args.workload == "qap-kron":
         kron_bench(args)
         return
-    if args.workload == "torus":
+    if args.workload == "torus" and not args.throughput_only:
         # 16 hardware queues (HIP's default is 4): the batched legs run two
         # streams per solve, and streams that share a queue serialise (eight
         # solves in lockstep: 235 it/s with 4 queues, 375 with 16).  Set
