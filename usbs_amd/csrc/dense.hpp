@@ -327,29 +327,45 @@ __device__ __forceinline__ void dense_svec_inv(const double* v, int k, double* M
 }
 
 // ---------------------------------------------------------------------------
-// Symmetric eigendecomposition for m <= 32 by Householder tridiagonalisation
-// (the reflectors accumulated into Q as they are formed), multisection on
-// Sturm counts for every eigenvalue, eigenvectors of the tridiagonal from
-// twisted factorisations, Gram-Schmidt (two passes) inside clusters
-// (consecutive eigenvalues closer than 1e-3 |T|), and Y = Q Z.  Eigenvalues
-// come out in ascending order by construction; each is bracketed to
-// max(2 eps |lambda|, eps |T|).  The code is written for latency: small
-// reductions are done redundantly by every thread from LDS broadcasts
-// rather than with cross-lane shuffles, and the Sturm and twisted-
-// factorisation chains run on a register copy of the tridiagonal with a
-// Newton-refined reciprocal.
-//   A  : m x lda in LDS (lda >= m), symmetric; destroyed.
-//   Qw : m x lda LDS scratch (the accumulated orthogonal factor).
-//   w  : m eigenvalues ascending;  Y: m x m row-major, column j = eigenvector
-//        of w[j].
-//   scr: tridiag_scratch_doubles(m, blockDim.x) doubles of LDS.
-// Called by all threads of the block (blockDim 256).  Returns 1 (in every
-// thread) when a cluster is degenerate -- a vector loses more than half its
-// norm to the Gram-Schmidt projection, as for an exactly repeated eigenvalue
-// -- and the caller must use another eigensolver; 0 otherwise.
+// Eigendecomposition of the thick-restart arrowhead T of a Lanczos restart
+// (m <= 32): diagonal alpha, couplings beta[a] between rows k and a for
+// a < k (the arrow) and between a and a + 1 for a >= k (the Krylov tail);
+// k >= m means T = diag(alpha).
+//
+// T is taken in the row order P = (m-1, m-2, ..., k, 0, 1, ..., k-1): the
+// tail, reversed, comes first and is already tridiagonal, and the arrow is
+// the trailing (k+1) x (k+1) block B = [alpha_k b^T; b diag(alpha_0..k-1)].
+// Only B is tridiagonalised, by Householder reflectors that map column s of
+// the trailing block to (mu, 0, ...) and are accumulated into the k x k
+// factor Qk.  Each step is computed by all four wavefronts at once: the
+// reflector, p = tau B22 v, K and w redundantly in every wavefront (identical
+// arithmetic, so identical values), then each wavefront updates its own rows
+// (row mod 4) of B, written into the other of two LDS copies, and of Qk in
+// place -- one workgroup barrier per step.
+//
+// Eigenvalues of the tridiagonal: multisection on Sturm counts, 8 points per
+// eigenvalue per round, eight eigenvalues per wavefront, each wavefront
+// iterating on its own (ballots, no workgroup barrier) until its intervals
+// are below max(2 eps |lambda|, eps |T|).  Eigenvectors: twisted
+// factorisations (both Sturm chains of an eigenvalue in one thread, in
+// registers), Gram-Schmidt (two passes) inside clusters of eigenvalues closer
+// than 1e-3 |T|, then Y = P^T blockdiag(I, Qk) Z.
+//   w  : m eigenvalues, ascending (LDS);  Y: m x m row-major (LDS), column j
+//        = eigenvector of w[j] in T's own row order.
+//   alpha, beta: T's coefficients (read only).
+//   scr: arrow_scratch_doubles(m) doubles of LDS.
+// Called by all 256 threads of the block.  Returns 1 (in every thread) when
+// a cluster is degenerate -- a vector loses more than half its norm to the
+// Gram-Schmidt projection, as for an exactly repeated eigenvalue -- and the
+// caller must use another eigensolver; 0 otherwise.
 #define TD_MAXM 32
-__host__ __device__ constexpr int tridiag_scratch_doubles(int m, int nt) {
-  return 9 * m + 64 + (nt + 1) / 2 + 3 * m * m;
+#define AB_LD (TD_MAXM + 1)
+// Each B copy is AB_ROWS x AB_BLD: the unrolled reads of a step run up to 31
+// rows and columns past the trailing block's corner, into zeros.
+#define AB_ROWS (2 * TD_MAXM)
+#define AB_BLD (2 * TD_MAXM)
+__host__ __device__ constexpr int arrow_scratch_doubles(int m) {
+  return 2 * AB_ROWS * AB_BLD + (TD_MAXM + 1) * AB_LD + 4 * 3 * TD_MAXM + m * m + 3 * m + 64;
 }
 
 __device__ __forceinline__ double td_guard(double q, double pivmin) {
@@ -379,268 +395,374 @@ __device__ __forceinline__ int td_sturm(const double (&dr)[TD_MAXM], const doubl
   return c;
 }
 
-__device__ int tridiag_eigh(double* A, int lda, int m, double* w, double* Y, double* Qw,
-                            double* scr) {
+// Sum / min / max over the wavefront, the same value in every lane: a DPP
+// butterfly inside each row of 16 lanes (each level combines the same two
+// numbers in both lanes of a pair), then the four row results read from
+// lanes 0, 16, 32, 48 and combined in a fixed order.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += lane_xor<1>(v);
+  v += lane_xor<2>(v);
+  v += lane_xor<4>(v);
+  v += lane_xor<8>(v);
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+__device__ __forceinline__ double wave_min_dpp(double v) {
+  v = fmin(v, lane_xor<1>(v));
+  v = fmin(v, lane_xor<2>(v));
+  v = fmin(v, lane_xor<4>(v));
+  v = fmin(v, lane_xor<8>(v));
+  return fmin(fmin(readlane_d(v, 0), readlane_d(v, 16)), fmin(readlane_d(v, 32), readlane_d(v, 48)));
+}
+__device__ __forceinline__ double wave_max_dpp(double v) {
+  v = fmax(v, lane_xor<1>(v));
+  v = fmax(v, lane_xor<2>(v));
+  v = fmax(v, lane_xor<4>(v));
+  v = fmax(v, lane_xor<8>(v));
+  return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
+}
+
+__device__ __forceinline__ int arrow_eigh(int m, int k, const double* __restrict__ alpha,
+                                          const double* __restrict__ beta, double* w, double* Y,
+                                          double* scr) {
   constexpr int MT = TD_MAXM;
-  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63;
-  double* d = scr;           // diagonal
-  double* e = d + m;         // e[i] couples i and i + 1
-  double* e2 = e + m;
-  double* hv = e2 + m;       // current reflector, by absolute row (hv[j+1] = 1)
-  double* p = hv + m;        // A22 v by absolute row, then Gram-Schmidt coefficients
-  double* lo = p + m;
-  double* hi = lo + m;
-  double* misc = hi + m;     // 2m + 64 doubles: [0] tnorm, [1] pivmin, [2] tau, [8..8+m) Q v,
-                             // [8+m..8+2m) p*v
-  int* cnt = (int*)(misc + 2 * m + 64);                  // nt ints
-  double* dp = misc + 2 * m + 64 + (nt + 1) / 2;         // m x m: D+ of eigenvector j (row j)
-  double* dm = dp + m * m;                               // m x m: D-
-  double* Z = dm + m * m;                                // m x m: eigenvectors of the tridiagonal
-  double* qv = misc + 8;
-  double* pvp = qv + m;     // p_r v_r of the current step
-  for (int idx = t; idx < m * m; idx += nt) {
-    const int r = idx / m, c = idx - (idx / m) * m;
-    Qw[r * lda + c] = r == c ? 1.0 : 0.0;
-  }
-  // 1. Householder: step j maps A[j+1.., j] to (|x|, 0, ...), Q <- Q H_j
-  for (int j = 0; j + 2 < m; ++j) {
-    const int n1 = m - j - 1;
-    __syncthreads();
-    if (t < 64) {
-      const double xi = lane < n1 ? A[(j + 1 + lane) * lda + j] : 0.0;
-      const double sg = wave_sum((lane >= 1 && lane < n1) ? xi * xi : 0.0);
-      const double x0 = __shfl(xi, 0, 64);
-      double v0 = 1.0, mu = x0, tj = 0.0;
-      if (sg != 0.0) {
-        mu = sqrt(x0 * x0 + sg);
-        v0 = x0 <= 0.0 ? x0 - mu : -sg / (x0 + mu);
-        tj = 2.0 * v0 * v0 / (sg + v0 * v0);
-      }
-      if (lane < n1) hv[j + 1 + lane] = lane == 0 ? 1.0 : (sg != 0.0 ? xi / v0 : 0.0);
-      if (lane == 0) {
-        misc[2] = tj;
-        e[j] = mu;
-        d[j] = A[j * lda + j];
-      }
-    }
-    __syncthreads();
-    const double tj = misc[2];
-    if (tj == 0.0) continue;  // uniform
-    {
-      // p = tau A22 v: 8 threads per row
-      const int g = t >> 3, sub = t & 7;
-      const bool isp = g < n1;
-      double s = 0.0;
-      if (isp)
-        for (int c = sub; c < n1; c += 8) s = fma(A[(j + 1 + g) * lda + j + 1 + c], hv[j + 1 + c], s);
-      s += __shfl_xor(s, 4, 8);
-      s += __shfl_xor(s, 2, 8);
-      s += __shfl_xor(s, 1, 8);
-      if (isp && sub == 0) {
-        p[j + 1 + g] = tj * s;
-        pvp[g] = tj * s * hv[j + 1 + g];
-      }
-    }
-    {
-      // (Q v)_i: 8 threads per row of Q
-      const int g = t >> 3, sub = t & 7;
-      double s = 0.0;
-      if (g < m)
-        for (int c = sub; c < n1; c += 8) s = fma(Qw[g * lda + j + 1 + c], hv[j + 1 + c], s);
-      s += __shfl_xor(s, 4, 8);
-      s += __shfl_xor(s, 2, 8);
-      s += __shfl_xor(s, 1, 8);
-      if (g < m && sub == 0) qv[g] = s;
-    }
-    __syncthreads();
-    double K0 = 0.0, K1 = 0.0, K2 = 0.0, K3 = 0.0;
+  constexpr int P = 8;  // multisection points per eigenvalue and round
+  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wv = t >> 6;
+  const int kp = k < m ? k : m - 1;  // arrow size of the reordered T
+  const int j0 = m - kp - 1;         // reordered row of T's row k (first row of B)
+  double* Bb = scr;                          // two copies of B, AB_ROWS x AB_BLD each
+  double* Qk = scr + 2 * AB_ROWS * AB_BLD;   // kp x kp, row stride AB_LD, zero pad row
+  double* bc = Qk + (TD_MAXM + 1) * AB_LD;   // per wavefront: hv, w, Qk v (32 each)
+  double* Z = bc + 4 * 3 * TD_MAXM;          // m x m: eigenvectors of the tridiagonal
+  double* d = Z + m * m;             // diagonal
+  double* e = d + m;                 // e[p] couples p and p + 1
+  double* misc = e + m;              // [0] degenerate flag
+
+  // B (local index 0 = T's row k, 1 + a = T's row a) in copy 0, zeros
+  // elsewhere in both copies; Qk = I with zeros around it
+  for (int i2 = t; i2 < AB_ROWS * AB_BLD; i2 += nt) {  // pairs of doubles
+    const int idx = 2 * i2;
+    const int r = (idx / AB_BLD) & (AB_ROWS - 1), c = idx & (AB_BLD - 1);
+    const bool first = idx < AB_ROWS * AB_BLD;
+    double v[2] = {0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < MT; c += 4) {
-      K0 += c < n1 ? pvp[c < n1 ? c : 0] : 0.0;
-      K1 += c + 1 < n1 ? pvp[c + 1 < n1 ? c + 1 : 0] : 0.0;
-      K2 += c + 2 < n1 ? pvp[c + 2 < n1 ? c + 2 : 0] : 0.0;
-      K3 += c + 3 < n1 ? pvp[c + 3 < n1 ? c + 3 : 0] : 0.0;
+    for (int u = 0; u < 2; ++u) {
+      const int cu = c + u;
+      if (first && r <= kp && cu <= kp) {
+        if (r == cu) v[u] = alpha[r == 0 ? kp : r - 1];
+        else if (k < m && (r == 0 || cu == 0)) v[u] = beta[(r == 0 ? cu : r) - 1];
+      }
     }
-    const double K = 0.5 * tj * ((K0 + K1) + (K2 + K3));
-    for (int idx = t; idx < n1 * n1; idx += nt) {
-      const int r = j + 1 + idx / n1, c = j + 1 + idx - (idx / n1) * n1;
-      const double wr = p[r] - K * hv[r], wc = p[c] - K * hv[c];
-      A[r * lda + c] -= hv[r] * wc + wr * hv[c];
+    *reinterpret_cast<double2*>(Bb + idx) = make_double2(v[0], v[1]);
+  }
+  for (int idx = t; idx < (TD_MAXM + 1) * AB_LD; idx += nt) {
+    const int r = idx / AB_LD, c = idx - (idx / AB_LD) * AB_LD;
+    Qk[idx] = (r == c && r < kp) ? 1.0 : 0.0;
+  }
+  for (int p = t; p < j0; p += nt) {
+    d[p] = alpha[m - 1 - p];
+    e[p] = beta[m - 2 - p];
+  }
+  __syncthreads();
+
+  // 1. Householder on B: step s maps B[s+1.., s] to (mu, 0, ...).  Values a
+  // wavefront shares between its lanes (hv, w, Qk v) go through its own LDS
+  // slots: written, then read back as broadcasts by the same wavefront.
+  double* hvs = bc + wv * 3 * TD_MAXM;
+  double* wvs = hvs + TD_MAXM;
+  double* qvs = wvs + TD_MAXM;
+  int cur = 0;
+  for (int s = 0; s + 2 <= kp; ++s) {
+    const double* Bc = Bb + (cur ? AB_ROWS * AB_BLD : 0);
+    double* Bn = Bb + (cur ? 0 : AB_ROWS * AB_BLD);
+    const int n1 = kp - s;  // trailing block rows s+1 .. kp
+    // the reflector, in every lane from broadcast reads of row s (= column
+    // s; entries past kp are zero)
+    const double* xrow = Bc + s * AB_BLD + s + 1;
+    double xs[TD_MAXM];
+#pragma unroll
+    for (int c = 0; c < TD_MAXM; ++c) xs[c] = xrow[c];
+    asm volatile("" ::: "memory");
+    double sg0 = 0.0, sg1 = 0.0, sg2 = 0.0, sg3 = 0.0;
+#pragma unroll
+    for (int c = 0; c < TD_MAXM; c += 4) {
+      if (c > 0) sg0 = fma(xs[c], xs[c], sg0);
+      sg1 = fma(xs[c + 1], xs[c + 1], sg1);
+      sg2 = fma(xs[c + 2], xs[c + 2], sg2);
+      sg3 = fma(xs[c + 3], xs[c + 3], sg3);
     }
-    for (int idx = t; idx < m * n1; idx += nt) {
-      const int i = idx / n1, c = j + 1 + idx - (idx / n1) * n1;
-      Qw[i * lda + c] -= tj * qv[i] * hv[c];
+    const double sg = (sg0 + sg1) + (sg2 + sg3);
+    const double x0 = xs[0];
+    double v0 = 1.0, mu = x0, tj = 0.0;
+    if (sg != 0.0) {
+      mu = sqrt(x0 * x0 + sg);
+      v0 = x0 <= 0.0 ? x0 - mu : -sg / (x0 + mu);
+      tj = 2.0 * v0 * v0 / (sg + v0 * v0);
+    }
+    if (t == 0) {
+      d[j0 + s] = Bc[s * AB_BLD + s];
+      e[j0 + s] = mu;
+    }
+    if (tj == 0.0) continue;  // uniform: B and Qk unchanged
+    const double rv = 1.0 / v0;
+    double hb[TD_MAXM];
+    hb[0] = 1.0;
+#pragma unroll
+    for (int c = 1; c < TD_MAXM; ++c) hb[c] = xs[c] * rv;  // 0 past n1
+    const double hv = lane < n1 ? xrow[lane] * rv : 0.0;
+    const double hvl = lane == 0 ? 1.0 : hv;
+    if (lane < TD_MAXM) hvs[lane] = hvl;
+    // p = tau B22 v (lane = column; B symmetric, so column l = row l; rows
+    // past kp are zero)
+    const double* bcol = Bc + (s + 1) * AB_BLD + s + 1 + (lane < n1 ? lane : 0);
+    double bv[TD_MAXM];
+#pragma unroll
+    for (int c = 0; c < TD_MAXM; ++c) bv[c] = bcol[c * AB_BLD];
+    asm volatile("" ::: "memory");  // every load issued before the first FMA waits
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+#pragma unroll
+    for (int c = 0; c < TD_MAXM; c += 4) {
+      acc0 = fma(bv[c], hb[c], acc0);
+      acc1 = fma(bv[c + 1], hb[c + 1], acc1);
+      acc2 = fma(bv[c + 2], hb[c + 2], acc2);
+      acc3 = fma(bv[c + 3], hb[c + 3], acc3);
+    }
+    const double p = lane < n1 ? tj * ((acc0 + acc1) + (acc2 + acc3)) : 0.0;
+    const double K = 0.5 * tj * wave_sum_dpp(p * hvl);
+    const double wl = p - K * hvl;
+    // this wavefront's rows of Qk: qv = Qk v (lane l < 8: row wv + 4 l); reads
+    // past column kp - 1 meet zeros of hb
+    const int qr = wv + 4 * lane;
+    if (lane < 8) {
+      const double* qrow = Qk + (qr < kp ? qr : 0) * AB_LD + s;
+      double qa[TD_MAXM];
+#pragma unroll
+      for (int c = 0; c < TD_MAXM; ++c) qa[c] = qrow[c];
+      asm volatile("" ::: "memory");
+      double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+#pragma unroll
+      for (int c = 0; c < TD_MAXM; c += 4) {
+        q0 = fma(qa[c], hb[c], q0);
+        q1 = fma(qa[c + 1], hb[c + 1], q1);
+        q2 = fma(qa[c + 2], hb[c + 2], q2);
+        q3 = fma(qa[c + 3], hb[c + 3], q3);
+      }
+      qvs[lane] = (q0 + q1) + (q2 + q3);
+    }
+    if (lane < TD_MAXM) wvs[lane] = wl;
+    __builtin_amdgcn_wave_barrier();
+    // rank-2 update of this wavefront's rows into Bn, Qk -= tau (Qk v) v^T:
+    // two rows per pass, lane & 31 = column
+    const int col = lane & 31, half = lane >> 5;
+    const double hvc = hvs[col], wc = wvs[col];
+    double hr[4], wr[4], qq[4], bo[4], qo[4];
+#pragma unroll
+    for (int ps = 0; ps < 4; ++ps) {
+      const int ri = 2 * ps + half;  // this wavefront's ri-th row
+      const int r = wv + 4 * ri;     // trailing-block row
+      hr[ps] = hvs[r];
+      wr[ps] = wvs[r];
+      qq[ps] = qvs[ri];
+      bo[ps] = Bc[(s + 1 + r) * AB_BLD + s + 1 + col];  // zero rows / columns past kp
+      qo[ps] = Qk[(r < kp ? r : 0) * AB_LD + s + col];
+    }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int ps = 0; ps < 4; ++ps) {
+      const int r = wv + 4 * (2 * ps + half);
+      if (r < n1 && col < n1) Bn[(s + 1 + r) * AB_BLD + s + 1 + col] = bo[ps] - (hr[ps] * wc + wr[ps] * hvc);
+      if (r < kp && col < n1) Qk[r * AB_LD + s + col] = qo[ps] - tj * qq[ps] * hvc;
+    }
+    cur ^= 1;
+    __syncthreads();
+  }
+  {
+    const double* Bf = Bb + (cur ? AB_ROWS * AB_BLD : 0);
+    if (t == 0) {
+      if (kp >= 1) {
+        d[m - 2] = Bf[(kp - 1) * AB_BLD + kp - 1];
+        e[m - 2] = Bf[kp * AB_BLD + kp - 1];
+      }
+      d[m - 1] = Bf[kp * AB_BLD + kp];
+      e[m - 1] = 0.0;
     }
   }
   __syncthreads();
-  if (t == 0) {
-    if (m >= 2) {
-      d[m - 2] = A[(m - 2) * lda + m - 2];
-      e[m - 2] = A[(m - 1) * lda + m - 2];
-    }
-    d[m - 1] = A[(m - 1) * lda + m - 1];
-    e[m - 1] = 0.0;
-    double gl = 0.0, gu = 0.0, emax = 0.0;
-    for (int i = 0; i < m; ++i) {
-      const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < m ? fabs(e[i]) : 0.0);
-      gl = (i == 0 || d[i] - r < gl) ? d[i] - r : gl;
-      gu = (i == 0 || d[i] + r > gu) ? d[i] + r : gu;
-      e2[i] = e[i] * e[i];
-      emax = e2[i] > emax ? e2[i] : emax;
-    }
-    const double tnorm = fmax(fabs(gl), fabs(gu));
-    const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
-    const double fudge = 2.1 * 2.220446049250313e-16 * tnorm * m + 4.0 * pivmin;
-    misc[0] = tnorm;
-    misc[1] = pivmin;
-    for (int i = 0; i < m; ++i) {
-      lo[i] = gl - fudge;
-      hi[i] = gu + fudge;
-    }
-  }
-  __syncthreads();
-  const double tnorm = misc[0], pivmin = misc[1];
+
+  // 2. Gershgorin bounds (every wavefront), register copy of the tridiagonal
   const double eps = 2.220446049250313e-16;
+  double gl, gu, emax;
+  {
+    const bool in = lane < m;
+    const double di = in ? d[lane] : 0.0, ei = in ? e[lane] : 0.0;
+    const double em = (in && lane > 0) ? e[lane - 1] : 0.0;
+    const double rr = fabs(em) + fabs(ei);
+    gl = wave_min_dpp(in ? di - rr : INFINITY);
+    gu = wave_max_dpp(in ? di + rr : -INFINITY);
+    emax = wave_max_dpp(ei * ei);
+  }
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+  const double fudge = 2.1 * eps * tnorm * m + 4.0 * pivmin;
   double dr[MT], e2r[MT];
 #pragma unroll
   for (int i = 0; i < MT; ++i) {
     dr[i] = i < m ? d[i] : 0.0;
-    e2r[i] = i < m ? e2[i] : 0.0;
+    const double ev = i < m ? e[i] : 0.0;
+    e2r[i] = ev * ev;
   }
-  // 2. every eigenvalue by multisection: P points per eigenvalue per round
+
+  // 3. eigenvalue jj = 8 wv + (lane >> 3), point lane & 7
   {
-    int P = nt / m;
-    if (P > 32) P = 32;
-    const int jj = t / P, pp = t - (t / P) * P;
+    const int g = lane >> 3, pp = lane & 7, jj = 8 * wv + g;
+    double lo = gl - fudge, hi = gu + fudge;
     for (int it = 0; it < 80; ++it) {
-      if (jj < m) {
-        const double l0 = lo[jj], h0 = hi[jj];
-        const double h = (h0 - l0) / (P + 1);
-        cnt[t] = td_sturm(dr, e2r, m, l0 + (pp + 1) * h, pivmin);
-      }
-      __syncthreads();
-      int open = 0;
-      if (t < m) {
-        const double l0 = lo[t], h0 = hi[t];
-        const double h = (h0 - l0) / (P + 1);
-        int ps = P;
-        for (int q = 0; q < P; ++q)
-          if (cnt[t * P + q] > t) { ps = q; break; }
-        const double nl = ps == 0 ? l0 : l0 + ps * h;
-        const double nh = ps == P ? h0 : l0 + (ps + 1) * h;
-        lo[t] = nl;
-        hi[t] = nh;
-        const double tol = fmax(2.0 * eps * fmax(fabs(nl), fabs(nh)), eps * tnorm);
-        open = (nh - nl > tol && nh - nl < h0 - l0) ? 1 : 0;
-      }
-      if (!__syncthreads_or(open)) break;
+      const double h = (hi - lo) / (P + 1);
+      const int cnt = jj < m ? td_sturm(dr, e2r, m, lo + (pp + 1) * h, pivmin) : 0;
+      const unsigned long long bal = __ballot(cnt > jj);
+      const unsigned seg = (unsigned)(bal >> (8 * g)) & 0xFFu;
+      const int ps = seg ? __builtin_ctz(seg) : P;
+      const double nl = ps == 0 ? lo : lo + ps * h;
+      const double nh = ps == P ? hi : lo + (ps + 1) * h;
+      const double tol = fmax(2.0 * eps * fmax(fabs(nl), fabs(nh)), eps * tnorm);
+      const bool open = jj < m && nh - nl > tol && nh - nl < hi - lo;
+      lo = nl;
+      hi = nh;
+      if (!__any(open)) break;
     }
-    if (t < m) w[t] = 0.5 * (lo[t] + hi[t]);
+    if (jj < m && pp == 0) w[jj] = 0.5 * (lo + hi);
   }
   __syncthreads();
-  // 3. eigenvectors of the tridiagonal: D+ (threads < m) and D- (64..64+m)
+
+  // 4. eigenvectors of the tridiagonal: thread j < m, both chains in registers
   const double pv = fmax(pivmin, eps * (tnorm > 0.0 ? tnorm : 1.0));
   if (t < m) {
     const double lam = w[t];
-    double* q = dp + t * m;
-    double qq = td_guard(dr[0] - lam, pv);
-    q[0] = qq;
+    double qp[MT], qm[MT];
+    {
+      double qq = td_guard(dr[0] - lam, pv);
+      qp[0] = qq;
 #pragma unroll
-    for (int i = 1; i < MT; ++i) {
-      const double qn = td_guard((dr[i] - lam) - e2r[i - 1] * td_rcp(qq), pv);
-      if (i < m) q[i] = qn;
-      qq = i < m ? qn : qq;
+      for (int i = 1; i < MT; ++i) {
+        const double qn = td_guard((dr[i] - lam) - e2r[i - 1] * td_rcp(qq), pv);
+        qp[i] = qn;
+        qq = i < m ? qn : qq;
+      }
+      double qb = 0.0;
+#pragma unroll
+      for (int i = MT - 1; i >= 0; --i) {
+        const double nb2 = (i == m - 1) ? td_guard(dr[i] - lam, pv)
+                                        : td_guard((dr[i] - lam) - e2r[i] * td_rcp(qb), pv);
+        qb = i <= m - 1 ? nb2 : qb;
+        qm[i] = qb;
+      }
     }
-  } else if (t >= 64 && t < 64 + m) {
-    // backward chain on the reversed index k = m - 1 - i, read from LDS
-    const int j = t - 64;
-    const double lam = w[j];
-    double* q = dm + j * m;
-    double qq = td_guard(d[m - 1] - lam, pv);
-    q[m - 1] = qq;
-    for (int i = m - 2; i >= 0; --i) {
-      qq = td_guard((d[i] - lam) - e2[i] * td_rcp(qq), pv);
-      q[i] = qq;
-    }
-  }
-  __syncthreads();
-  if (t < m) {
-    const int j = t;
-    const double lam = w[j];
-    const double* qp = dp + j * m;
-    const double* qm = dm + j * m;
     int r = 0;
     double best = 0.0;
-    for (int i = 0; i < m; ++i) {
-      const double g = fabs(qp[i] + qm[i] - (d[i] - lam));
-      if (i == 0 || g < best) { best = g; r = i; }
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      const double gg = fabs(qp[i] + qm[i] - (dr[i] - lam));
+      if (i < m && (i == 0 || gg < best)) {
+        best = gg;
+        r = i;
+      }
     }
-    Z[r * m + j] = 1.0;
-    double s = 1.0, z = 1.0;
-    for (int i = r - 1; i >= 0; --i) {
-      z *= -e[i] * td_rcp(qp[i]);
-      Z[i * m + j] = z;
-      s = fma(z, z, s);
+    // z_r = 1; downwards z_i = -e_i / qp_i z_{i+1}; upwards z_i = -e_{i-1} / qm_i z_{i-1}
+    double zz = 1.0;
+#pragma unroll
+    for (int i = MT - 1; i >= 0; --i) {
+      if (i < r) zz *= -e[i] * td_rcp(qp[i]);
+      qp[i] = zz;  // z_i for i <= r
     }
-    z = 1.0;
-    for (int i = r + 1; i < m; ++i) {
-      z *= -e[i - 1] * td_rcp(qm[i]);
-      Z[i * m + j] = z;
-      s = fma(z, z, s);
+    zz = 1.0;
+#pragma unroll
+    for (int i = 1; i < MT; ++i) {
+      if (i > r) zz *= -e[i - 1] * td_rcp(qm[i]);
+      qm[i] = zz;  // z_i for i > r
     }
-    const double inv = 1.0 / sqrt(s);
-    for (int i = 0; i < m; ++i) Z[i * m + j] *= inv;
+    double ss = 0.0;
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      const double z = i <= r ? qp[i] : qm[i];
+      if (i < m) ss = fma(z, z, ss);
+    }
+    const double inv = 1.0 / sqrt(ss);
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+      if (i < m) Z[i * m + t] = (i <= r ? qp[i] : qm[i]) * inv;
   }
   __syncthreads();
-  // 4. two Gram-Schmidt passes inside clusters, in ascending order
+
+  // 5. two Gram-Schmidt passes inside clusters, in ascending order
   const double ortol = fmax(1e-3 * tnorm, 2.2250738585072014e-308);
+  unsigned long long starts;
+  {
+    const bool in = lane < m;
+    const double wl2 = in ? w[lane] : 0.0, wp = (in && lane > 0) ? w[lane - 1] : 0.0;
+    starts = __ballot(in && (lane == 0 || !(wl2 - wp < ortol)));
+  }
   int degenerate = 0;
   for (int j = 1; j < m; ++j) {
-    int c0 = j;
-    while (c0 > 0 && w[c0] - w[c0 - 1] < ortol) --c0;
-    if (c0 == j) continue;
-    const int nb = j - c0;
+    const int c0 = 63 - __builtin_clzll(starts & ((2ull << j) - 1ull));
+    if (c0 == j) continue;  // uniform
+    const int nbk = j - c0;
     for (int pass = 0; pass < 2; ++pass) {
       {
         const int i = t >> 3, sub = t & 7;
-        double s = 0.0;
-        if (i < nb)
-          for (int row = sub; row < m; row += 8) s = fma(Z[row * m + c0 + i], Z[row * m + j], s);
-        s += __shfl_xor(s, 4, 8);
-        s += __shfl_xor(s, 2, 8);
-        s += __shfl_xor(s, 1, 8);
-        if (i < nb && sub == 0) p[i] = s;
+        double sacc = 0.0;
+        if (i < nbk)
+          for (int row = sub; row < m; row += 8) sacc = fma(Z[row * m + c0 + i], Z[row * m + j], sacc);
+        sacc += __shfl_xor(sacc, 4, 8);
+        sacc += __shfl_xor(sacc, 2, 8);
+        sacc += __shfl_xor(sacc, 1, 8);
+        if (i < nbk && sub == 0) misc[8 + i] = sacc;
       }
       __syncthreads();
       for (int row = t; row < m; row += nt) {
-        double s = 0.0;
-        for (int i = 0; i < nb; ++i) s = fma(p[i], Z[row * m + c0 + i], s);
-        Z[row * m + j] -= s;
+        double sacc = 0.0;
+        for (int i = 0; i < nbk; ++i) sacc = fma(misc[8 + i], Z[row * m + c0 + i], sacc);
+        Z[row * m + j] -= sacc;
       }
       __syncthreads();
     }
     if (t < 64) {
       const double z = lane < m ? Z[lane * m + j] : 0.0;
-      const double s = wave_sum(z * z);
-      if (lane < m) Z[lane * m + j] = z / sqrt(s);
-      if (lane == 0) misc[3] = (s >= 0.25) ? 0.0 : 1.0;
+      const double sn = wave_sum(z * z);
+      if (lane < m) Z[lane * m + j] = z / sqrt(sn);
+      if (lane == 0) misc[0] = (sn >= 0.25) ? 0.0 : 1.0;
     }
     __syncthreads();
-    if (misc[3] != 0.0) degenerate = 1;
+    if (misc[0] != 0.0) degenerate = 1;
     __syncthreads();
   }
   if (degenerate) return 1;
-  // 5. Y = Q Z
-  for (int idx = t; idx < m * m; idx += nt) {
-    const int r = idx / m, c = idx - (idx / m) * m;
-    const double* qr = Qw + r * lda;
-    double s0 = 0.0, s1 = 0.0;
+
+  // 6. Y = P^T blockdiag(I_{j0+1}, Qk) Z: reordered row p is T's row
+  //    m-1-p (p < m-kp) or p-(m-kp)
+  {
+    const int j = t & 31;
 #pragma unroll
-    for (int i = 0; i < MT; i += 2) {
-      s0 = fma(i < m ? qr[i] : 0.0, i < m ? Z[(i < m ? i : 0) * m + c] : 0.0, s0);
-      s1 = fma(i + 1 < m ? qr[i + 1] : 0.0, i + 1 < m ? Z[(i + 1 < m ? i + 1 : 0) * m + c] : 0.0, s1);
+    for (int it = 0; it < 4; ++it) {
+      const int p = (t >> 5) + 8 * it;
+      if (p < m && j < m) {
+        double y;
+        if (p <= j0) {
+          y = Z[p * m + j];
+        } else {
+          const double* qrow = Qk + (p - j0 - 1) * AB_LD;
+          const double* zc = Z + (j0 + 1) * m + j;
+          double y0 = 0.0, y1 = 0.0;
+          int c = 0;
+          for (; c + 2 <= kp; c += 2) {
+            y0 = fma(qrow[c], zc[c * m], y0);
+            y1 = fma(qrow[c + 1], zc[(c + 1) * m], y1);
+          }
+          if (c < kp) y0 = fma(qrow[c], zc[c * m], y0);
+          y = y0 + y1;
+        }
+        const int x = p < m - kp ? m - 1 - p : p - (m - kp);
+        Y[x * m + j] = y;
+      }
     }
-    Y[idx] = s0 + s1;
   }
   __syncthreads();
   return 0;

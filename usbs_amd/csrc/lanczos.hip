@@ -65,13 +65,17 @@ int lanczos_heavy_tile_rows(int n) {
 // through LDS (block_reduce_cols): that scratch is larger.
 #define LZ_BIG (USBS_RCH * USBS_NT + USBS_RCH * USBS_NPART + 64)
 
-// Ritz eigensolve: tridiagonalisation + multisection for m <= 32 (LZ_TRIDIAG_MAX),
-// parallel Jacobi above.
+// Ritz eigensolve: arrowhead tridiagonalisation + multisection for m <= 32
+// (LZ_TRIDIAG_MAX, arrow_eigh), parallel Jacobi above or on a degenerate
+// cluster.  LDS: Y, w, cs, red, the small ints, then either Jacobi's T, V and
+// pairing schedule or arrow_eigh's scratch (one after the other in time).
 #define LZ_TRIDIAG_MAX 32
 static inline size_t ritz_lds_bytes(int m) {
   const int M = m + (m & 1);
-  const size_t base = (size_t)(2 * M * (M + 1) + m * m + 2 * M + 16 + (8 + M + (M - 1) * M) / 2 + 16);
-  return sizeof(double) * (base + (m <= LZ_TRIDIAG_MAX ? tridiag_scratch_doubles(m, 256) : 0));
+  const size_t head = (size_t)(m * m + 2 * M + 16 + (8 + M) / 2);
+  const size_t jac = (size_t)(2 * M * (M + 1) + ((M - 1) * M + 1) / 2);
+  const size_t arr = m <= LZ_TRIDIAG_MAX ? (size_t)arrow_scratch_doubles(m) : 0;
+  return sizeof(double) * (head + (jac > arr ? jac : arr));
 }
 static inline size_t tile_lds_bytes(int maxc) { return sizeof(double) * (size_t)(LZ_BIG + maxc + 8); }
 
@@ -214,73 +218,75 @@ __device__ void lz_build_T(int m, int k, const double* alpha, const double* beta
 // ----------------------------------------------------------------------------
 // Ritz step (one block): arrowhead T, eigh, convergence count, saved set,
 // reorder, new alpha/beta and the rotation Yr (lanczos.py:128-168, 202-221).
+// The convergence count, the gamma-ratio choice and the saved-set order are
+// computed by wavefront 0, a lane per Ritz value (m <= 48 < 64), with the
+// loops' tie-breaking: first non-converged value, first NaN ratio, else the
+// first index of the largest ratio.
 __device__ __forceinline__ void lz_ritz(int m, int nd, double tol, double rtol, int max_restarts, double* alpha, double* beta,
                         double* Yr, int* ctrl, double* lds) {
-  const int t = threadIdx.x, nt = blockDim.x;
+  const int t = threadIdx.x, nt = blockDim.x, lane = t & 63;
   const int M = m + (m & 1), lda = M + 1;
-  double* Tw = lds;                 // M x lda working copy of T (destroyed)
-  double* Vw = Tw + M * lda;        // M x lda eigenvector accumulation
-  double* Y = Vw + M * lda;         // m x m sorted eigenvectors
+  double* Y = lds;                  // m x m sorted eigenvectors
   double* w = Y + m * m;            // m eigenvalues
   double* cs = w + M;               // M rotation coefficients
   double* red = cs + M;             // 16
   int* sh = (int*)(red + 16);       // small int scratch
   int* part = sh + 8;               // m ints: saved-set order
-  int* sched = part + M;            // (M-1) * M ints: Jacobi pairing
+  double* U = (double*)(part + M);  // Jacobi's Tw, Vw, sched, or arrow_eigh's scratch
+  double* Tw = U;                   // M x lda working copy of T (destroyed)
+  double* Vw = Tw + M * lda;        // M x lda eigenvector accumulation
+  int* sched = (int*)(Vw + M * lda);  // (M-1) * M ints: Jacobi pairing
   const int k = ctrl[LZ_K];
-  lz_build_T(m, k, alpha, beta, Tw, M, lda);
-  __syncthreads();
   bool jacobi = m > LZ_TRIDIAG_MAX;
-  if (!jacobi && tridiag_eigh(Tw, lda, m, w, Y, Vw, (double*)(sched + (M - 1) * M + 16))) {
-    // a degenerate cluster: rebuild T and take the Jacobi solver
+  if (!jacobi && arrow_eigh(m, k, alpha, beta, w, Y, U)) jacobi = true;  // degenerate cluster
+  if (jacobi) {
     lz_build_T(m, k, alpha, beta, Tw, M, lda);
     __syncthreads();
-    jacobi = true;
+    block_eigh(Tw, lda, m, w, Y, Vw, cs, sched, red);
   }
-  if (jacobi) block_eigh(Tw, lda, m, w, Y, Vw, cs, sched, red);
   const double bm = beta[m - 1];
-  if (t == 0) {
+  if (t < 64) {
+    const bool in = lane < m;
+    const double wj = in ? w[lane] : 0.0;
     // convergence test (lanczos.py:205-208); with rtol > 0 the threshold has
     // a floor relative to the spectrum's scale (the FP32 basis policy)
-    const double scale = fmax(fabs(w[0]), fabs(w[m - 1]));
+    const double scale = fmax(fabs(readlane_d(wj, 0)), fabs(readlane_d(wj, m - 1)));
     const double thr = (rtol > 0.0 && rtol * scale > tol) ? rtol * scale : tol;
-    int nc = m;
-    for (int j = 0; j < m; ++j) {
-      const double res = bm * fabs(Y[(m - 1) * m + j]);
-      if (!(res < thr)) { nc = j; break; }
-    }
+    const double res = bm * fabs(in ? Y[(m - 1) * m + lane] : 0.0);
+    const unsigned long long bad = __ballot(in && !(res < thr));
+    const int nc = bad ? __builtin_ctzll(bad) : m;
     int delta = m - nd;
     const int d2 = 2 * (m - nc) / 5;
     if (d2 < delta) delta = d2;
-    const double lt = w[nc < m ? nc : m - 1];
-    int kl = 0;
-    bool nan_seen = false;
-    double best = 0.0;
-    for (int j = 0; j < m; ++j) {
-      const int kr = j + delta;
-      double g = 0.0;
-      if (kr <= m) g = (w[j] - lt) / (w[kr < m ? kr : m - 1] - lt);
-      if (g != g) {
-        if (!nan_seen) { nan_seen = true; kl = j; }
-      } else if (!nan_seen && (j == 0 || g > best)) {
-        best = g;
-        kl = j;
-      }
+    const double lt = readlane_d(wj, nc < m ? nc : m - 1);
+    const int kr = lane + delta;
+    const double wkr = __shfl(wj, kr < m ? kr : m - 1, 64);
+    double g = 0.0;
+    if (kr <= m) g = (wj - lt) / (wkr - lt);
+    const unsigned long long nanm = __ballot(in && g != g);
+    int kl;
+    if (nanm) {
+      kl = __builtin_ctzll(nanm);
+    } else {
+      const double gmax = wave_max_dpp(in ? g : -INFINITY);
+      kl = __builtin_ctzll(__ballot(in && g == gmax));
     }
     const int kright = kl + delta;
-    int cnt = 0;
-    for (int j = 0; j < m; ++j)
-      if (kl >= j || kright < j) part[cnt++] = j;
-    const int knext = cnt;
-    for (int j = 0; j < m; ++j)
-      if (!(kl >= j || kright < j)) part[cnt++] = j;
-    sh[0] = knext;
-    sh[1] = nc;
-    const int restart = ctrl[LZ_RESTART] + 1;
-    ctrl[LZ_K] = knext;
-    ctrl[LZ_RESTART] = restart;
-    ctrl[LZ_NCONV] = nc;
-    ctrl[LZ_DONE] = (nc >= nd || restart >= max_restarts) ? 1 : 0;
+    const bool kept = in && (kl >= lane || kright < lane);
+    const unsigned long long km = __ballot(kept);
+    const unsigned long long inm = __ballot(in);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int knext = __popcll(km);
+    if (in) part[kept ? __popcll(km & below) : knext + __popcll(~km & inm & below)] = lane;
+    if (t == 0) {
+      sh[0] = knext;
+      sh[1] = nc;
+      const int restart = ctrl[LZ_RESTART] + 1;
+      ctrl[LZ_K] = knext;
+      ctrl[LZ_RESTART] = restart;
+      ctrl[LZ_NCONV] = nc;
+      ctrl[LZ_DONE] = (nc >= nd || restart >= max_restarts) ? 1 : 0;
+    }
   }
   __syncthreads();
   const int knext = sh[0];
