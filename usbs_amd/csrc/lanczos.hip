@@ -15,7 +15,11 @@
 //   C  h2 = sum part2; y -= Q h2; part3 = |y|^2; w_i = y; alpha_i = h1_i + h2_i
 // Normalising the previous residual is fused into the next step's SpMV: the
 // operator gathers (T)(w_{i-1}[c] / beta), the same value the basis column
-// holds, so a step needs three device-wide reductions.  A breakdown
+// holds, so a step needs three device-wide reductions.  Light-row (RR)
+// tiles need two: phase B also sums |y|^2 and phase C takes
+// beta^2 = |y|^2 - |h2|^2 (y - Q h2 with h2 = Q^T y removes exactly |h2|^2,
+// and h2 is a rounding-level correction, so nothing cancels), except on a
+// step whose dot products fill every butterfly slot, which keeps part3.  A breakdown
 // (beta < 1e-15, lanczos.py:107-117) is handled before phase A by
 // orthogonalising a counter-based random vector in place of w_{i-1}.  After
 // the last step the residual is normalised into Q[:, m].
@@ -796,11 +800,28 @@ __device__ __forceinline__ double rr_beta(const LzInst<T>& L, double* red) {
   return b;
 }
 
+// Whether step i's phase B sums |y|^2 with its dot products (one more
+// column) and phase C takes beta from it: when the butterfly has a slot left
+// and the instance does not ask for exact norms.
+template <int MAXC, typename T>
+__device__ __forceinline__ bool rr_pyth(const LzInst<T>& L, int i) {
+  return !L.exact_norm && i + 2 <= RRSlots<MAXC>::C;
+}
+
+// |w_{i-1}| at the start of step i: phase C of step i-1 left it in h2[m]
+// (rr_pyth; a breakdown restart vector's norm is stored there too), else the
+// part3 partials.  known >= 0: the caller holds that value already.
+template <typename T, int MAXC, typename M>
+__device__ __forceinline__ double rr_wnorm(const LzInst<T>& L, int i, double* red, double known = -1.0) {
+  if (rr_pyth<MAXC>(L, i - 1)) return known >= 0.0 ? known : M::ld(L.h2 + L.m);
+  return rr_beta<T, M>(L, red);
+}
+
 // beta_{i-1} recorded in beta[i-1] (writer); returns whether it broke down.
-template <typename T, typename M>
+template <typename T, int MAXC, typename M>
 __device__ __forceinline__ bool rr_record_beta(const LzInst<T>& L, int i, bool writer, double* red,
-                                               double* beta_out = nullptr) {
-  const double b = rr_beta<T, M>(L, red);
+                                               double* beta_out = nullptr, double known = -1.0) {
+  const double b = rr_wnorm<T, MAXC, M>(L, i, red, known);
   if (beta_out) *beta_out = b;
   const bool brk = b < 1e-15;
   if (writer && threadIdx.x == 0) {
@@ -823,26 +844,38 @@ __device__ __forceinline__ void rr_load_q(const LzInst<T>& L, int r, bool valid,
   });
 }
 
-// v[j] = q[j] y for j <= i (and j < MAXC), else 0.
+// v[j] = q[j] y for j <= i (and j < MAXC), v[i+1] = y^2 when sq, else 0.
 template <typename T, int MAXC, int C>
-__device__ __forceinline__ void rr_products(const T (&q)[MAXC], double yd, int i, double (&v)[C]) {
+__device__ __forceinline__ void rr_products(const T (&q)[MAXC], double yd, int i, double (&v)[C],
+                                            bool sq = false) {
+  const double y2 = sq ? yd * yd : 0.0;
   static_for<C>([&](auto J) {
     constexpr int j = J;
-    if constexpr (j < MAXC) v[j] = (j <= i) ? (double)q[j] * yd : 0.0;
-    else v[j] = 0.0;
+    if constexpr (j < MAXC) v[j] = (j <= i) ? (double)q[j] * yd : (j == i + 1 ? y2 : 0.0);
+    else v[j] = (j == i + 1) ? y2 : 0.0;
   });
 }
 
 // sum_{j < c} q[j] h[j], j ascending.  h: at least MAXC readable doubles
-// (the terms past c are loaded with the others and not added).
+// (the terms past c are loaded with the others and not added).  hh (when
+// given): sum_{j < c} h[j]^2, over j = 4u + v as four chains (v = 0..3)
+// added (0 + 1) + (2 + 3).
 template <typename T, int MAXC>
-__device__ __forceinline__ double rr_qdot(const T (&q)[MAXC], const double* h, int c) {
+__device__ __forceinline__ double rr_qdot(const T (&q)[MAXC], const double* h, int c,
+                                          double* hh = nullptr) {
   double hv[MAXC];
   static_for<MAXC>([&](auto J) { hv[J] = h[J]; });
   double s = 0.0;
   static_for<MAXC>([&](auto J) {
     if (J < c) s += (double)q[J] * hv[J];
   });
+  if (hh) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    static_for<MAXC>([&](auto J) {
+      if (J < c) a[J & 3] = fma(hv[J], hv[J], a[J & 3]);
+    });
+    *hh = (a[0] + a[1]) + (a[2] + a[3]);
+  }
   return s;
 }
 
@@ -935,33 +968,48 @@ __device__ __forceinline__ void rr_phaseA(const LzInst<T>& L, int tl, int i, boo
   rr_reduce_cols<C, M>(v, i + 1, red, L.part1 + (long long)tl * L.pstride);
 }
 
-// Phases B (final 0: part2 = Q^T y) and C (final 1: part3 = |y|^2) after
-// y -= Q h with h the sum of the previous pass's tile partials.
+// Phases B (final 0: part2 = Q^T y, and |y|^2 in column i + 1 when
+// rr_pyth) and C (final 1) after y -= Q h with h the sum of the previous
+// pass's tile partials.  C returns beta = |y| = sqrt(|y_B|^2 - |h2|^2) and
+// tile 0 stores it in h2[m] (rr_pyth), or writes part3 = |y|^2 partials and
+// returns -1.
 template <typename T, int MAXC, typename M>
-__device__ __forceinline__ void rr_phaseBC(const LzInst<T>& L, int tl, int i, int final_pass, T (&q)[MAXC], T& y,
+__device__ __forceinline__ double rr_phaseBC(const LzInst<T>& L, int tl, int i, int final_pass, T (&q)[MAXC], T& y,
                            double* red) {
   constexpr int C = RRSlots<MAXC>::C;
+  const bool pyth = rr_pyth<MAXC>(L, i);
   const int t = threadIdx.x;
   const int r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
   const bool valid = t < R;
   double* h = lz_hbuf(L, red);
-  rr_sum_tiles<M>(final_pass ? L.part2 : L.part1, L.ntiles, L.pstride, i + 1, h);
+  rr_sum_tiles<M>(final_pass ? L.part2 : L.part1, L.ntiles, L.pstride,
+                  (final_pass && pyth) ? i + 2 : i + 1, h);
   if (tl == 0)
     for (int j = t; j <= i; j += USBS_NT) M::st((final_pass ? L.h2 : L.h1) + j, h[j]);
-  y = (T)((double)y - rr_qdot<T, MAXC>(q, h, i + 1));
+  double hh = 0.0;
+  y = (T)((double)y - rr_qdot<T, MAXC>(q, h, i + 1, (final_pass && pyth) ? &hh : nullptr));
   const double yd = valid ? (double)y : 0.0;
+  double beta = -1.0;
   if (!final_pass) {
     double v[C];
-    rr_products<T, MAXC, C>(q, yd, i, v);
-    rr_reduce_cols<C, M>(v, i + 1, red, L.part2 + (long long)tl * L.pstride);
+    rr_products<T, MAXC, C>(q, yd, i, v, pyth);
+    rr_reduce_cols<C, M>(v, pyth ? i + 2 : i + 1, red, L.part2 + (long long)tl * L.pstride);
   } else {
-    double v[1] = {yd * yd};
-    rr_reduce_cols<1, M>(v, 1, red, L.part3 + (long long)tl * L.pstride);
+    if (pyth) {
+      const double b2 = h[i + 1] - hh;
+      beta = sqrt(b2 > 0.0 ? b2 : 0.0);
+      if (tl == 0 && t == 0) M::st(L.h2 + L.m, beta);
+    } else {
+      double v[1] = {yd * yd};
+      rr_reduce_cols<1, M>(v, 1, red, L.part3 + (long long)tl * L.pstride);
+    }
     if (tl == 0 && t == 0) {
       L.alpha[i] = M::ld(L.h1 + i) + h[i];
       L.ctrl[LZ_STEPS] += 1;
     }
+    __syncthreads();  // h is read above and rewritten by the next sum
   }
+  return beta;
 }
 
 // Breakdown restart vector for tile tl (lanczos.py:107-117): counter normals
@@ -1007,7 +1055,7 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_A_rr(LzInst<T> L, int i, int fir
   rr_prefetch<T, MemPlain>(L, tl, i, first != 0, mode, L.row_ptr + tl * L.tile, L.col, pf);
   double beta = -1.0;
   if (!first) {
-    beta = rr_beta<T, MemPlain>(L, lds);
+    beta = rr_wnorm<T, MAXC, MemPlain>(L, i, lds);
     if (record) {
       __syncthreads();
       const bool brk = beta < 1e-15;
@@ -1054,7 +1102,7 @@ template <typename T, int MAXC>
 __global__ void __launch_bounds__(USBS_NT) k_lz_pre_rr(LzInst<T> L, int i) {
   extern __shared__ double lds[];
   if (threadIdx.x == 0) L.ctrl[LZ_PENDING] = 0;
-  if (!rr_record_beta<T, MemPlain>(L, i, true, lds)) return;
+  if (!rr_record_beta<T, MAXC, MemPlain>(L, i, true, lds)) return;
   for (int sub = 0; sub < 3; ++sub) {
     for (int tl = 0; tl < L.ntiles; ++tl) {
       const int t = threadIdx.x, r0 = tl * L.tile, R = min(L.n, r0 + L.tile) - r0;
@@ -1063,6 +1111,10 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_pre_rr(LzInst<T> L, int i) {
       rr_breakdown_sub<T, MAXC, MemPlain>(L, tl, i, sub, q, lds);
     }
     __syncthreads();
+  }
+  if (rr_pyth<MAXC>(L, i - 1)) {  // the restart vector's norm, where phase A looks
+    const double b = rr_beta<T, MemPlain>(L, lds);
+    if (threadIdx.x == 0) L.h2[L.m] = b;
   }
 }
 
@@ -1119,13 +1171,14 @@ __device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsi
     __syncthreads();
   }
   bool ok = true;
+  double bnext = -1.0;  // |w_{i-1}| from phase C of the previous step (rr_pyth)
   for (int i = k; i <= L.m && ok; ++i) {
     double beta = -1.0;
     const int mode = i == L.m ? 3 : 0;
     RRPre<T> pf;
     if constexpr (RES) rr_prefetch<T, M>(L, lw, i, i == k, mode, rp, ci, pf);
     if (i > k) {
-      if (rr_record_beta<T, M>(L, i, lw == 0, red, &beta)) {
+      if (rr_record_beta<T, MAXC, M>(L, i, lw == 0, red, &beta, bnext)) {
         for (int sub = 0; sub < 3 && ok; ++sub) {
           if constexpr (RES) {
             rr_breakdown_sub<T, MAXC, M>(L, lw, i, sub, q, red);
@@ -1162,7 +1215,8 @@ __device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsi
     if (!(ok = coh_barrier(bar, lw, G, epoch, err, spin_limit))) break;
     for (int pass = 0; pass < 2 && ok; ++pass) {
       if constexpr (RES) {
-        rr_phaseBC<T, MAXC, M>(L, lw, i, pass, q, y, red);
+        const double b = rr_phaseBC<T, MAXC, M>(L, lw, i, pass, q, y, red);
+        if (pass == 1) bnext = b;
         if (pass == 1 && valid) M::st(L.w + (long long)(i & 1) * L.ldq + rr, y);
       } else {
         for (int tl = lw; tl < L.ntiles; tl += G) {
@@ -1172,7 +1226,8 @@ __device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsi
           T qt[MAXC];
           rr_load_q<T, MAXC, M>(L, sr, t < SR, i + 1, qt);
           T yt = M::ld(wc + sr);
-          rr_phaseBC<T, MAXC, M>(L, tl, i, pass, qt, yt, red);
+          const double b = rr_phaseBC<T, MAXC, M>(L, tl, i, pass, qt, yt, red);
+          if (pass == 1) bnext = b;
           if (t < SR) M::st(wc + sr, yt);
         }
       }

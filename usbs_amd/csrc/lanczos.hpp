@@ -54,8 +54,12 @@ struct LzInst {
   int tile_nnz_max;  // largest number of operator entries in one row tile
   int heavy;         // 1: a wavefront per operator row in the SpMV (rows of
                      // 32+ entries on average), else a thread per row
-  int reserved;      // keeps the struct free of padding bytes (it is compared
-                     // bytewise as the key of the captured per-op graphs)
+  int exact_norm;    // 1: on light-row (RR) tiles every step takes |w| from
+                     // its own reduction (the bundle QR, whose columns must
+                     // be unit to storage precision); 0: |w| may come from
+                     // the second CGS pass (|y|^2 - |h2|^2).  Also keeps the
+                     // struct free of padding bytes (it is compared bytewise
+                     // as the key of the captured per-op graphs)
 };
 static_assert(sizeof(LzInst<double>) == 8 * 4 + 8 * 2 + 8 * 17 + 4 * 4, "LzInst has padding");
 

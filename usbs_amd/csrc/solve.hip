@@ -227,6 +227,7 @@ struct Solver {
     if (rc) return rc;
     lz.rtol = p.lanczos_rtol;
     rc = make_lanczos<T>(c, pool, *P, k, k, 1, 0.0, (const T*)nullptr, qr);
+    qr.exact_norm = 1;
     if (rc) return rc;
     if (lz.ldq != ld || qr.ldq != ld) return set_err(c, USBS_ERR_INVALID, "leading dimension mismatch");
     if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
