@@ -240,7 +240,7 @@ struct CgalSolver {
       const LzStop stop{&ctrl_ptr, 2, nullptr};
       CK(lanczos_persistent<T>(d_lz, 1, nb, mL, lz.max_restarts, s, d_map, d_map + 2 * nb,
                                lz.ntiles, LZ_SPIN_LIMIT, cache_bytes, nullptr,
-                               lz.tile > 256 ? 1 : 0, &stop));
+                               lanczos_tile_form(lz), &stop));
     }
     return USBS_OK;
   }

@@ -330,7 +330,7 @@ static int run_lanczos_standalone(Ctx* c, double C_coef, const double* y, const 
     int* ctrl_ptr = L.ctrl;
     const LzStop stop{&ctrl_ptr, 2, nullptr};
     rc = lanczos_persistent<T>(d_inst, 1, nb, m, max_restarts, c->stream, d_map, d_map + 2 * nb,
-                               L.ntiles, LZ_SPIN_LIMIT, cache_bytes, nullptr, L.tile > 256 ? 1 : 0,
+                               L.ntiles, LZ_SPIN_LIMIT, cache_bytes, nullptr, lanczos_tile_form(L),
                                &stop);
     if (rc) return set_err(c, USBS_ERR_HIP, std::string("lanczos persistent: ") + hipGetErrorString((hipError_t)rc));
     USBS_HIP(c, hipMemcpyAsync(host_ctrl, L.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, c->stream));
@@ -373,13 +373,21 @@ int LzBatch<T>::build(Ctx* c0, std::vector<LzInst<T>>& L, int m_) {
     cb = std::max(cb, b);
   }
   if (!all_cache) cb = 0;
-  int rw = 0;
-  for (auto& x : L) rw |= (x.tile > 256) ? 1 : 0;
-  int cap = cb ? lanczos_max_coresident_blocks<T>(m, true, false, cb, c0->device)
-               : lanczos_max_coresident_blocks<T>(m, false, rw != 0, lanczos_tile_lds_bytes(m), c0->device);
+  int form = lanczos_tile_form(L[0]);
+  for (auto& x : L) form = lanczos_launch_form(form, lanczos_tile_form(x));
+  int cap = cb ? lanczos_max_coresident_blocks<T>(m, true, form, cb, c0->device)
+               : lanczos_max_coresident_blocks<T>(m, false, form, lanczos_tile_lds_bytes(m), c0->device);
+  if (cb && cap < total_tiles && form == LZ_FORM_RR) {
+    // the three-wavefront copy of the kernel keeps every tile resident
+    const int cap3 = lanczos_max_coresident_blocks<T>(m, true, LZ_FORM_RR3, cb, c0->device);
+    if (cap3 >= total_tiles) {
+      form = LZ_FORM_RR3;
+      cap = cap3;
+    }
+  }
   if (cb && cap < total_tiles) {
     cb = 0;
-    cap = lanczos_max_coresident_blocks<T>(m, false, rw != 0, lanczos_tile_lds_bytes(m), c0->device);
+    cap = lanczos_max_coresident_blocks<T>(m, false, form, lanczos_tile_lds_bytes(m), c0->device);
   }
   if (cap < ninst) return set_err(c0, USBS_ERR_INVALID, "batch does not fit one co-resident launch");
   total_wgs = 0;
@@ -403,12 +411,9 @@ int LzBatch<T>::build(Ctx* c0, std::vector<LzInst<T>>& L, int m_) {
   USBS_HIP(c0, hipMemcpy(d_insts, L.data(), sizeof(LzInst<T>) * ninst, hipMemcpyHostToDevice));
   USBS_HIP(c0, hipMemcpy(d_wg, wgmap.data(), sizeof(int2) * wgmap.size(), hipMemcpyHostToDevice));
   USBS_HIP(c0, hipMemcpy(d_tm, tilemap.data(), sizeof(int2) * tilemap.size(), hipMemcpyHostToDevice));
-  rowwise = 0;
+  tile_form = form;
   ctrls.assign(ninst, nullptr);
-  for (int q = 0; q < ninst; ++q) {
-    rowwise |= (L[q].tile > 256) ? 1 : 0;
-    ctrls[q] = L[q].ctrl;
-  }
+  for (int q = 0; q < ninst; ++q) ctrls[q] = L[q].ctrl;
   return USBS_OK;
 }
 
@@ -416,7 +421,7 @@ template <typename T>
 int LzBatch<T>::run(hipStream_t s, int max_restarts, int first_check, int* launched) {
   const LzStop stop{ctrls.data(), first_check, launched};
   return lanczos_persistent<T>(d_insts, ninst, total_wgs, m, max_restarts, s, d_wg, d_tm,
-                               total_tiles, LZ_SPIN_LIMIT, cache, nullptr, rowwise, &stop);
+                               total_tiles, LZ_SPIN_LIMIT, cache, nullptr, tile_form, &stop);
 }
 
 template struct LzBatch<double>;

@@ -156,7 +156,7 @@ struct LzBatch {
   LzInst<T>* d_insts = nullptr;
   int* d_wg = nullptr;
   int* d_tm = nullptr;
-  int ninst = 0, m = 0, total_wgs = 0, total_tiles = 0, rowwise = 0;
+  int ninst = 0, m = 0, total_wgs = 0, total_tiles = 0, tile_form = 0;
   size_t cache = 0;
   std::vector<int*> ctrls;
   ~LzBatch() { ctx_free_pool(pool); }

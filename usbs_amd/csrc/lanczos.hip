@@ -1532,17 +1532,55 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
     tile_steps<T, C, MAXC, RW>(L, mp.y, r, spin_limit, lds);  // a copy in registers
 }
 
+// The same for launches whose instances all have light rows in tiles of at
+// most 256 rows (LZ_FORM_RR): only rr_steps.  The _rr3 copy is compiled for
+// three wavefronts per SIMD (LZ_FORM_RR3), so that three workgroups share a
+// compute unit and a batch of solves stays co-resident with one workgroup
+// per tile.
+template <typename T, bool C, int MAXC>
+__global__ void __launch_bounds__(USBS_NT)
+k_lz_steps_coop_rr(const LzInst<T>* __restrict__ insts, const int2* __restrict__ wgmap, int r,
+                   unsigned long long spin_limit) {
+  extern __shared__ double lds[];
+  const int2 mp = wgmap[blockIdx.x];
+  const LzInst<T>& L = insts[mp.x];
+  if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;
+  rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds);
+}
+template <typename T, bool C, int MAXC>
+__global__ void __launch_bounds__(USBS_NT) __attribute__((amdgpu_waves_per_eu(3)))
+k_lz_steps_coop_rr3(const LzInst<T>* __restrict__ insts, const int2* __restrict__ wgmap, int r,
+                    unsigned long long spin_limit) {
+  extern __shared__ double lds[];
+  const int2 mp = wgmap[blockIdx.x];
+  const LzInst<T>& L = insts[mp.x];
+  if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;
+  rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds);
+}
+
+// The step kernel for (T, MAXC, cache, form).
+template <typename T, int MAXC>
+static const void* coop_kernel_m(bool cache, int form) {
+  if (form == LZ_FORM_RR)
+    return cache ? (const void*)k_lz_steps_coop_rr<T, true, MAXC>
+                 : (const void*)k_lz_steps_coop_rr<T, false, MAXC>;
+  if (form == LZ_FORM_RR3)
+    return cache ? (const void*)k_lz_steps_coop_rr3<T, true, MAXC>
+                 : (const void*)k_lz_steps_coop_rr3<T, false, MAXC>;
+  return cache ? (const void*)k_lz_steps_coop<T, true, MAXC, false>
+               : form == LZ_FORM_ROW ? (const void*)k_lz_steps_coop<T, false, MAXC, true>
+                                     : (const void*)k_lz_steps_coop<T, false, MAXC, false>;
+}
+
 template <typename T, int MAXC>
 static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                            int max_restarts, hipStream_t s, const int* dev_wgmap,
                            const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                           size_t cache_bytes, hipEvent_t* evs, int rowwise,
+                           size_t cache_bytes, hipEvent_t* evs, int form,
                            const LzStop* stop) {
   const bool cache = cache_bytes > 0;
   const size_t lds = cache ? cache_bytes : tile_lds_bytes(m + 1);
-  const void* kern = cache ? (const void*)k_lz_steps_coop<T, true, MAXC, false>
-                     : rowwise ? (const void*)k_lz_steps_coop<T, false, MAXC, true>
-                               : (const void*)k_lz_steps_coop<T, false, MAXC, false>;
+  const void* kern = coop_kernel_m<T, MAXC>(cache, form);
   int next_check = (stop && stop->ctrls && stop->first_check > 0) ? stop->first_check : max_restarts;
   std::vector<int> hc;
   int launched = 0;
@@ -1603,41 +1641,48 @@ template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
                        const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                       size_t cache_bytes, hipEvent_t* evs, int rowwise, const LzStop* stop) {
+                       size_t cache_bytes, hipEvent_t* evs, int form, const LzStop* stop) {
   if (m + 1 <= 17)
     return persistent_impl<T, 17>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
                                   dev_tilemap, total_tiles, spin_limit, cache_bytes, evs,
-                                  rowwise, stop);
+                                  form, stop);
   if (m + 1 <= 33)
     return persistent_impl<T, 33>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
                                   dev_tilemap, total_tiles, spin_limit, cache_bytes, evs,
-                                  rowwise, stop);
+                                  form, stop);
   if (m + 1 <= 49)
     return persistent_impl<T, 49>(dev_insts, ninst, total_wgs, m, max_restarts, s, dev_wgmap,
                                   dev_tilemap, total_tiles, spin_limit, cache_bytes, evs,
-                                  rowwise, stop);
+                                  form, stop);
   return (int)hipErrorInvalidValue;
 }
 
 size_t lanczos_tile_lds_bytes(int m) { return tile_lds_bytes(m + 1); }
 
-// The step kernel persistent_impl launches for (T, m, cache, rowwise).
+// The step kernel persistent_impl launches for (T, m, cache, form).
 template <typename T>
-static const void* coop_kernel(int m, bool cache, bool rowwise) {
-#define USBS_COOP_K(MC)                                                          \
-  return cache ? (const void*)k_lz_steps_coop<T, true, MC, false>                \
-               : rowwise ? (const void*)k_lz_steps_coop<T, false, MC, true>      \
-                         : (const void*)k_lz_steps_coop<T, false, MC, false>
-  if (m + 1 <= 17) USBS_COOP_K(17);
-  if (m + 1 <= 33) USBS_COOP_K(33);
-  USBS_COOP_K(49);
-#undef USBS_COOP_K
+static const void* coop_kernel(int m, bool cache, int form) {
+  if (m + 1 <= 17) return coop_kernel_m<T, 17>(cache, form);
+  if (m + 1 <= 33) return coop_kernel_m<T, 33>(cache, form);
+  return coop_kernel_m<T, 49>(cache, form);
 }
 
 template <typename T>
-int lanczos_max_coresident_blocks(int m, bool cache, bool rowwise, size_t lds_bytes, int device) {
+int lanczos_tile_form(const LzInst<T>& L) {
+  if (L.tile > USBS_NT) return LZ_FORM_ROW;
+  return L.heavy ? LZ_FORM_COL : LZ_FORM_RR;
+}
+template int lanczos_tile_form<double>(const LzInst<double>&);
+template int lanczos_tile_form<float>(const LzInst<float>&);
+int lanczos_launch_form(int a, int b) {
+  if (a == LZ_FORM_ROW || b == LZ_FORM_ROW) return LZ_FORM_ROW;
+  return (a == LZ_FORM_RR && b == LZ_FORM_RR) ? LZ_FORM_RR : LZ_FORM_COL;
+}
+
+template <typename T>
+int lanczos_max_coresident_blocks(int m, bool cache, int form, size_t lds_bytes, int device) {
   int per_cu = 0;
-  const void* kern = coop_kernel<T>(m, cache, rowwise);
+  const void* kern = coop_kernel<T>(m, cache, form);
   if (lds_bytes > 64 * 1024 &&
       hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
     return 0;
@@ -1647,8 +1692,8 @@ int lanczos_max_coresident_blocks(int m, bool cache, bool rowwise, size_t lds_by
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
   return per_cu * prop.multiProcessorCount;
 }
-template int lanczos_max_coresident_blocks<double>(int, bool, bool, size_t, int);
-template int lanczos_max_coresident_blocks<float>(int, bool, bool, size_t, int);
+template int lanczos_max_coresident_blocks<double>(int, bool, int, size_t, int);
+template int lanczos_max_coresident_blocks<float>(int, bool, int, size_t, int);
 
 template <typename T>
 int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cache_bytes) {
@@ -1656,12 +1701,12 @@ int lanczos_plan_persistent(LzInst<T>& L, int device, int want_wgs, size_t* cach
   if (want_wgs > 0 && want_wgs < nb) nb = want_wgs;
   L.nblocks = nb;
   size_t cb = lanczos_cache_bytes(L);
-  const bool rw = L.tile > 256;
-  int cores = cb ? lanczos_max_coresident_blocks<T>(L.m, true, false, cb, device)
-                 : lanczos_max_coresident_blocks<T>(L.m, false, rw, tile_lds_bytes(L.m + 1), device);
+  const int form = lanczos_tile_form(L);
+  int cores = cb ? lanczos_max_coresident_blocks<T>(L.m, true, form, cb, device)
+                 : lanczos_max_coresident_blocks<T>(L.m, false, form, tile_lds_bytes(L.m + 1), device);
   if (cb && cores < nb) {
     cb = 0;
-    cores = lanczos_max_coresident_blocks<T>(L.m, false, rw, tile_lds_bytes(L.m + 1), device);
+    cores = lanczos_max_coresident_blocks<T>(L.m, false, form, tile_lds_bytes(L.m + 1), device);
   }
   if (cores < 1) return (int)hipErrorInvalidConfiguration;
   if (nb > cores) nb = cores;

@@ -63,6 +63,18 @@ struct LzInst {
 };
 static_assert(sizeof(LzInst<double>) == 8 * 4 + 8 * 2 + 8 * 17 + 4 * 4, "LzInst has padding");
 
+// Tile forms of the step kernels: column-wise reductions (tiles of at most
+// 256 rows with heavy rows), row-wise reductions (tiles above 256 rows), rows
+// of Q in registers (light rows, tiles of at most 256 rows).  The persistent
+// kernel for LZ_FORM_RR carries only that code; LZ_FORM_RR3 is the same code
+// compiled for three wavefronts per SIMD (some registers spill: slower per
+// step, but three workgroups share a compute unit), for batches that would
+// not be co-resident otherwise.
+#define LZ_FORM_COL 0
+#define LZ_FORM_ROW 1
+#define LZ_FORM_RR 2
+#define LZ_FORM_RR3 3
+
 // Largest LDS block of a persistent workgroup that caches its tile.
 #define LZ_CACHE_MAX_BYTES (64 * 1024)
 // Barrier counters of one persistent launch (restart): a region of
@@ -122,7 +134,7 @@ template <typename T>
 int lanczos_persistent(const LzInst<T>* dev_insts, int ninst, int total_wgs, int m,
                        int max_restarts, hipStream_t s, const int* dev_wgmap,
                        const int* dev_tilemap, int total_tiles, unsigned long long spin_limit,
-                       size_t cache_bytes, hipEvent_t* evs = nullptr, int rowwise = 1,
+                       size_t cache_bytes, hipEvent_t* evs = nullptr, int form = LZ_FORM_ROW,
                        const LzStop* stop = nullptr);
 // Workgroup count (L.nblocks) and LDS cache size for a persistent launch of
 // one instance: one workgroup per tile unless want_wgs > 0 asks for fewer,
@@ -135,10 +147,15 @@ size_t lanczos_cache_bytes(const LzInst<T>& L);
 // case where it beats per-op launches): light rows, tiles of at most 256
 // rows, cache within LZ_CACHE_MAX_BYTES.
 int lanczos_prefers_persistent(int tile, int heavy, int m, int tile_nnz_max, int value_bytes);
-// Workgroups of the persistent step kernel for (T, m, cache, rowwise) that
+// Workgroups of the persistent step kernel for (T, m, cache, form) that
 // fit on the device at once with lds_bytes of LDS each.
 template <typename T>
-int lanczos_max_coresident_blocks(int m, bool cache, bool rowwise, size_t lds_bytes, int device);
+int lanczos_max_coresident_blocks(int m, bool cache, int form, size_t lds_bytes, int device);
+// The tile form of an instance (LZ_FORM_*), and of a launch holding several:
+// row-wise if any instance is, row-in-registers only if every one is.
+template <typename T>
+int lanczos_tile_form(const LzInst<T>& L);
+int lanczos_launch_form(int form_a, int form_b);
 // LDS of a persistent workgroup that does not cache its tile (basis size m).
 size_t lanczos_tile_lds_bytes(int m);
 template <typename T>

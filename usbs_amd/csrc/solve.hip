@@ -309,7 +309,7 @@ struct Solver {
       return hipGetLastError() != hipSuccess ||
              lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
                                    d_map + 2 * nb, lz.ntiles, LZ_SPIN_LIMIT, cache_bytes, nullptr,
-                                   lz.tile > 256 ? 1 : 0, &stop);
+                                   lanczos_tile_form(lz), &stop);
     };
     do {
       if (perop() || persistent()) { rc = USBS_ERR_HIP; break; }
@@ -375,7 +375,7 @@ struct Solver {
       const LzStop stop{&ctrl_ptr, lz_hint, &launched};
       CK(lanczos_persistent<T>(nd == k ? d_lz_k : d_lz_kc, 1, nb, mL, lz.max_restarts, s, d_map,
                                d_map + 2 * nb, lz.ntiles, LZ_SPIN_LIMIT, cache_bytes,
-                               tev.empty() ? nullptr : tev.data(), lz.tile > 256 ? 1 : 0, &stop));
+                               tev.empty() ? nullptr : tev.data(), lanczos_tile_form(lz), &stop));
       tev_launched = launched;
       tev_pending = !tev.empty();
     }
