@@ -532,6 +532,12 @@ void usbs_destroy(usbs_ctx* h) {
   ctx_free_pool(h->c.prob_bufs);
   ctx_free_pool(h->c.keep_bufs);
   ctx_release_cache(&h->c);
+  if (h->c.side) {
+    (void)hipStreamSynchronize(h->c.side);
+    (void)hipStreamDestroy(h->c.side);
+  }
+  if (h->c.rec_h) (void)hipHostFree(h->c.rec_h);
+  if (h->c.ctrl_h) (void)hipHostFree(h->c.ctrl_h);
   (void)hipStreamDestroy(h->c.stream);
   delete h;
 }

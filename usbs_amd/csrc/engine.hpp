@@ -119,6 +119,12 @@ struct Ctx {
   int qap_l = 0;
   std::vector<double> qap_W, qap_D;
   usbs_stats stats{};
+  // kept for the context's solves, created on first use: the side stream of
+  // the lower-bound IPM, and pinned host buffers for the records (scalar
+  // block, Lanczos control words)
+  hipStream_t side = nullptr;
+  double* rec_h = nullptr;
+  int* ctrl_h = nullptr;
 };
 
 // error helpers

@@ -122,9 +122,16 @@ struct Solver {
   std::vector<hipEvent_t> tev;       // timing: 2 events per persistent launch
   bool tev_pending = false;
   std::vector<int> lstat_h;
+  // the scalar block and the Lanczos control words of a record, copied into
+  // the context's pinned host buffers behind ev_rec (record_launch /
+  // record_finish)
+  double* rec_h = nullptr;
+  int* ctrl_h = nullptr;
+  hipEvent_t ev_rec = nullptr;
 
   ~Solver() {
-    if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
+    if (s2) (void)hipStreamSynchronize(s2);  // the context's side stream: kept
+    if (ev_rec) { (void)hipEventSynchronize(ev_rec); (void)hipEventDestroy(ev_rec); }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
@@ -230,10 +237,17 @@ struct Solver {
     qr.exact_norm = 1;
     if (rc) return rc;
     if (lz.ldq != ld || qr.ldq != ld) return set_err(c, USBS_ERR_INVALID, "leading dimension mismatch");
-    if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+    if ((!c->side && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) ||
         hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_rec, hipEventDisableTiming) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
+    s2 = c->side;
+    if ((!c->rec_h && hipHostMalloc((void**)&c->rec_h, sizeof(double) * SC_COUNT) != hipSuccess) ||
+        (!c->ctrl_h && hipHostMalloc((void**)&c->ctrl_h, sizeof(int) * LZ_NCTRL) != hipSuccess))
+      return set_err(c, USBS_ERR_OOM, "pinned host buffers for the records");
+    rec_h = c->rec_h;
+    ctrl_h = c->ctrl_h;
     tuning = p.exec_mode == USBS_EXEC_TUNE;
     if (c->timing) {
       if (p.exec_mode == USBS_EXEC_PERSISTENT || tuning) {
@@ -516,19 +530,26 @@ struct Solver {
     return (int)hipGetLastError();
   }
 
-  int read_record(usbs_record& rec, double t0) {
-    double h[SC_COUNT];
-    if (hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
+  // A record in two parts: record_launch queues the copies of the scalar
+  // block (and, persistent, the Lanczos control words) behind the stream's
+  // work; record_finish waits for them and fills rec.
+  int record_launch() {
+    if (hipMemcpyAsync(rec_h, sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        (p.exec_mode == USBS_EXEC_PERSISTENT &&
+         hipMemcpyAsync(ctrl_h, lz.ctrl, sizeof(int) * LZ_NCTRL, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipEventRecord(ev_rec, s) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "reading the scalar block");
+    return USBS_OK;
+  }
+  int record_finish(usbs_record& rec, double t0) {
+    if (hipEventSynchronize(ev_rec) != hipSuccess) return set_err(c, USBS_ERR_HIP, "reading the scalar block");
+    const double* h = rec_h;
     if (tev_pending) {
       const int rc = collect_timing();
       if (rc) return rc;
     }
     if (p.exec_mode == USBS_EXEC_PERSISTENT) {
-      int ctrl[LZ_NCTRL];
-      if (hipMemcpy(ctrl, lz.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost) != hipSuccess)
-        return set_err(c, USBS_ERR_HIP, "reading Lanczos control");
+      const int* ctrl = ctrl_h;
       if (ctrl[LZ_ERROR]) return set_err(c, USBS_ERR_TIMEOUT, "persistent Lanczos barrier wait expired");
       lz_restarts = ctrl[LZ_RESTART];
       lz_hint = lz_restarts > 1 ? lz_restarts : 1;
@@ -712,14 +733,24 @@ struct SolveRun {
   }
 
   // condition check (solver/usbs.py:503-540): rounding, record, stop test
+  // a condition check: cond_launch queues its kernels and the record's
+  // copies, cond_finish waits for them (the batched driver launches every
+  // due check before it finishes any)
   int cond() {
+    const int rc = cond_launch();
+    return rc ? rc : cond_finish();
+  }
+  int cond_launch() {
     hipStream_t s = c->stream;
     int rc2 = S.rounding();
     if (rc2) return rc2;
     if (launch_dots3(c->P, S.y, c->P.b, nullptr, S.partial, S.tmp4, s)) return set_err(c, USBS_ERR_HIP, "dots");
     k_set_by<<<1, 64, 0, s>>>(S.sc, S.tmp4);
+    return S.record_launch();
+  }
+  int cond_finish() {
     usbs_record rec{};
-    rc2 = S.read_record(rec, t_origin);
+    const int rc2 = S.record_finish(rec, t_origin);
     if (rc2) return rc2;
     if (log && nlog < log_cap) log[nlog] = rec;
     ++nlog;
@@ -983,12 +1014,28 @@ static int run_batch(const std::vector<Ctx*>& cs, const std::vector<usbs_params>
       if ((rc = R[q]->S.body_pre())) return rc;
     }
     if ((rc = batched_lanczos(act, lb, R[act[0]]->S.kc))) return rc;
+    // every solve's rest of the iteration, then every due condition check,
+    // is enqueued before the first check waits on its stream, so the solves
+    // run them together
     for (int q : act) {
       failed = q;
       if ((rc = R[q]->S.body_post())) return rc;
       ++R[q]->iters;
       --left[q];
-      if ((rc = checks(q))) return rc;
+    }
+    for (int q : act) {
+      if (!(on[q] && left[q] == 0)) continue;
+      failed = q;
+      if ((rc = R[q]->cond_launch())) return rc;
+    }
+    for (int q : act) {
+      if (!(on[q] && left[q] == 0)) continue;
+      failed = q;
+      if ((rc = R[q]->cond_finish())) return rc;
+      ++pos[q];
+      on[q] = R[q]->go && pos[q] < sched[q].count;
+      if (on[q]) left[q] = sched[q].at(pos[q]);
+      if ((rc = checks(q))) return rc;  // a further check due at once
     }
   }
   for (int q = 0; q < B; ++q) {
