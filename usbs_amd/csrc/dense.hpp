@@ -381,15 +381,27 @@ __device__ __forceinline__ double td_rcp(double q) {
 
 // Number of eigenvalues of the tridiagonal (register copy dr, e2r; entries
 // at and past m ignored) below x.  Straight-line code: steps past m keep q.
+// A guarded pivot is never zero, so its sign bit counts it.
+__device__ __forceinline__ int td_negbit(double q) {
+  return (int)((unsigned)(__double_as_longlong(q) >> 32) >> 31);
+}
 __device__ __forceinline__ int td_sturm(const double (&dr)[TD_MAXM], const double (&e2r)[TD_MAXM],
                                         int m, double x, double pivmin) {
   double q = td_guard(dr[0] - x, pivmin);
-  int c = q < 0.0 ? 1 : 0;
+  int c = td_negbit(q);
+  if (m == TD_MAXM) {  // every entry in use: no masking
+#pragma unroll
+    for (int i = 1; i < TD_MAXM; ++i) {
+      q = td_guard((dr[i] - x) - e2r[i - 1] * td_rcp(q), pivmin);
+      c += td_negbit(q);
+    }
+    return c;
+  }
 #pragma unroll
   for (int i = 1; i < TD_MAXM; ++i) {
     const double qn = td_guard((dr[i] - x) - e2r[i - 1] * td_rcp(q), pivmin);
     const bool in = i < m;
-    c += (in && qn < 0.0) ? 1 : 0;
+    c += in ? td_negbit(qn) : 0;
     q = in ? qn : q;
   }
   return c;
