@@ -1135,8 +1135,11 @@ static inline size_t rr_res_lds_bytes(int tile, int m, int nnz_tile) {
   return (b + 255) / 256 * 256;
 }
 
+// mode0: 0 Lanczos steps (SpMV), 2 the column-by-column QR (y = Q[:, i+1]);
+// the step i = m normalises the last vector (mode 3) either way.
 template <typename T, int MAXC, bool RES>
-__device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsigned long long spin_limit, double* lds) {
+__device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsigned long long spin_limit, double* lds,
+                                         int mode0 = 0) {
   using M = MemCoherent;
   const int G = L.nblocks, t = threadIdx.x;
   int* err = L.ctrl + LZ_ERROR;
@@ -1163,10 +1166,12 @@ __device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsi
   const bool valid = t < R;
   const int rr = r0 + (valid ? t : 0);
   if constexpr (RES) {
-    const int pbase = L.row_ptr[r0];
-    const int nnzt = L.row_ptr[r0 + R] - pbase;
-    for (int x = t; x <= R; x += USBS_NT) rp[x] = L.row_ptr[r0 + x] - pbase;
-    for (int p = t; p < nnzt; p += USBS_NT) { ci[p] = L.col[pbase + p]; cv[p] = L.val[pbase + p]; }
+    if (mode0 == 0) {  // the QR reads no operator
+      const int pbase = L.row_ptr[r0];
+      const int nnzt = L.row_ptr[r0 + R] - pbase;
+      for (int x = t; x <= R; x += USBS_NT) rp[x] = L.row_ptr[r0 + x] - pbase;
+      for (int p = t; p < nnzt; p += USBS_NT) { ci[p] = L.col[pbase + p]; cv[p] = L.val[pbase + p]; }
+    }
     rr_load_q<T, MAXC, MemPlain>(L, rr, valid, k + 1, q);
     __syncthreads();
   }
@@ -1174,7 +1179,7 @@ __device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsi
   double bnext = -1.0;  // |w_{i-1}| from phase C of the previous step (rr_pyth)
   for (int i = k; i <= L.m && ok; ++i) {
     double beta = -1.0;
-    const int mode = i == L.m ? 3 : 0;
+    const int mode = i == L.m ? 3 : mode0;
     RRPre<T> pf;
     if constexpr (RES) rr_prefetch<T, M>(L, lw, i, i == k, mode, rp, ci, pf);
     if (i > k) {
@@ -1208,7 +1213,7 @@ __device__ __forceinline__ void rr_steps(const LzInst<T>& L, int lw, int r, unsi
         T yt = (T)0;
         rr_prefetch<T, M>(L, tl, i, i == k, mode, L.row_ptr + s0, L.col, pf);
         rr_phaseA<T, MAXC, M>(L, tl, i, i == k, mode, beta, pf, qt, yt, L.row_ptr + s0, L.col, L.val, red);
-        if (mode == 0 && t < SR) M::st(L.w + (long long)(i & 1) * L.ldq + sr, yt);
+        if (mode != 3 && t < SR) M::st(L.w + (long long)(i & 1) * L.ldq + sr, yt);
       }
     }
     if (mode == 3) break;
@@ -1521,13 +1526,13 @@ __device__ __forceinline__ void tile_steps(const LzInst<T> L, int lw, int r,
 template <typename T, bool C, int MAXC, bool RW>
 __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __restrict__ insts,
                                                            const int2* __restrict__ wgmap, int r,
-                                                           unsigned long long spin_limit) {
+                                                           unsigned long long spin_limit, int mode0) {
   extern __shared__ double lds[];
   const int2 mp = wgmap[blockIdx.x];
   const LzInst<T>& L = insts[mp.x];
   if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;  // uniform over the instance's workgroups
   if (!RW && !L.heavy && L.tile <= USBS_NT)
-    rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds);  // the instance read in place
+    rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds, mode0);  // the instance read in place
   else
     tile_steps<T, C, MAXC, RW>(L, mp.y, r, spin_limit, lds);  // a copy in registers
 }
@@ -1540,22 +1545,22 @@ __global__ void __launch_bounds__(USBS_NT) k_lz_steps_coop(const LzInst<T>* __re
 template <typename T, bool C, int MAXC>
 __global__ void __launch_bounds__(USBS_NT)
 k_lz_steps_coop_rr(const LzInst<T>* __restrict__ insts, const int2* __restrict__ wgmap, int r,
-                   unsigned long long spin_limit) {
+                   unsigned long long spin_limit, int mode0) {
   extern __shared__ double lds[];
   const int2 mp = wgmap[blockIdx.x];
   const LzInst<T>& L = insts[mp.x];
   if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;
-  rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds);
+  rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds, mode0);
 }
 template <typename T, bool C, int MAXC>
 __global__ void __launch_bounds__(USBS_NT) __attribute__((amdgpu_waves_per_eu(3)))
 k_lz_steps_coop_rr3(const LzInst<T>* __restrict__ insts, const int2* __restrict__ wgmap, int r,
-                    unsigned long long spin_limit) {
+                    unsigned long long spin_limit, int mode0) {
   extern __shared__ double lds[];
   const int2 mp = wgmap[blockIdx.x];
   const LzInst<T>& L = insts[mp.x];
   if (L.ctrl[LZ_DONE] || L.ctrl[LZ_ERROR]) return;
-  rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds);
+  rr_steps<T, MAXC, C>(L, mp.y, r, spin_limit, lds, mode0);
 }
 
 // The step kernel for (T, MAXC, cache, form).
@@ -1585,7 +1590,8 @@ static int persistent_impl(const LzInst<T>* dev_insts, int ninst, int total_wgs,
   std::vector<int> hc;
   int launched = 0;
   for (int r = 0; r < max_restarts; ++r) {
-    void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&r, (void*)&spin_limit};
+    int mode0 = 0;
+    void* args[] = {(void*)&dev_insts, (void*)&dev_wgmap, (void*)&r, (void*)&spin_limit, (void*)&mode0};
     if (evs && hipEventRecord(evs[2 * r], s) != hipSuccess) return (int)hipErrorUnknown;
     hipError_t e = hipLaunchCooperativeKernel(kern, dim3(total_wgs), dim3(USBS_NT), args, lds, s);
     if (e != hipSuccess) return (int)e;
@@ -1666,6 +1672,35 @@ static const void* coop_kernel(int m, bool cache, int form) {
   if (m + 1 <= 33) return coop_kernel_m<T, 33>(cache, form);
   return coop_kernel_m<T, 49>(cache, form);
 }
+
+// The column-by-column QR (orth_perop's steps) as one cooperative launch of
+// the row-in-registers kernel, one workgroup per tile with its rows of Q in
+// registers and only the reduction scratch in LDS: the same arithmetic as
+// the per-op launches.  Restart region 0 of L.bar, zeroed by the caller.
+static inline size_t orth_lds_bytes(int m) {
+  return ((size_t)rr_head_doubles(m) * sizeof(double) + 255) / 256 * 256;
+}
+template <typename T>
+int orth_persistent_fits(const LzInst<T>& L, int device) {
+  if (L.m + 1 > 49 || L.tile > USBS_NT || L.heavy) return 0;
+  return lanczos_max_coresident_blocks<T>(L.m, true, LZ_FORM_RR, orth_lds_bytes(L.m), device) >= L.ntiles;
+}
+template int orth_persistent_fits<double>(const LzInst<double>&, int);
+template int orth_persistent_fits<float>(const LzInst<float>&, int);
+
+template <typename T>
+int orth_persistent(const LzInst<T>* dev_inst, int ntiles, int m, hipStream_t s, const int* dev_wgmap) {
+  if (m + 1 > 49) return (int)hipErrorInvalidValue;
+  const void* kern = coop_kernel<T>(m, true, LZ_FORM_RR);
+  int r = 0, mode0 = 2;
+  unsigned long long spin = LZ_SPIN_LIMIT;
+  void* args[] = {(void*)&dev_inst, (void*)&dev_wgmap, (void*)&r, (void*)&spin, (void*)&mode0};
+  const hipError_t e = hipLaunchCooperativeKernel(kern, dim3(ntiles), dim3(USBS_NT), args,
+                                                  orth_lds_bytes(m), s);
+  return e == hipSuccess ? 0 : (int)e;
+}
+template int orth_persistent<double>(const LzInst<double>*, int, int, hipStream_t, const int*);
+template int orth_persistent<float>(const LzInst<float>*, int, int, hipStream_t, const int*);
 
 template <typename T>
 int lanczos_tile_form(const LzInst<T>& L) {

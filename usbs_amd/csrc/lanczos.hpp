@@ -160,5 +160,13 @@ int lanczos_launch_form(int form_a, int form_b);
 size_t lanczos_tile_lds_bytes(int m);
 template <typename T>
 int orth_perop(const LzInst<T>& L, hipStream_t s);
+// The same QR as one cooperative launch of one workgroup per tile
+// (dev_inst->nblocks = ntiles, dev_wgmap the identity map over the tiles),
+// for instances orth_persistent_fits accepts: light rows in tiles of at most
+// 256 rows, every tile's workgroup co-resident on the device.
+template <typename T>
+int orth_persistent_fits(const LzInst<T>& L, int device);
+template <typename T>
+int orth_persistent(const LzInst<T>* dev_inst, int ntiles, int m, hipStream_t s, const int* dev_wgmap);
 
 }  // namespace usbs

@@ -108,6 +108,15 @@ struct Solver {
   LzInst<T>* d_lz_kc = nullptr;
   int* d_map = nullptr;
   int nb = 1;
+  // the bundle QR as one cooperative launch (qr_mode 1: persistent single
+  // solves whose tiles are resident row-in-registers tiles): descriptors of
+  // qr over each bundle buffer (d_qr[0] over qr_buf0, d_qr[1] over the
+  // other); -1 not planned yet, 0 per-op launches.  Batch members keep the
+  // per-op launches (their streams run side by side).
+  LzInst<T>* d_qr = nullptr;
+  T* qr_buf0 = nullptr;
+  int qr_mode = -1;
+  bool batch_member = false;
   size_t cache_bytes = 0;            // LDS per persistent workgroup when tiles are cached
   int host_ctrl[LZ_NCTRL] = {0};
   int lz_restarts = 0;
@@ -244,7 +253,7 @@ struct Solver {
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
     s2 = c->side;
     if ((!c->rec_h && hipHostMalloc((void**)&c->rec_h, sizeof(double) * SC_COUNT) != hipSuccess) ||
-        (!c->ctrl_h && hipHostMalloc((void**)&c->ctrl_h, sizeof(int) * LZ_NCTRL) != hipSuccess))
+        (!c->ctrl_h && hipHostMalloc((void**)&c->ctrl_h, sizeof(int) * (LZ_NCTRL + 1)) != hipSuccess))
       return set_err(c, USBS_ERR_OOM, "pinned host buffers for the records");
     rec_h = c->rec_h;
     ctrl_h = c->ctrl_h;
@@ -518,10 +527,39 @@ struct Solver {
       return set_err(c, USBS_ERR_HIP, "copy of candidate eigenvectors");
     LzInst<T> q = qr;
     q.Q = Qb;
-    k_lz_start<T><<<1, 64, 0, s>>>(q, q0);  // resets control words only (column 0 set below)
+    if (qr_mode < 0) {
+      const int rc = plan_qr();
+      if (rc) return rc;
+    }
+    // resets control words only (column 0 set below); the persistent QR's
+    // barrier error stays set for the records to report
+    k_lz_start<T><<<1, 64, 0, s>>>(q, q0, qr_mode);
     CK(fill_T(Qb, ld));
-    CK(orth_perop<T>(q, s));
+    if (qr_mode == 1)
+      CK(orth_persistent<T>(d_qr + (Qb == qr_buf0 ? 0 : 1), lz.ntiles, k, s, d_map));
+    else
+      CK(orth_perop<T>(q, s));
     std::swap(Qa, Qb);
+    return USBS_OK;
+  }
+
+  int plan_qr() {
+    qr_mode = 0;
+    if (batch_member || tuning || p.exec_mode != USBS_EXEC_PERSISTENT || cache_bytes == 0 ||
+        nb != lz.ntiles || !orth_persistent_fits<T>(qr, c->device))
+      return USBS_OK;
+    if (!(d_qr = alloc<LzInst<T>>(2))) return USBS_ERR_OOM;
+    LzInst<T> h[2] = {qr, qr};
+    h[0].Q = Qa;
+    h[1].Q = Qb;
+    h[0].nblocks = h[1].nblocks = lz.ntiles;
+    // the error word starts clear (a reused buffer holds old contents); the
+    // per-call resets leave it alone from here on
+    if (hipMemcpy(d_qr, h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemsetAsync(qr.ctrl, 0, sizeof(int) * LZ_NCTRL, s) != hipSuccess)
+      return set_err(c, USBS_ERR_HIP, "upload of the QR descriptors failed");
+    qr_buf0 = Qa;
+    qr_mode = 1;
     return USBS_OK;
   }
 
@@ -537,6 +575,8 @@ struct Solver {
     if (hipMemcpyAsync(rec_h, sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s) != hipSuccess ||
         (p.exec_mode == USBS_EXEC_PERSISTENT &&
          hipMemcpyAsync(ctrl_h, lz.ctrl, sizeof(int) * LZ_NCTRL, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        (qr_mode == 1 &&
+         hipMemcpyAsync(ctrl_h + LZ_NCTRL, qr.ctrl + LZ_ERROR, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) ||
         hipEventRecord(ev_rec, s) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "reading the scalar block");
     return USBS_OK;
@@ -551,6 +591,7 @@ struct Solver {
     if (p.exec_mode == USBS_EXEC_PERSISTENT) {
       const int* ctrl = ctrl_h;
       if (ctrl[LZ_ERROR]) return set_err(c, USBS_ERR_TIMEOUT, "persistent Lanczos barrier wait expired");
+      if (qr_mode == 1 && ctrl[LZ_NCTRL]) return set_err(c, USBS_ERR_TIMEOUT, "persistent QR barrier wait expired");
       lz_restarts = ctrl[LZ_RESTART];
       lz_hint = lz_restarts > 1 ? lz_restarts : 1;
     }
@@ -908,6 +949,7 @@ static int run_batch(const std::vector<Ctx*>& cs, const std::vector<usbs_params>
     rc = R[q]->prepare(cs[q], ps[q], it.y0, it.z0, it.tr_X0, it.primal_obj0, it.Omega, it.P0,
                        it.q0, it.log, it.log_cap);
     if (rc) return rc;
+    R[q]->S.batch_member = true;
     if ((rc = R[q]->assemble_initial())) return rc;
   }
   failed = -1;

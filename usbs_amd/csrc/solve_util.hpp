@@ -46,13 +46,14 @@ __global__ void k_cast(long long count, const TS* __restrict__ src, TD* __restri
   if (i < count) dst[i] = (TD)src[i];
 }
 
-// Lanczos per-call reset: Q[:, 0] = q0, alpha = beta = 0, control and barrier 0.
+// Lanczos per-call reset: Q[:, 0] = q0, alpha = beta = 0, control and barrier 0
+// (keep_error: the barrier error word stays set across calls).
 template <typename T>
-__global__ void k_lz_start(LzInst<T> L, const double* q0) {
+__global__ void k_lz_start(LzInst<T> L, const double* q0, int keep_error = 0) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < L.n) L.Q[i] = (T)q0[i];
   if (i < L.m) { L.alpha[i] = 0.0; L.beta[i] = 0.0; }
-  if (i < LZ_NCTRL) L.ctrl[i] = 0;
+  if (i < LZ_NCTRL && !(keep_error && i == LZ_ERROR)) L.ctrl[i] = 0;
   for (long long x = i; x < (long long)LZ_BAR_STRIDE * L.nbar; x += (long long)gridDim.x * blockDim.x) L.bar[x] = 0u;
   for (long long x = i; x < 2 * L.nbar; x += (long long)gridDim.x * blockDim.x) L.lstat[x] = 0;
 }
