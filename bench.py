@@ -311,9 +311,10 @@ def batch_bench(args):
     form one after another, which separates the gain of batching from the
     gain of precision."""
     import threading
-    # one hardware queue per solve in flight (HIP reads this when it starts;
-    # nothing here has touched the device yet)
-    os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, max(4, args.batch)))
+    # a hardware queue per stream (two per solve in flight, and a few
+    # spare); HIP reads this when it starts and nothing here has touched the
+    # device yet
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(4, 2 * args.batch + 8)))
     from usbs_amd import _lib
     from usbs_amd.usbs import upload_problem
     B, K, W0 = args.batch, args.steps, args.warmup
@@ -690,11 +691,12 @@ def main():
         kron_bench(args)
         return
     if args.workload == "torus" and not args.throughput_only:
-        # 16 hardware queues (HIP's default is 4): the batched legs run two
-        # streams per solve, and streams that share a queue serialise (eight
-        # solves in lockstep: 235 it/s with 4 queues, 375 with 16).  Set
-        # before the first HIP call of the process.
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+        # 24 hardware queues (HIP's default is 4): the lockstep leg runs two
+        # streams per solve and a batch stream (17 for eight solves), and
+        # streams that share a queue serialise (eight solves in lockstep:
+        # 235 it/s with 4 queues, 471 with 16, 519 with 24).  Set before the
+        # first HIP call of the process.
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "24")
         gap_bench(args)
         return
 
