@@ -492,6 +492,7 @@ def gap_batch_leg(B, target, max_iters, max_time):
     wall_s = sum(x[0] for x in seq)
     return {"problems": B, "target_gap": target, "rho": GAP_RHO,
             "workload": "torus 100x200, w in {-1,+1}, seeds 0..%d" % (B - 1),
+            "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
             "mixed_in_flight": {"wall_s": round(wall_b, 3), "iterations": [r[2] for r in res],
                                 "all_reached": all(r[3] for r in res), "exec_mode": "perop"},
             "mixed_lockstep": lock,
@@ -696,7 +697,7 @@ def main():
         # streams that share a queue serialise (eight solves in lockstep:
         # 235 it/s with 4 queues, 471 with 16, 519 with 24).  Set before the
         # first HIP call of the process.
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "24")
+        os.environ["GPU_MAX_HW_QUEUES"] = "24"
         gap_bench(args)
         return
 

@@ -265,106 +265,129 @@ __device__ __forceinline__ double symkron(const IpmLds& L, int k, int a, int b) 
 }
 
 
-// LDL^T (no pivoting) of the SPD Newton block A (D x D row-major in LDS), in
-// place: on return the strict lower triangle holds the unit factor L and
-// rD[k] = 1 / D_k.  Panels of 8 columns: wavefront 0 factors the panel in
-// registers (a lane per row, the pivot and the panel rows broadcast with
-// readlane), then every thread applies the panel's updates to the trailing
-// lower triangle, column by column in order, so each entry sees exactly the
-// right-looking sequence A[i][j] -= (A[i][k] / D_k) A[j][k], k ascending.
-// Two barriers per panel instead of one per column.  A zero pivot yields
-// inf/NaN, as an LU solve of a singular matrix does.  Lp: D x 8 doubles of
-// LDS (the panel's scaled column values).
+// LDL^T (no pivoting) of the SPD Newton block A (D x D row-major in LDS,
+// D <= 66), in place: on return the strict lower triangle holds the unit
+// factor L and rD[k] = 1 / D_k.  The rows 0..63 are held in registers by
+// column slots: thread (row i = t mod 64, wavefront w) holds A[i][j] for the
+// columns j = 4 s + w < 64, as a window that drops its first slot every
+// four steps.  For D > 64 every thread also holds A[64][i], A[65][i] and
+// the 2 x 2 corner (the same values in each wavefront).  Step k: the owner
+// wavefront of column k has written it, and 1 / D_k, into an LDS slot (two
+// slots, alternating); after one workgroup barrier every thread reads them,
+// stores l_i = A[i][k] / D_k as L[i][k] (owner wavefront), updates column
+// k + 1 first and, if it owns it, writes that column and its pivot's
+// reciprocal into the other slot, then applies A[i][j] -= l_i A[j][k] to its
+// other columns.  Each entry thus sees the right-looking sequence, k
+// ascending, with one barrier per column.  Entries above the diagonal or
+// already final are updated too and never read.  A zero pivot yields
+// inf/NaN, as an LU solve of a singular matrix does.  Lp: at least
+// 2 (D + 1) doubles of LDS.
 template <int D>
 __device__ __forceinline__ void ipm_ldlt(double* A, double* rD, double* Lp) {
-  constexpr int PW = 8;
-  constexpr int NC = (D + 15) / 16;
-  const int t = threadIdx.x, tr = t >> 4, tc = t & 15, lane = t & 63, w = t >> 6;
-  for (int k0 = 0; k0 < D - 1; k0 += PW) {
-    const int pw = min(PW, D - 1 - k0);
-    if (w == 0) {
-      const int r0 = k0 + lane, r1 = k0 + 64 + lane;
-      const int r0c = r0 < D ? r0 : D - 1, r1c = r1 < D ? r1 : D - 1;
-      double p0[PW], p1[PW];
+  static_assert(D >= 1 && D <= 66, "ipm_ldlt: D <= 66");
+  constexpr bool XR = D > 64;              // rows 64, 65 (D = 66)
+  constexpr int RM = D < 64 ? D : 64;      // rows and columns of the slot layout
+  constexpr int NS = (RM + 3) / 4;         // column slots per thread
+  constexpr int SL = D + 1;                // LDS slot: column k (D entries), then 1 / D_k
+  const int t = threadIdx.x, i = t & 63, w = t >> 6;
+  const int ic = i < RM ? i : RM - 1;
+  double a[NS];  // a[u] = A[i][4 (q + u) + w] in step group q
 #pragma unroll
-      for (int c = 0; c < PW; ++c) {
-        const int col = k0 + c < D ? k0 + c : D - 1;
-        const double v0 = A[r0c * D + col], v1 = A[r1c * D + col];
-        p0[c] = (c < pw && r0 < D && k0 + c <= r0) ? v0 : 0.0;
-        p1[c] = (c < pw && r1 < D) ? v1 : 0.0;
+  for (int u = 0; u < NS; ++u) {
+    const int j = 4 * u + w, jc = j < RM ? j : RM - 1;
+    a[u] = A[ic * D + jc];
+  }
+  double e0 = 0.0, e1 = 0.0, g00 = 0.0, g10 = 0.0, g11 = 0.0;
+  if constexpr (XR) {
+    e0 = A[64 * D + ic];
+    e1 = A[65 * D + ic];
+    g00 = A[64 * D + 64];
+    g10 = A[65 * D + 64];
+    g11 = A[65 * D + 65];
+  }
+  if (w == 0) {  // column 0 and 1 / D_0 into slot 0
+    if (i < RM) Lp[i] = a[0];
+    if (i == 0) {
+      Lp[D] = 1.0 / a[0];
+      if (D == 1) rD[0] = Lp[D];
+      if (XR) {
+        Lp[64] = e0;  // lane 0: A[64][0], A[65][0]
+        Lp[65] = e1;
       }
+    }
+  }
+  for (int q = 0; 4 * q < D - 1; ++q) {
 #pragma unroll
-      for (int c = 0; c < PW; ++c) {
-        if (c < pw) {
-          const double rdk = 1.0 / readlane_d(p0[c], c);
-          const double l0 = p0[c] * rdk, l1 = p1[c] * rdk;
-          if (r0 > k0 + c && r0 < D) Lp[r0 * PW + c] = l0;
-          if (r1 < D) Lp[r1 * PW + c] = l1;
-#pragma unroll
-          for (int j = c + 1; j < PW; ++j) {
-            if (j < pw) {
-              const double ajc = readlane_d(p0[c], j);  // A[k0 + j][k0 + c]
-              if (r0 >= k0 + j) p0[j] = p0[j] - l0 * ajc;
-              p1[j] = p1[j] - l1 * ajc;
-            }
+    for (int r = 0; r < 4; ++r) {
+      const int k = 4 * q + r;
+      if (k < D - 1) {
+        const double* cb = Lp + (r & 1) * SL;
+        double* cn = Lp + ((r + 1) & 1) * SL;
+        __syncthreads();
+        const double rdk = cb[D];
+        const double ck = cb[ic];  // A[i][k]
+        const double li = ck * rdk;
+        double l64 = 0.0, l65 = 0.0, c64 = 0.0, c65 = 0.0;
+        if constexpr (XR) {
+          c64 = cb[64];
+          c65 = cb[65];
+          l64 = c64 * rdk;
+          l65 = c65 * rdk;
+        }
+        if (t == 0) {
+          rD[k] = rdk;
+          if (XR) {
+            if (k < 64) A[64 * D + k] = l64;
+            A[65 * D + k] = l65;
           }
         }
-      }
+        if (k < RM && w == r && i > k && i < RM) A[i * D + k] = li;
+        if constexpr (XR) {
+          e0 = fma(-l64, ck, e0);  // A[64][i] -= l_64 A[i][k]
+          e1 = fma(-l65, ck, e1);
+          g00 = fma(-l64, c64, g00);
+          g10 = fma(-l65, c64, g10);
+          g11 = fma(-l65, c65, g11);
+        }
+        // column k + 1 first, by its owner wavefront, into the next slot
+        const int u1 = (r + 1) / 4;  // 0 or 1 (static once r is unrolled)
+        const bool own = w == ((r + 1) & 3);
+        const bool ahead = k + 1 < RM && own;
+        if (ahead) {
+          const double v1 = fma(-li, cb[k + 1], a[u1]);
+          a[u1] = v1;
+          if (k + 2 <= D - 1) {
+            if (i < RM) cn[i] = v1;
+            if (i == k + 1) cn[D] = 1.0 / v1;
+            if constexpr (XR) {
+              const double n64 = readlane_d(e0, k + 1), n65 = readlane_d(e1, k + 1);
+              if (i == 0) {
+                cn[64] = n64;
+                cn[65] = n65;
+              }
+            }
+          } else if (i == k + 1) {
+            rD[k + 1] = 1.0 / v1;  // the last pivot
+          }
+        } else if (XR && k + 1 == 64 && own && i == 0) {  // the corner's first column
+          cn[64] = g00;
+          cn[65] = g10;
+          cn[D] = 1.0 / g00;
+        }
 #pragma unroll
-      for (int c = 0; c < PW; ++c) {
-        if (c < pw) {
-          if (r0 < D && r0 >= k0 + c) A[r0 * D + k0 + c] = p0[c];
-          if (r1 < D) A[r1 * D + k0 + c] = p1[c];
+        for (int u = 0; u < NS; ++u) {
+          const int j = 4 * (q + u) + w;
+          const double cj = cb[j < RM ? j : RM - 1];
+          if (!(u == u1 && ahead)) a[u] = fma(-li, cj, a[u]);
         }
       }
     }
-    __syncthreads();
-    const int kt = k0 + pw;
-    if (kt < D) {
-      // Loads are unconditional from clamped in-bounds addresses and masked
-      // by selects (no branch per load); a masked term is 0 * 0 and leaves v
-      // unchanged, so each entry gets its updates k ascending as above.
-      double aj[NC][PW];
 #pragma unroll
-      for (int cc = 0; cc < NC; ++cc) {
-        const int j = kt + tc + 16 * cc;
-        const int jc = j < D ? j : D - 1;
-#pragma unroll
-        for (int c = 0; c < PW; ++c) {
-          const int col = k0 + c < D ? k0 + c : D - 1;
-          const double a = A[jc * D + col];
-          aj[cc][c] = (c < pw) ? a : 0.0;
-        }
-      }
-      for (int i = kt + tr; i < D; i += 16) {
-        double li[PW], v[NC];
-#pragma unroll
-        for (int c = 0; c < PW; ++c) {
-          const double l = Lp[i * PW + c];
-          li[c] = (c < pw) ? l : 0.0;
-        }
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc) {
-          const int j = kt + tc + 16 * cc;
-          v[cc] = A[i * D + (j < D ? j : D - 1)];
-        }
-#pragma unroll
-        for (int c = 0; c < PW; ++c)
-#pragma unroll
-          for (int cc = 0; cc < NC; ++cc) v[cc] = v[cc] - li[c] * aj[cc][c];
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc) {
-          const int j = kt + tc + 16 * cc;
-          if (j <= i) A[i * D + j] = v[cc];
-        }
-      }
-    }
-    __syncthreads();
+    for (int u = 0; u + 1 < NS; ++u) a[u] = a[u + 1];
   }
-  for (int x = t; x < D; x += blockDim.x) rD[x] = 1.0 / A[x * D + x];
-  __syncthreads();
-  for (int i = tr; i < D; i += 16)
-    for (int j = tc; j < i; j += 16) A[i * D + j] *= rD[j];
+  if constexpr (XR) {
+    if (t == 0) rD[65] = 1.0 / g11;
+  }
   __syncthreads();
 }
 
