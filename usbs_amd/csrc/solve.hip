@@ -531,9 +531,10 @@ struct Solver {
       const int rc = plan_qr();
       if (rc) return rc;
     }
-    // resets control words only (column 0 set below); the persistent QR's
-    // barrier error stays set for the records to report
-    k_lz_start<T><<<1, 64, 0, s>>>(q, q0, qr_mode);
+    // resets the control words and barrier counters (column 0 is set
+    // below); the persistent QR's barrier error stays set for the records to
+    // report
+    k_lz_start<T><<<(unsigned)((LZ_BAR_STRIDE * q.nbar + 255) / 256), 256, 0, s>>>(q, q0, qr_mode);
     CK(fill_T(Qb, ld));
     if (qr_mode == 1)
       CK(orth_persistent<T>(d_qr + (Qb == qr_buf0 ? 0 : 1), lz.ntiles, k, s, d_map));
