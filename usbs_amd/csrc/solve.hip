@@ -38,6 +38,14 @@ __global__ void k_set_by(double* sc, const double* by4) {
   if (threadIdx.x == 0) sc[SC_BY] = by4[1];
 }
 
+// the early rounding's two scalars (its own block) into the scalar block
+__global__ void k_copy_round(double* sc, const double* rsc) {
+  if (threadIdx.x == 0) {
+    sc[SC_NYS_RHO] = rsc[SC_NYS_RHO];
+    sc[SC_CB] = rsc[SC_CB];
+  }
+}
+
 __global__ void k_copy1(double* dst, const double* src) {
   if (threadIdx.x == 0) *dst = *src;
 }
@@ -137,10 +145,24 @@ struct Solver {
   double* rec_h = nullptr;
   int* ctrl_h = nullptr;
   hipEvent_t ev_rec = nullptr;
+  // max-cut rounding launched early: when the driver sets round_early
+  // before body(), body_pre forks the rounding of the coming condition check
+  // onto the side stream right after the sketch update (its inputs, the
+  // sketch and Omega, do not change again in the iteration), with its own
+  // partial-sum buffer; it overlaps the candidate Lanczos call, and
+  // cond_launch waits for ev_round instead of running it.
+  // Its scalars go to rsc (a block of SC_COUNT doubles), copied into sc when
+  // the check runs, so the two streams never write the same block.
+  bool round_early = false, round_done = false;
+  double* partial_r = nullptr;
+  double* rsc = nullptr;
+  hipEvent_t ev_rfork = nullptr, ev_round = nullptr;
 
   ~Solver() {
     if (s2) (void)hipStreamSynchronize(s2);  // the context's side stream: kept
     if (ev_rec) { (void)hipEventSynchronize(ev_rec); (void)hipEventDestroy(ev_rec); }
+    if (ev_rfork) (void)hipEventDestroy(ev_rfork);
+    if (ev_round) (void)hipEventDestroy(ev_round);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
@@ -216,7 +238,8 @@ struct Solver {
         std::max((size_t)P->ntiles * (size_t)std::max(std::max(k * k, 2 * rr * rr), std::max(k * rr, 16)),
                  (size_t)P->mntiles * 4),
         std::max((size_t)((m + qt - 1) / qt) * d * d, (size_t)objective_ntiles(*P) * 16));
-    if (!(partial = alloc<double>(pw)) || !(dpart = alloc<double>((size_t)ntl * rr)) ||
+    if (!(partial = alloc<double>(pw)) || !(partial_r = alloc<double>(pw)) || !(rsc = alloc<double>(SC_COUNT)) ||
+        !(dpart = alloc<double>((size_t)ntl * rr)) ||
         !(ipart = alloc<long long>((size_t)ntl * rr)))
       return USBS_ERR_OOM;
     if (!(Gq = alloc<double>(d * d)) || !(Gz = alloc<double>(k * k)) || !(Gh = alloc<double>(k * k)) ||
@@ -249,7 +272,9 @@ struct Solver {
     if ((!c->side && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) ||
         hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_rec, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ev_rec, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_rfork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_round, hipEventDisableTiming) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
     s2 = c->side;
     if ((!c->rec_h && hipHostMalloc((void**)&c->rec_h, sizeof(double) * SC_COUNT) != hipSuccess) ||
@@ -419,22 +444,35 @@ struct Solver {
   int project(double* out) { return project(out, s); }
 
   int rounding() {
+    if (round_done) {  // launched early on the side stream
+      round_done = false;
+      CK((int)hipStreamWaitEvent(s, ev_round, 0));
+      k_copy_round<<<1, 64, 0, s>>>(sc, rsc);
+      CK((int)hipGetLastError());
+      return USBS_OK;
+    }
+    return rounding_on(s, partial, sc);
+  }
+  bool rounds_early() const { return r > 0 && p.rounding == 1; }
+  // the rounding on stream st with partial-sum buffer part, its scalars
+  // (SC_NYS_RHO, SC_CB) written into the block sco
+  int rounding_on(hipStream_t st, double* part, double* sco) {
     if (r <= 0 || !p.rounding) return USBS_OK;
-    CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, P_(), ld, r, PO, ld, 2 * r, partial, PtPO, s)));
-    CK(launch_nystrom_core(n, r, PtPO, OtO, Gn, lam, sc, s));
+    CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, P_(), ld, r, PO, ld, 2 * r, part, PtPO, st)));
+    CK(launch_nystrom_core(n, r, PtPO, OtO, Gn, lam, sco, st));
     if (p.rounding == 2) {
       // QAP: the factor on the device, the assignments on the host
-      CK(launch_nystrom_E<T>(n, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, dE, s));
-      if (hipMemcpyAsync(hE.data(), dE, sizeof(double) * hE.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
+      CK(launch_nystrom_E<T>(n, r, P_(), Om(), ld, Gn, sco + SC_NYS_RHO, dE, st));
+      if (hipMemcpyAsync(hE.data(), dE, sizeof(double) * hE.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
         return set_err(c, USBS_ERR_HIP, "reading the Nystrom factor");
       const int rc = qap_round_host(c->qap_l, r, hE.data(), c->qap_W.data(), c->qap_D.data(), nullptr,
                                     nullptr, &qap_cb);
       if (rc) return set_err(c, rc, "QAP rounding failed");
       return USBS_OK;
     }
-    CK(launch_round<T>(*P, cutw, r, P_(), Om(), ld, Gn, sc + SC_NYS_RHO, sg, ipart, dpart, cut, s));
-    k_copy1<<<1, 64, 0, s>>>(sc + SC_CB, cut + r);
+    CK(launch_round<T>(*P, cutw, r, P_(), Om(), ld, Gn, sco + SC_NYS_RHO, sg, ipart, dpart, cut, st));
+    k_copy1<<<1, 64, 0, st>>>(sco + SC_CB, cut + r);
     CK((int)hipGetLastError());
     return USBS_OK;
   }
@@ -491,6 +529,15 @@ struct Solver {
                                    sc + SC_BYC, s));
     if (r > 0) CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, F, ld, k, Om(), ld, r, partial, FO, s)));
     CK(launch_sketch_update<T>(n, k, kc, r, F, ld, FO, sc + SC_ETA, Pbar, P_(), ld, s));
+    const bool early = round_early && rounds_early();
+    round_early = false;
+    if (early) {
+      CK((int)hipEventRecord(ev_rfork, s));
+      CK((int)hipStreamWaitEvent(s2, ev_rfork, 0));
+      if (rounding_on(s2, partial_r, rsc)) return USBS_ERR_HIP;
+      CK((int)hipEventRecord(ev_round, s2));
+      round_done = true;
+    }
     CK(launch_objective<T>(*P, k, kc, F, ld, partial, sc + SC_OBJF, s));
     // --- candidate eigenvectors (solver/usbs.py:588-599)
     CK(assemble_op(-1.0, ycand, 1.0, nullptr, 0.0, nullptr, 0.0, sizeof(T) == 4));
@@ -885,6 +932,7 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
     for (long long steps : geometric_schedule(p.max_iters, p.cond_exp_base)) {
       if (!R.go) break;
       for (long long i = 0; i < steps; ++i) {
+        S.round_early = i == steps - 1;  // the check follows this iteration
         rc = S.body();
         if (rc) return rc;
         ++R.iters;
@@ -894,6 +942,7 @@ static int run_solve(Ctx* c, const usbs_params& p, const double* y0, const doubl
     }
   } else {
     for (long long it = 0; it < p.max_iters && R.go; ++it) {
+      S.round_early = true;  // a check follows every iteration
       rc = S.body();
       if (rc) return rc;
       ++R.iters;
@@ -1054,6 +1103,7 @@ static int run_batch(const std::vector<Ctx*>& cs, const std::vector<usbs_params>
     }
     for (int q : act) {
       failed = q;
+      R[q]->S.round_early = left[q] == 1;  // its check is due after this iteration
       if ((rc = R[q]->S.body_pre())) return rc;
     }
     if ((rc = batched_lanczos(act, lb, R[act[0]]->S.kc))) return rc;
