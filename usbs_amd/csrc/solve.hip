@@ -146,8 +146,8 @@ struct Solver {
   int* ctrl_h = nullptr;
   hipEvent_t ev_rec = nullptr;
   // max-cut rounding launched early: when the driver sets round_early
-  // before body(), body_pre forks the rounding of the coming condition check
-  // onto the side stream right after the sketch update (its inputs, the
+  // before body(), body_pre queues the rounding of the coming condition
+  // check on the side stream right after the sketch update (its inputs, the
   // sketch and Omega, do not change again in the iteration), with its own
   // partial-sum buffer; it overlaps the candidate Lanczos call, and
   // cond_launch waits for ev_round instead of running it.
@@ -156,12 +156,11 @@ struct Solver {
   bool round_early = false, round_done = false;
   double* partial_r = nullptr;
   double* rsc = nullptr;
-  hipEvent_t ev_rfork = nullptr, ev_round = nullptr;
+  hipEvent_t ev_round = nullptr;
 
   ~Solver() {
     if (s2) (void)hipStreamSynchronize(s2);  // the context's side stream: kept
     if (ev_rec) { (void)hipEventSynchronize(ev_rec); (void)hipEventDestroy(ev_rec); }
-    if (ev_rfork) (void)hipEventDestroy(ev_rfork);
     if (ev_round) (void)hipEventDestroy(ev_round);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
@@ -273,7 +272,6 @@ struct Solver {
         hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_rec, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_rfork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_round, hipEventDisableTiming) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
     s2 = c->side;
@@ -527,24 +525,26 @@ struct Solver {
     CK(launch_constraint_update<T>(*P, k, kc, F, ld, sc + SC_ETA, zbar, y,
                                    P->has_ineq ? ups : nullptr, rho, z, zbn, ycand, partial,
                                    sc + SC_BYC, s));
-    if (r > 0) CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, F, ld, k, Om(), ld, r, partial, FO, s)));
-    CK(launch_sketch_update<T>(n, k, kc, r, F, ld, FO, sc + SC_ETA, Pbar, P_(), ld, s));
+    // The candidate Lanczos call needs only ycand.  The sketch update, the
+    // objective, a due check's rounding and the spectral lower bound go to
+    // the side stream and overlap it; body_post waits for them (ev_join).
+    // Nothing on the main stream touches F, the sketch, `partial` or the
+    // scalar block until then.
+    CK((int)hipEventRecord(ev_fork, s));
+    CK((int)hipStreamWaitEvent(s2, ev_fork, 0));
+    if (r > 0) CK((launch_tsgram<T, T>(n, P->tile, P->ntiles, F, ld, k, Om(), ld, r, partial, FO, s2)));
+    CK(launch_sketch_update<T>(n, k, kc, r, F, ld, FO, sc + SC_ETA, Pbar, P_(), ld, s2));
+    CK(launch_objective<T>(*P, k, kc, F, ld, partial, sc + SC_OBJF, s2));
     const bool early = round_early && rounds_early();
     round_early = false;
     if (early) {
-      CK((int)hipEventRecord(ev_rfork, s));
-      CK((int)hipStreamWaitEvent(s2, ev_rfork, 0));
       if (rounding_on(s2, partial_r, rsc)) return USBS_ERR_HIP;
       CK((int)hipEventRecord(ev_round, s2));
       round_done = true;
     }
-    CK(launch_objective<T>(*P, k, kc, F, ld, partial, sc + SC_OBJF, s));
     // --- candidate eigenvectors (solver/usbs.py:588-599)
     CK(assemble_op(-1.0, ycand, 1.0, nullptr, 0.0, nullptr, 0.0, sizeof(T) == 4));
-    // --- spectral lower bound with the old V (solver/usbs.py:606-621) on the
-    // side stream, overlapping the Lanczos call (it needs only ycand)
-    CK((int)hipEventRecord(ev_fork, s));
-    CK((int)hipStreamWaitEvent(s2, ev_fork, 0));
+    // --- spectral lower bound with the old V (solver/usbs.py:606-621)
     if (project(Glb, s2)) return USBS_ERR_HIP;
     CK(launch_ipm_lb(k, p.subprob_max_iters, tu, p.subprob_eps, Glb, sc, s2));
     CK((int)hipEventRecord(ev_join, s2));
