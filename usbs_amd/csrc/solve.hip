@@ -136,6 +136,10 @@ struct Solver {
   int tev_launched = 0;   // restarts launched by the last timed persistent call
   hipStream_t s2 = nullptr;          // side stream: the lower-bound IPM overlaps the Lanczos call
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // recorded on the main stream after the candidate operator -C + A*(ycand)
+  // is assembled into op64; the side stream's lower-bound projection reads
+  // op64 and waits for it
+  hipEvent_t ev_asm = nullptr;
   std::vector<hipEvent_t> tev;       // timing: 2 events per persistent launch
   bool tev_pending = false;
   std::vector<int> lstat_h;
@@ -164,6 +168,7 @@ struct Solver {
     if (ev_round) (void)hipEventDestroy(ev_round);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_asm) (void)hipEventDestroy(ev_asm);
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
     if (ptm.e0) (void)hipEventDestroy(ptm.e0);
     for (auto& g : graphs) lanczos_graphs_clear(&g);
@@ -271,6 +276,7 @@ struct Solver {
     if ((!c->side && hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) ||
         hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_asm, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_rec, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_round, hipEventDisableTiming) != hipSuccess)
       return set_err(c, USBS_ERR_HIP, "stream/event creation failed");
@@ -544,7 +550,11 @@ struct Solver {
     }
     // --- candidate eigenvectors (solver/usbs.py:588-599)
     CK(assemble_op(-1.0, ycand, 1.0, nullptr, 0.0, nullptr, 0.0, sizeof(T) == 4));
-    // --- spectral lower bound with the old V (solver/usbs.py:606-621)
+    CK((int)hipEventRecord(ev_asm, s));
+    // --- spectral lower bound with the old V (solver/usbs.py:606-621): the
+    // projection reads the candidate operator just assembled on the main
+    // stream
+    CK((int)hipStreamWaitEvent(s2, ev_asm, 0));
     if (project(Glb, s2)) return USBS_ERR_HIP;
     CK(launch_ipm_lb(k, p.subprob_max_iters, tu, p.subprob_eps, Glb, sc, s2));
     CK((int)hipEventRecord(ev_join, s2));
